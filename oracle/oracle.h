@@ -1,0 +1,116 @@
+/*
+ * oracle.h -- CPU restatement of libigl's generalized-winding-number path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in libigl_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Every function restates the reference algorithm step by step (same tree
+ * topology, same operation order, no FMA contraction when built with
+ * -ffp-contract=off) and cites the reference file:line it follows.
+ *
+ * Pinning (what checks each function against something independent of it):
+ *   oracle_solid_angle / oracle_winding_number_bf
+ *       closed forms: closed mesh inside = 1 / outside = 0, single-tet signed
+ *       distance case (tests/include/igl/signed_distance.cpp:4-44), orientation
+ *       flip w -> -w, hemisphere cap values; and the compiled reference
+ *       UTsignedSolidAngleTri in double (oracle/_ref).
+ *   oracle_soup_*
+ *       bit-exact against the compiled reference UT_SolidAngle (oracle/_ref):
+ *       node arrays, BoxData and per-query Omega; plus closed-mesh invariants.
+ *   oracle_octree / oracle_cloud_*
+ *       the reference golden values of tests/include/igl/fast_winding_number.cpp:46-52
+ *       (one-point cloud, 1e-15), the beta<=0 brute-force branch
+ *       (fast_winding_number.cpp:300-310) at large beta, hand-checked tiny
+ *       octrees (octree.cpp rules) and the closed-form Taylor terms.
+ */
+#ifndef GWN_ORACLE_H
+#define GWN_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- scheduler: igl::parallel_for (parallel_for.h:140-204) ---- */
+typedef void (*oracle_range_fn)(int64_t begin, int64_t end, void* ctx);
+/* Runs fn over [0,n) in contiguous slices of round((n+1)/T) on T std::threads
+ * (T = nthreads, or IGL_NUM_THREADS / hardware concurrency when nthreads<=0);
+ * serial when n < min_parallel.  Returns the thread count used. */
+int oracle_parallel_for(int64_t n, oracle_range_fn fn, void* ctx, int64_t min_parallel, int nthreads);
+int oracle_default_num_threads(void);
+
+/* ---- exact FP64 (solid_angle.cpp:10-55, winding_number.cpp:62-89) ---- */
+double oracle_solid_angle(const double* a, const double* b, const double* c, const double* p);
+/* brute-force sum over faces in F order; V nv x 3, F nf x 3 row-major; W[no] */
+void oracle_winding_number_bf(const double* V, int64_t nv, const int32_t* F, int64_t nf,
+                              const double* O, int64_t no, double* W, int nthreads);
+/* FP64 restatement of UTsignedSolidAngleTri (FastWindingNumberForSoups.h:6070-6110) */
+double oracle_solid_angle_tri_f64(const double* a, const double* b, const double* c, const double* q);
+
+/* ---- triangle soup: UT_SolidAngle<float,float> over UT::BVH<4> ---- */
+typedef struct oracle_soup oracle_soup;
+/* UT_SolidAngle::init (FastWindingNumberForSoups.h:6543-7133) */
+oracle_soup* oracle_soup_build(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order);
+void oracle_soup_free(oracle_soup* s);
+int32_t oracle_soup_num_nodes(const oracle_soup* s);
+/* node child arrays, 4 uint32 per node (FastWindingNumberForSoups.h:4217-4232) */
+void oracle_soup_nodes(const oracle_soup* s, uint32_t* out);
+/* BoxData, 92 floats per node in the reference struct order (6477-6521) */
+void oracle_soup_boxdata(const oracle_soup* s, float* out);
+/* computeSolidAngle (7148-7284) for one query; returns Omega (not /4pi) */
+float oracle_soup_solid_angle(const oracle_soup* s, const float* q, float accuracy_scale);
+/* fast_winding_number(bvh, accuracy_scale, Q, W) (fast_winding_number.cpp:418-437):
+ * Q nq x 3 double narrowed to float, W = Omega / (4 pi) in double. */
+void oracle_soup_query(const oracle_soup* s, const double* Q, int64_t nq, float accuracy_scale,
+                       double* W, int nthreads);
+/* Counted FLOPs / visits for the roofline: per query returns visits (node pre()
+ * calls), accepted far-field child slots and exact triangles. */
+void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale,
+                       int64_t* visits, int64_t* far_accepted, int64_t* exact_tris);
+
+/* ---- point cloud: igl::octree + fast_winding_number(P,N,A,...) ---- */
+/* octree (octree.cpp:7-169).  Cells are numbered exactly as the reference
+ * numbers them.  Outputs are malloc'ed and owned by the caller (free()).
+ *   CH: ncells x 8 int32 (-1 for leaves)
+ *   CN: ncells x 3, W: ncells (cell centres / widths, in REAL)
+ *   cell_start/cell_count + point_list: the points of each cell in ascending
+ *   index order (point_indices in the reference), as CSR. */
+typedef struct {
+  int32_t ncells;
+  int32_t* CH;
+  void* CN;
+  void* W;
+  int64_t* cell_start;
+  int32_t* cell_count;
+  int32_t* point_list;
+  int64_t npoint_list;
+} oracle_octree_t;
+
+int oracle_octree_f64(const double* P, int32_t n, oracle_octree_t* out);
+int oracle_octree_f32(const float* P, int32_t n, oracle_octree_t* out);
+void oracle_octree_release(oracle_octree_t* t);
+
+/* precompute (fast_winding_number.cpp:8-124): CM ncells x 3, R ncells,
+ * EC ncells x num_terms (3 / 12 / 39 for order 0 / 1 / 2) */
+int oracle_cloud_precompute_f64(const double* P, const double* N, const double* A,
+                                const oracle_octree_t* t, int order,
+                                double* CM, double* R, double* EC);
+int oracle_cloud_precompute_f32(const float* P, const float* N, const float* A,
+                                const oracle_octree_t* t, int order,
+                                float* CM, float* R, float* EC);
+/* evaluation (fast_winding_number.cpp:126-311); beta <= 0 is the brute-force
+ * branch (300-310). */
+void oracle_cloud_eval_f64(const double* P, const double* N, const double* A,
+                           const oracle_octree_t* t, int order,
+                           const double* CM, const double* R, const double* EC,
+                           const double* Q, int64_t nq, double beta, double* WN, int nthreads);
+void oracle_cloud_eval_f32(const float* P, const float* N, const float* A,
+                           const oracle_octree_t* t, int order,
+                           const float* CM, const float* R, const float* EC,
+                           const float* Q, int64_t nq, double beta, float* WN, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,255 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the reference
+build (oracle/_ref/libref_soup.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline.  The product
+(libigl_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_soup.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+
+_lib = None
+_ref = None
+
+
+class OctreeT(C.Structure):
+    _fields_ = [("ncells", C.c_int32), ("CH", C.POINTER(C.c_int32)), ("CN", C.c_void_p), ("W", C.c_void_p),
+                ("cell_start", C.POINTER(C.c_int64)), ("cell_count", C.POINTER(C.c_int32)),
+                ("point_list", C.POINTER(C.c_int32)), ("npoint_list", C.c_int64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        L = C.CDLL(ORACLE_SO)
+        L.oracle_default_num_threads.restype = C.c_int
+        L.oracle_solid_angle.restype = C.c_double
+        L.oracle_solid_angle.argtypes = [_f64p, _f64p, _f64p, _f64p]
+        L.oracle_solid_angle_tri_f64.restype = C.c_double
+        L.oracle_solid_angle_tri_f64.argtypes = [_f64p, _f64p, _f64p, _f64p]
+        L.oracle_winding_number_bf.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, _f64p, C.c_int]
+        L.oracle_soup_build.restype = C.c_void_p
+        L.oracle_soup_build.argtypes = [_f32p, C.c_int32, _i32p, C.c_int32, C.c_int]
+        L.oracle_soup_free.argtypes = [C.c_void_p]
+        L.oracle_soup_num_nodes.restype = C.c_int32
+        L.oracle_soup_num_nodes.argtypes = [C.c_void_p]
+        L.oracle_soup_nodes.argtypes = [C.c_void_p, _u32p]
+        L.oracle_soup_boxdata.argtypes = [C.c_void_p, _f32p]
+        L.oracle_soup_solid_angle.restype = C.c_float
+        L.oracle_soup_solid_angle.argtypes = [C.c_void_p, _f32p, C.c_float]
+        L.oracle_soup_query.argtypes = [C.c_void_p, _f64p, C.c_int64, C.c_float, _f64p, C.c_int]
+        L.oracle_soup_count.argtypes = [C.c_void_p, _f32p, C.c_float, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        for sfx, fp in (("f64", _f64p), ("f32", _f32p)):
+            f = getattr(L, f"oracle_octree_{sfx}")
+            f.restype = C.c_int
+            f.argtypes = [fp, C.c_int32, C.POINTER(OctreeT)]
+            f = getattr(L, f"oracle_cloud_precompute_{sfx}")
+            f.restype = C.c_int
+            f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp]
+            f = getattr(L, f"oracle_cloud_eval_{sfx}")
+            f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
+        L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"reference build missing: {REF_SO}")
+        R = C.CDLL(REF_SO)
+        R.ref_soup_build.restype = C.c_void_p
+        R.ref_soup_build.argtypes = [_f32p, C.c_int, _i32p, C.c_int, C.c_int]
+        R.ref_soup_free.argtypes = [C.c_void_p]
+        R.ref_soup_num_nodes.restype = C.c_int
+        R.ref_soup_num_nodes.argtypes = [C.c_void_p]
+        R.ref_soup_nodes.argtypes = [C.c_void_p, _u32p]
+        R.ref_soup_boxdata.argtypes = [C.c_void_p, _f32p]
+        R.ref_soup_solid_angle.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_float, _f32p]
+        R.ref_soup_query.argtypes = [C.c_void_p, _f64p, C.c_int64, C.c_float, _f64p]
+        R.ref_solid_angle_tri_f64.restype = C.c_double
+        R.ref_solid_angle_tri_f64.argtypes = [_f64p, _f64p, _f64p, _f64p]
+        R.ref_num_threads.restype = C.c_int
+        _ref = R
+    return _ref
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------- exact ----------------
+def winding_number_bf(V, F, O, nthreads=0):
+    V = _c(V, np.float64)
+    F = _c(F, np.int32)
+    O = _c(O, np.float64).reshape(-1, 3)
+    W = np.empty(O.shape[0], np.float64)
+    lib().oracle_winding_number_bf(V, V.shape[0], F, F.shape[0], O, O.shape[0], W, nthreads)
+    return W
+
+
+def solid_angle(a, b, c, p):
+    return lib().oracle_solid_angle(_c(a, np.float64), _c(b, np.float64), _c(c, np.float64), _c(p, np.float64))
+
+
+# ---------------- soup ----------------
+class Soup:
+    """Oracle UT_SolidAngle<float,float> (restated)."""
+
+    def __init__(self, V, F, order=2):
+        self.U = _c(V, np.float32)
+        self.F = _c(F, np.int32)
+        self.h = lib().oracle_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_soup_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        n = lib().oracle_soup_num_nodes(self.h)
+        out = np.empty((n, 4), np.uint32)
+        lib().oracle_soup_nodes(self.h, out)
+        return out
+
+    def boxdata(self):
+        n = lib().oracle_soup_num_nodes(self.h)
+        out = np.empty((n, 92), np.float32)
+        lib().oracle_soup_boxdata(self.h, out)
+        return out
+
+    def omega(self, Q, accuracy_scale=2.0):
+        Q = _c(Q, np.float32).reshape(-1, 3)
+        out = np.empty(Q.shape[0], np.float32)
+        for i in range(Q.shape[0]):
+            out[i] = lib().oracle_soup_solid_angle(self.h, Q[i].copy(), accuracy_scale)
+        return out
+
+    def query(self, Q, accuracy_scale=2.0, nthreads=0):
+        Q = _c(Q, np.float64).reshape(-1, 3)
+        W = np.empty(Q.shape[0], np.float64)
+        lib().oracle_soup_query(self.h, Q, Q.shape[0], accuracy_scale, W, nthreads)
+        return W
+
+    def count(self, q, accuracy_scale=2.0):
+        v, a, e = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().oracle_soup_count(self.h, _c(q, np.float32), accuracy_scale, C.byref(v), C.byref(a), C.byref(e))
+        return v.value, a.value, e.value
+
+
+class RefSoup:
+    """The reference's own UT_SolidAngle (compiled from the reference header)."""
+
+    def __init__(self, V, F, order=2):
+        self.U = _c(V, np.float32)
+        self.F = _c(F, np.int32)
+        self.h = ref().ref_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_soup_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        n = ref().ref_soup_num_nodes(self.h)
+        out = np.empty((n, 4), np.uint32)
+        ref().ref_soup_nodes(self.h, out)
+        return out
+
+    def boxdata(self):
+        n = ref().ref_soup_num_nodes(self.h)
+        out = np.empty((n, 92), np.float32)
+        ref().ref_soup_boxdata(self.h, out)
+        return out
+
+    def omega(self, Q, accuracy_scale=2.0):
+        Q = _c(Q, np.float32).reshape(-1, 3)
+        out = np.empty(Q.shape[0], np.float32)
+        ref().ref_soup_solid_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
+        return out
+
+    def query(self, Q, accuracy_scale=2.0):
+        Q = _c(Q, np.float64).reshape(-1, 3)
+        W = np.empty(Q.shape[0], np.float64)
+        ref().ref_soup_query(self.h, Q, Q.shape[0], accuracy_scale, W)
+        return W
+
+
+# ---------------- cloud ----------------
+class Octree:
+    def __init__(self, P, dtype=np.float64):
+        self.dt = dtype
+        self.sfx = "f64" if dtype == np.float64 else "f32"
+        self.P = _c(P, dtype)
+        self.t = OctreeT()
+        rc = getattr(lib(), f"oracle_octree_{self.sfx}")(self.P, self.P.shape[0], C.byref(self.t))
+        if rc != 0:
+            raise MemoryError("oracle octree failed")
+        n = self.t.ncells
+        ctp = C.c_double if dtype == np.float64 else C.c_float
+        self.CH = np.ctypeslib.as_array(self.t.CH, shape=(n, 8)).copy()
+        self.CN = np.ctypeslib.as_array(C.cast(self.t.CN, C.POINTER(ctp)), shape=(n, 3)).copy()
+        self.W = np.ctypeslib.as_array(C.cast(self.t.W, C.POINTER(ctp)), shape=(n,)).copy()
+        self.cell_start = np.ctypeslib.as_array(self.t.cell_start, shape=(n,)).copy()
+        self.cell_count = np.ctypeslib.as_array(self.t.cell_count, shape=(n,)).copy()
+        m = max(self.t.npoint_list, 1)
+        self.point_list = np.ctypeslib.as_array(self.t.point_list, shape=(m,))[: self.t.npoint_list].copy()
+
+    def point_indices(self, c):
+        s = self.cell_start[c]
+        return self.point_list[s: s + self.cell_count[c]]
+
+    def __del__(self):
+        if getattr(self, "t", None) is not None:
+            lib().oracle_octree_release(C.byref(self.t))
+            self.t = None
+
+
+class Cloud:
+    """Oracle octree + precompute + evaluation (restated)."""
+
+    def __init__(self, P, N, A, order=2, dtype=np.float64):
+        self.dt = dtype
+        self.sfx = "f64" if dtype == np.float64 else "f32"
+        self.P = _c(P, dtype)
+        self.N = _c(N, dtype)
+        self.A = _c(A, dtype).reshape(-1)
+        self.order = order
+        self.oct = Octree(self.P, dtype)
+        n = self.oct.t.ncells
+        nt = {0: 3, 1: 12, 2: 39}[order]
+        self.CM = np.empty((n, 3), dtype)
+        self.R = np.empty(n, dtype)
+        self.EC = np.empty((n, nt), dtype)
+        getattr(lib(), f"oracle_cloud_precompute_{self.sfx}")(self.P, self.N, self.A, C.byref(self.oct.t), order,
+                                                              self.CM, self.R, self.EC)
+
+    def eval(self, Q, beta=2.0, nthreads=0):
+        Q = _c(Q, self.dt).reshape(-1, 3)
+        W = np.empty(Q.shape[0], self.dt)
+        getattr(lib(), f"oracle_cloud_eval_{self.sfx}")(self.P, self.N, self.A, C.byref(self.oct.t), self.order,
+                                                        self.CM, self.R, self.EC, Q, Q.shape[0], float(beta), W,
+                                                        nthreads)
+        return W
