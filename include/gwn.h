@@ -1,0 +1,160 @@
+/*
+ * gwn.h -- C ABI of the MI355X generalized-winding-number library (libgwn.so).
+ *
+ * Drop-in boundary for libigl's winding-number query path.  Each entry point
+ * names the reference interface it replaces.  Conventions:
+ *   - plain pointers and sizes, row-major contiguous buffers (n x 3);
+ *   - caller-owned inputs and outputs; a handle owns its device tree;
+ *   - every function returns a gwn_status; gwn_last_error() describes the last
+ *     failure on the calling thread (the reference asserts instead);
+ *   - *_device entry points take device pointers and a hipStream_t (passed as
+ *     void*, NULL = the default stream) and are asynchronous;
+ *     the others take host pointers and return when the result is in place;
+ *   - an empty mesh gives W = 0 (the reference returns an uninitialised value,
+ *     FastWindingNumberForSoups.h:7281-7283).
+ */
+#ifndef GWN_H
+#define GWN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GWN_OK = 0,
+  GWN_ERR_INVALID = 1,  /* bad argument (null pointer, size, order, index) */
+  GWN_ERR_HIP = 2,      /* a HIP runtime call failed */
+  GWN_ERR_NOMEM = 3,    /* device allocation failed / over the admission budget */
+  GWN_ERR_NODEVICE = 4  /* no usable gfx950 device */
+} gwn_status;
+
+/* Tree construction for the triangle soup. */
+typedef enum {
+  /* The reference topology: UT::BVH<4>::init<BOX_AREA> restated on the host
+   * (FastWindingNumberForSoups.h:4579-5944); moments and traversal on the GPU.
+   * Results track the reference fast_winding_number to float rounding. */
+  GWN_TREE_SAH4 = 0,
+  /* Built on the GPU: Morton codes, LDS radix sort, Karras binary radix tree
+   * collapsed to a 4-wide tree. */
+  GWN_TREE_LBVH = 1
+} gwn_tree_kind;
+
+/* Arithmetic of a soup query. */
+typedef enum {
+  GWN_PREC_FP32 = 0,  /* as the reference: FP32 far field and FP32 atan2 */
+  /* FP32 far field; FP64 VOS solid angle for triangles within near_dist of the
+   * query; FP64 running sum; queries with |w-0.5| < ambiguous_band are
+   * re-evaluated entirely in FP64. */
+  GWN_PREC_MIXED = 1,
+  GWN_PREC_FP64 = 2   /* everything in FP64 (moments, far field, atan2) */
+} gwn_precision;
+
+typedef struct gwn_soup gwn_soup;   /* igl::FastWindingNumberBVH (fast_winding_number.h:140-152) */
+typedef struct gwn_cloud gwn_cloud; /* octree + CM/R/EC of the point-cloud path */
+
+typedef struct {
+  int64_t num_vertices, num_triangles;
+  int64_t num_nodes;   /* 4-wide nodes */
+  int64_t num_slots;   /* non-empty child slots (rope layout) */
+  int32_t order, tree_kind, depth;
+  int32_t reserved;
+  int64_t device_bytes;
+  float bbox[6];       /* min xyz, max xyz of the vertices */
+} gwn_soup_info;
+
+typedef struct {
+  float near_dist;       /* FP64 solid angle for triangles closer than this (default 1e-3) */
+  float ambiguous_band;  /* re-evaluate in FP64 when |w-0.5| < band (default 0.05) */
+} gwn_mixed_params;
+
+const char* gwn_last_error(void);
+int gwn_device_count(void);
+/* Library version string and the device it runs on (for logs). */
+const char* gwn_version(void);
+
+/* Kernel timing for measurement tools (no counterpart in the reference).
+ * While enabled, the library brackets its named kernels ("soup_traverse",
+ * "soup_traverse64", "exact", "cloud_traverse", "radix_sort") with HIP events
+ * recorded on the stream they are launched on; gwn_profile_read synchronises
+ * on those events and returns the accumulated time and launch count. */
+int gwn_profile_enable(int on);
+int gwn_profile_read(const char* kernel, double* total_ms, int64_t* launches);
+void gwn_profile_reset(void);
+
+/* ------------------------------------------------------------------------
+ * Exact FP64 generalized winding number.
+ * Replaces igl::winding_number(V,F,O,W), winding_number.h:41-48
+ * (winding_number.cpp:22-56, 3D branch): W(o) = sum_f solid_angle(f, O(o)),
+ * solid_angle.cpp:10-55.  V nv x 3, F nf x 3, O no x 3, W no.
+ */
+int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                             int64_t no, double* W);
+int gwn_exact_winding_number_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                    int64_t no, double* W, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Triangle soup (UT_SolidAngle).
+ * gwn_soup_build replaces igl::fast_winding_number(V,F,order,fwn_bvh),
+ * fast_winding_number.h:176-180 (fast_winding_number.cpp:380-416): V is
+ * narrowed to float as the reference narrows it; order in {0,1,2}.
+ */
+int gwn_soup_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, int order, int tree_kind,
+                   gwn_soup** out);
+int gwn_soup_build_f32(const float* V, int64_t nv, const int32_t* F, int64_t nf, int order, int tree_kind,
+                       gwn_soup** out);
+void gwn_soup_free(gwn_soup* s);
+int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info);
+
+/* Replaces igl::fast_winding_number(fwn_bvh, accuracy_scale, Q, W),
+ * fast_winding_number.h:191-196 (fast_winding_number.cpp:418-437): each query
+ * is narrowed to float, W = Omega / (4 pi) in double.  The single-point
+ * overload (fast_winding_number.h:203-206) is nq = 1. */
+int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accuracy_scale, int precision,
+                   double* W);
+/* Device form: Q float nq x 3, W double nq. */
+int gwn_soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, void* stream);
+int gwn_soup_set_mixed_params(gwn_soup* s, const gwn_mixed_params* p);
+
+/* Host-only (no GPU needed): the reference-topology tree of a mesh, i.e.
+ * UT_SolidAngle::init's triangle boxes (FastWindingNumberForSoups.h:6566-6590)
+ * followed by UT::BVH<4>::init<BOX_AREA> (4579-4634).  Writes up to
+ * `capacity` nodes (4 uint32 each, reference encoding) and the node count. */
+int gwn_sah4_tree(const float* V, int64_t nv, const int32_t* F, int64_t nf, uint32_t* nodes, int64_t capacity,
+                  int64_t* num_nodes);
+
+/* Introspection for parity checks: the 4-wide node array in the reference
+ * encoding (uint32 child[4]: item index, 0x80000000|node, 0xFFFFFFFF empty,
+ * FastWindingNumberForSoups.h:4217-4232) and the per-node BoxData, 92 floats in
+ * the reference member order (6477-6521). */
+int gwn_soup_copy_nodes(const gwn_soup* s, uint32_t* out);
+int gwn_soup_copy_boxdata(const gwn_soup* s, float* out);
+
+/* ------------------------------------------------------------------------
+ * Point cloud.
+ * gwn_cloud_build replaces igl::octree(P,point_indices,CH,CN,W) (octree.h,
+ * octree.cpp:7-169) followed by the precompute
+ * igl::fast_winding_number(P,N,A,point_indices,CH,order,CM,R,EC),
+ * fast_winding_number.h:44-53.  P, N n x 3, A n; FP32.
+ */
+int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, int order, gwn_cloud** out);
+void gwn_cloud_free(gwn_cloud* c);
+/* number of octree cells (CH rows) */
+int gwn_cloud_num_cells(const gwn_cloud* c, int64_t* ncells);
+/* CH ncells x 8 (int32, -1 for leaves), CN ncells x 3, W ncells,
+ * CM ncells x 3, R ncells, EC ncells x (3/12/39), and the points of each cell
+ * (CSR: cell_start[ncells+1], point_list) in ascending point order. */
+int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, int64_t* cell_start,
+                          int32_t* point_list);
+int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC);
+/* Replaces igl::fast_winding_number(P,N,A,point_indices,CH,CM,R,EC,Q,beta,WN),
+ * fast_winding_number.h:88-99; beta <= 0 is the reference's brute-force
+ * branch (fast_winding_number.cpp:300-310).  Q nq x 3 float, WN float. */
+int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN);
+int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

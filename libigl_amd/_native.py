@@ -1,0 +1,93 @@
+"""ctypes binding of libgwn.so (the HIP library).  No fallback: if the library
+is missing or a call fails, this raises.  The symbols bound here are exactly
+the ones include/gwn.h declares."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgwn.so")
+
+GWN_OK, GWN_ERR_INVALID, GWN_ERR_HIP, GWN_ERR_NOMEM, GWN_ERR_NODEVICE = 0, 1, 2, 3, 4
+TREE_SAH4, TREE_LBVH = 0, 1
+PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+
+class SoupInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_int64), ("num_triangles", C.c_int64), ("num_nodes", C.c_int64),
+                ("num_slots", C.c_int64), ("order", C.c_int32), ("tree_kind", C.c_int32), ("depth", C.c_int32),
+                ("reserved", C.c_int32), ("device_bytes", C.c_int64), ("bbox", C.c_float * 6)]
+
+
+class MixedParams(C.Structure):
+    _fields_ = [("near_dist", C.c_float), ("ambiguous_band", C.c_float)]
+
+
+# name -> (restype, argtypes); mirrors include/gwn.h
+SIGNATURES = {
+    "gwn_last_error": (C.c_char_p, []),
+    "gwn_device_count": (C.c_int, []),
+    "gwn_version": (C.c_char_p, []),
+    "gwn_profile_enable": (C.c_int, [C.c_int]),
+    "gwn_profile_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "gwn_profile_reset": (None, []),
+    "gwn_exact_winding_number": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
+    "gwn_exact_winding_number_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, vp, C.c_int64, vp, vp]),
+    "gwn_soup_build": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
+    "gwn_soup_build_f32": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
+    "gwn_soup_free": (None, [vp]),
+    "gwn_soup_get_info": (C.c_int, [vp, C.POINTER(SoupInfo)]),
+    "gwn_soup_query": (C.c_int, [vp, f64p, C.c_int64, C.c_float, C.c_int, f64p]),
+    "gwn_soup_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_float, C.c_int, vp, vp]),
+    "gwn_soup_set_mixed_params": (C.c_int, [vp, C.POINTER(MixedParams)]),
+    "gwn_sah4_tree": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64)]),
+    "gwn_soup_copy_nodes": (C.c_int, [vp, u32p]),
+    "gwn_soup_copy_boxdata": (C.c_int, [vp, f32p]),
+    "gwn_cloud_build": (C.c_int, [f32p, f32p, f32p, C.c_int64, C.c_int, C.POINTER(vp)]),
+    "gwn_cloud_free": (None, [vp]),
+    "gwn_cloud_num_cells": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+    "gwn_cloud_copy_octree": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
+    "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
+    "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),
+}
+
+_lib = None
+
+
+class GWNError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gwn error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libgwn.so; raises if it is not built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libgwn.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GWN_OK:
+        msg = lib().gwn_last_error()
+        raise GWNError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,397 @@
+// api.cpp -- extern "C" entry points of libgwn.so (declared in include/gwn.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gwn_internal.h"
+
+namespace gwn {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void ensure_device() {
+  static std::once_flag once;
+  static int ok = 0;
+  std::call_once(once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) == hipSuccess && n > 0) ok = 1;
+    (void)hipGetLastError();
+  });
+  if (!ok) throw Error{GWN_ERR_NODEVICE, "no HIP device visible"};
+}
+
+// defined in the .hip translation units
+void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
+                  hipStream_t st, DevBuf<double>& tri_scratch);
+void soup_setup(gwn_soup* s, hipStream_t st);
+void soup_finish_build(gwn_soup* s, hipStream_t st);
+void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
+                       hipStream_t st);
+void cloud_build_device(gwn_cloud* c, hipStream_t st);
+void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st);
+
+template <typename Fn>
+static int guard(Fn&& fn) {
+  try {
+    fn();
+    return GWN_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return GWN_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return GWN_ERR_INVALID;
+  }
+}
+
+template <typename T>
+static void upload(DevBuf<T>& d, const T* h, size_t n, hipStream_t st) {
+  d.alloc(n);
+  if (n) GWN_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+}
+
+static void check_faces(const int32_t* F, int64_t nf, int64_t nv) {
+  for (int64_t i = 0; i < 3 * nf; ++i)
+    GWN_CHECK(F[i] >= 0 && F[i] < nv, GWN_ERR_INVALID, "face index out of range at F[" + std::to_string(i) + "]");
+}
+
+static void soup_build_common(gwn_soup* s, std::vector<float>& U, const int32_t* F, int64_t nv, int64_t nf,
+                              int order, int tree_kind) {
+  GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
+  GWN_CHECK(tree_kind == GWN_TREE_SAH4 || tree_kind == GWN_TREE_LBVH, GWN_ERR_INVALID, "unknown tree kind");
+  GWN_CHECK(nv >= 0 && nf >= 0 && nf < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad sizes");
+  if (nf > 0) GWN_CHECK(F != nullptr && nv > 0, GWN_ERR_INVALID, "null faces/vertices");
+  check_faces(F, nf, nv);
+  ensure_device();
+  hipStream_t st = nullptr;
+  s->nv = nv;
+  s->nf = nf;
+  s->order = order;
+  s->tree_kind = tree_kind;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float v = U[3 * i + a];
+      if (std::isfinite(v)) {
+        lo[a] = std::min(lo[a], v);
+        hi[a] = std::max(hi[a], v);
+      }
+    }
+  for (int a = 0; a < 3; ++a) {
+    s->bbox[a] = std::isfinite(lo[a]) ? lo[a] : 0.0f;
+    s->bbox[3 + a] = std::isfinite(hi[a]) ? hi[a] : 0.0f;
+  }
+  upload(s->U, U.data(), U.size(), st);
+  upload(s->F, F, 3 * (size_t)nf, st);
+  soup_setup(s, st);
+  if (nf == 0) {
+    GWN_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  if (tree_kind == GWN_TREE_SAH4) {
+    std::vector<float> boxes(6 * (size_t)nf);
+    GWN_HIP(hipMemcpyAsync(boxes.data(), s->boxes.p, sizeof(float) * boxes.size(), hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t> nodes;
+    sah4_build(boxes.data(), (uint32_t)nf, nodes);
+    s->nnodes = (int64_t)(nodes.size() / 4);
+    if (s->nnodes) upload(s->nodes, nodes.data(), nodes.size(), st);
+  } else {
+    s->nnodes = lbvh_build(s->boxes.p, nf, s->bbox, s->nodes, st);
+  }
+  soup_finish_build(s, st);
+}
+
+}  // namespace gwn
+
+using namespace gwn;
+
+extern "C" {
+
+const char* gwn_last_error(void) { return g_last_error.c_str(); }
+
+const char* gwn_version(void) { return "gwn-mi355x 0.1 (gfx950)"; }
+
+int gwn_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int gwn_exact_winding_number_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                    int64_t no, double* W, void* stream) {
+  return guard([&] {
+    GWN_CHECK(nv >= 0 && nf >= 0 && no >= 0, GWN_ERR_INVALID, "negative size");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(nf == 0 || (V && F), GWN_ERR_INVALID, "null mesh");
+    ensure_device();
+    static thread_local DevBuf<double> scratch;
+    exact_device(V, nv, F, nf, O, no, W, (hipStream_t)stream, scratch);
+  });
+}
+
+int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no,
+                             double* W) {
+  return guard([&] {
+    GWN_CHECK(nv >= 0 && nf >= 0 && no >= 0, GWN_ERR_INVALID, "negative size");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    if (no == 0) return;
+    if (nf > 0) {
+      GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
+      check_faces(F, nf, nv);
+    }
+    ensure_device();
+    hipStream_t st = nullptr;
+    DevBuf<double> dV, dO, dW, scratch;
+    DevBuf<int32_t> dF;
+    upload(dV, V, 3 * (size_t)nv, st);
+    upload(dF, F, 3 * (size_t)nf, st);
+    upload(dO, O, 3 * (size_t)no, st);
+    dW.alloc(no);
+    exact_device(dV.p, nv, dF.p, nf, dO.p, no, dW.p, st, scratch);
+    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_soup_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, int order, int tree_kind,
+                   gwn_soup** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(nv == 0 || V != nullptr, GWN_ERR_INVALID, "null vertices");
+    // fast_winding_number.cpp:392-399: the BVH keeps a float copy of V
+    std::vector<float> U(3 * (size_t)nv);
+    for (size_t i = 0; i < U.size(); ++i) U[i] = (float)V[i];
+    gwn_soup* s = new gwn_soup();
+    try {
+      soup_build_common(s, U, F, nv, nf, order, tree_kind);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int gwn_soup_build_f32(const float* V, int64_t nv, const int32_t* F, int64_t nf, int order, int tree_kind,
+                       gwn_soup** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(nv == 0 || V != nullptr, GWN_ERR_INVALID, "null vertices");
+    std::vector<float> U(V, V + 3 * (size_t)nv);
+    gwn_soup* s = new gwn_soup();
+    try {
+      soup_build_common(s, U, F, nv, nf, order, tree_kind);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+void gwn_soup_free(gwn_soup* s) { delete s; }
+
+int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info) {
+  return guard([&] {
+    GWN_CHECK(s && info, GWN_ERR_INVALID, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->num_vertices = s->nv;
+    info->num_triangles = s->nf;
+    info->num_nodes = s->nnodes;
+    info->num_slots = s->nslots;
+    info->order = s->order;
+    info->tree_kind = s->tree_kind;
+    info->depth = s->depth;
+    info->device_bytes = (int64_t)(s->U.bytes() + s->F.bytes() + s->tri.bytes() + s->boxes.bytes() +
+                                   s->nodes.bytes() + s->boxdata.bytes() + s->slots.bytes() + s->slots64.bytes() +
+                                   s->work.bytes());
+    std::memcpy(info->bbox, s->bbox, sizeof(s->bbox));
+  });
+}
+
+int gwn_soup_set_mixed_params(gwn_soup* s, const gwn_mixed_params* p) {
+  return guard([&] {
+    GWN_CHECK(s && p, GWN_ERR_INVALID, "null argument");
+    GWN_CHECK(p->near_dist >= 0 && p->ambiguous_band >= 0, GWN_ERR_INVALID, "negative parameter");
+    s->mixed = *p;
+  });
+}
+
+int gwn_soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, void* stream) {
+  return guard([&] {
+    GWN_CHECK(s != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && W), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(precision >= GWN_PREC_FP32 && precision <= GWN_PREC_FP64, GWN_ERR_INVALID, "unknown precision");
+    soup_query_device(s, Q, nq, accuracy_scale, precision, W, (hipStream_t)stream);
+  });
+}
+
+int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accuracy_scale, int precision, double* W) {
+  return guard([&] {
+    GWN_CHECK(s != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && W), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(precision >= GWN_PREC_FP32 && precision <= GWN_PREC_FP64, GWN_ERR_INVALID, "unknown precision");
+    if (nq == 0) return;
+    // fast_winding_number.cpp:431-434: each query coordinate narrowed to float
+    std::vector<float> q(3 * (size_t)nq);
+    for (size_t i = 0; i < q.size(); ++i) q[i] = (float)Q[i];
+    hipStream_t st = nullptr;
+    DevBuf<float> dQ;
+    DevBuf<double> dW(nq);
+    upload(dQ, q.data(), q.size(), st);
+    soup_query_device(s, dQ.p, nq, accuracy_scale, precision, dW.p, st);
+    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_sah4_tree(const float* V, int64_t nv, const int32_t* F, int64_t nf, uint32_t* nodes, int64_t capacity,
+                  int64_t* num_nodes) {
+  return guard([&] {
+    GWN_CHECK(num_nodes != nullptr, GWN_ERR_INVALID, "null node count");
+    GWN_CHECK(nv >= 0 && nf >= 0 && nf < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad sizes");
+    if (nf > 0) {
+      GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
+      check_faces(F, nf, nv);
+    }
+    std::vector<float> boxes(6 * (size_t)nf);
+    auto hmin = [](float a, float b) { return (a > b) ? b : a; };
+    auto hmax = [](float a, float b) { return (a < b) ? b : a; };
+    for (int64_t i = 0; i < nf; ++i)
+      for (int a = 0; a < 3; ++a) {
+        float lo = V[3 * (int64_t)F[3 * i] + a], hi = lo;
+        for (int k = 1; k < 3; ++k) {
+          const float v = V[3 * (int64_t)F[3 * i + k] + a];
+          lo = hmin(lo, v);
+          hi = hmax(hi, v);
+        }
+        boxes[6 * i + 2 * a] = lo;
+        boxes[6 * i + 2 * a + 1] = hi;
+      }
+    std::vector<uint32_t> out;
+    sah4_build(boxes.data(), (uint32_t)nf, out);
+    *num_nodes = (int64_t)(out.size() / 4);
+    GWN_CHECK(nodes == nullptr || capacity >= *num_nodes, GWN_ERR_INVALID, "node capacity too small");
+    if (nodes && !out.empty()) std::memcpy(nodes, out.data(), sizeof(uint32_t) * out.size());
+  });
+}
+
+int gwn_soup_copy_nodes(const gwn_soup* s, uint32_t* out) {
+  return guard([&] {
+    GWN_CHECK(s && out, GWN_ERR_INVALID, "null argument");
+    if (s->nnodes)
+      GWN_HIP(hipMemcpy(out, s->nodes.p, sizeof(uint32_t) * 4 * (size_t)s->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwn_soup_copy_boxdata(const gwn_soup* s, float* out) {
+  return guard([&] {
+    GWN_CHECK(s && out, GWN_ERR_INVALID, "null argument");
+    if (s->nnodes)
+      GWN_HIP(hipMemcpy(out, s->boxdata.p, sizeof(float) * 92 * (size_t)s->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+// ---------------- point cloud ----------------
+int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, int order, gwn_cloud** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
+    GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+    GWN_CHECK(n == 0 || (P && N && A), GWN_ERR_INVALID, "null input");
+    ensure_device();
+    hipStream_t st = nullptr;
+    gwn_cloud* c = new gwn_cloud();
+    try {
+      c->n = n;
+      c->order = order;
+      c->nterms = order == 0 ? 3 : (order == 1 ? 12 : 39);
+      upload(c->P, P, 3 * (size_t)n, st);
+      upload(c->N, N, 3 * (size_t)n, st);
+      upload(c->A, A, (size_t)n, st);
+      cloud_build_device(c, st);
+      GWN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+void gwn_cloud_free(gwn_cloud* c) { delete c; }
+
+int gwn_cloud_num_cells(const gwn_cloud* c, int64_t* ncells) {
+  return guard([&] {
+    GWN_CHECK(c && ncells, GWN_ERR_INVALID, "null argument");
+    *ncells = c->ncells;
+  });
+}
+
+int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, int64_t* cell_start,
+                          int32_t* point_list) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    const size_t nc = (size_t)c->ncells;
+    if (CH && nc) GWN_HIP(hipMemcpy(CH, c->CH.p, sizeof(int32_t) * 8 * nc, hipMemcpyDeviceToHost));
+    if (CN && nc) GWN_HIP(hipMemcpy(CN, c->CN.p, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost));
+    if (W && nc) GWN_HIP(hipMemcpy(W, c->Wd.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
+    if (cell_start && nc) GWN_HIP(hipMemcpy(cell_start, c->cstart.p, sizeof(int64_t) * (nc + 1), hipMemcpyDeviceToHost));
+    if (point_list && c->n) GWN_HIP(hipMemcpy(point_list, c->perm.p, sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    const size_t nc = (size_t)c->ncells;
+    if (CM && nc) GWN_HIP(hipMemcpy(CM, c->CM.p, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost));
+    if (R && nc) GWN_HIP(hipMemcpy(R, c->R.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
+    if (EC && nc) GWN_HIP(hipMemcpy(EC, c->EC.p, sizeof(float) * (size_t)c->nterms * nc, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && WN), GWN_ERR_INVALID, "null query/output");
+    cloud_query_device(c, Q, nq, beta, WN, (hipStream_t)stream);
+  });
+}
+
+int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && WN), GWN_ERR_INVALID, "null query/output");
+    if (nq == 0) return;
+    hipStream_t st = nullptr;
+    DevBuf<float> dQ, dW(nq);
+    upload(dQ, Q, 3 * (size_t)nq, st);
+    cloud_query_device(c, dQ.p, nq, beta, dW.p, st);
+    GWN_HIP(hipMemcpyAsync(WN, dW.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

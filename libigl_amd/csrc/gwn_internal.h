@@ -1,0 +1,167 @@
+// gwn_internal.h -- shared definitions of the HIP winding-number library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/gwn.h"
+
+namespace gwn {
+
+// ---- error reporting (thread-local last error, returned as gwn_status) ----
+void set_error(const std::string& msg);
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define GWN_HIP(call)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess)                                                                               \
+      throw ::gwn::Error{GWN_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " (" + \
+                                          __FILE__ + ":" + std::to_string(__LINE__) + ")"};             \
+  } while (0)
+
+#define GWN_CHECK(cond, code, msg)                           \
+  do {                                                       \
+    if (!(cond)) throw ::gwn::Error{(code), std::string(msg)}; \
+  } while (0)
+
+// ---- device buffer owning hipMalloc'ed memory ----
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw Error{GWN_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    }
+    n = count;
+  }
+  // grow-only (keeps capacity); contents not preserved
+  void reserve(size_t count) {
+    if (count > n || p == nullptr) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Checks that a kernel launch was accepted (async errors surface at the sync).
+inline void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error{GWN_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void ensure_device();
+
+// Brackets a kernel launch with HIP events when profiling is enabled
+// (gwn_profile_enable); the elapsed time is collected by gwn_profile_read.
+class ProfileScope {
+ public:
+  ProfileScope(const char* name, hipStream_t st);
+  ~ProfileScope();
+  ProfileScope(const ProfileScope&) = delete;
+  ProfileScope& operator=(const ProfileScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t st_;
+  hipEvent_t a_{}, b_{};
+  bool active_ = false;
+};
+
+// ---- tree encoding shared with the reference (FastWindingNumberForSoups.h:4217-4232) ----
+constexpr uint32_t NODE_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t NODE_INTERNAL = 0x80000000u;
+
+// Host build of the reference topology (UT::BVH<4>::init<BOX_AREA>).
+// boxes: nf x 6 floats (xmin,xmax,ymin,ymax,zmin,zmax); returns nodes (4 per node).
+void sah4_build(const float* boxes, uint32_t nboxes, std::vector<uint32_t>& nodes);
+
+// ---- radix sort (LDS), device ----
+// Stable sort of (keys, values) by key; keys uint32 or uint64 with the low
+// `bits` bits significant.  tmp buffers are sized by radix_sort_temp_bytes.
+size_t radix_sort_temp_bytes(int64_t n, bool key64);
+void radix_sort_u32(uint32_t* keys, uint32_t* vals, int64_t n, int bits, void* tmp, size_t tmp_bytes,
+                    hipStream_t st);
+void radix_sort_u64(uint64_t* keys, uint32_t* vals, int64_t n, int bits, void* tmp, size_t tmp_bytes,
+                    hipStream_t st);
+
+// ---- device scans ----
+size_t scan_temp_bytes(int64_t n);
+// exclusive scan of uint32 -> uint32 (in place allowed), total written to *total_dev if non-null
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* tmp,
+                        hipStream_t st);
+
+// ---- LBVH build on the device: nodes in the reference encoding, DFS preorder ----
+// tri_boxes: nf x 6 floats on device. Writes nodes (device, 4 per node) and returns count.
+int64_t lbvh_build(const float* tri_boxes_dev, int64_t nf, const float* bbox_host, DevBuf<uint32_t>& nodes,
+                   hipStream_t st);
+
+}  // namespace gwn
+
+struct gwn_soup {
+  int64_t nv = 0, nf = 0;
+  int order = 2, tree_kind = 0, depth = 0;
+  float bbox[6] = {0, 0, 0, 0, 0, 0};
+  gwn::DevBuf<float> U;         // nv x 3
+  gwn::DevBuf<int32_t> F;       // nf x 3
+  gwn::DevBuf<float> tri;       // nf x 9 (a, b, c) for exact evaluation
+  gwn::DevBuf<float> boxes;     // nf x 6 triangle boxes
+  gwn::DevBuf<uint32_t> nodes;  // nnodes x 4
+  int64_t nnodes = 0;
+  gwn::DevBuf<float> boxdata;   // nnodes x 92, reference BoxData order
+  gwn::DevBuf<float4> slots;    // nslots x 6 (rope layout, FP32 moments)
+  int64_t nslots = 0;
+  gwn::DevBuf<double> slots64;  // nslots x 24 (FP64 moments), built on first FP64 use
+  gwn_mixed_params mixed{1e-3f, 0.05f};
+  // per-handle scratch for queries (grow-only)
+  gwn::DevBuf<uint8_t> work;
+};
+
+struct gwn_cloud {
+  int64_t n = 0, ncells = 0;
+  int order = 2, nterms = 39;
+  gwn::DevBuf<float> P, N, A;          // n x 3, n x 3, n (original order)
+  gwn::DevBuf<float> Ps, NAs;          // points sorted by octree key: xyz, n*A
+  gwn::DevBuf<int32_t> perm;           // sorted position -> original index
+  gwn::DevBuf<int32_t> CH;             // ncells x 8
+  gwn::DevBuf<float> CN, Wd;           // ncells x 3, ncells
+  gwn::DevBuf<int64_t> cstart;         // ncells + 1 (into sorted points)
+  gwn::DevBuf<float> CM, R, EC;        // ncells x 3, ncells, ncells x nterms
+  gwn::DevBuf<uint8_t> work;
+};

@@ -1,0 +1,93 @@
+// profile.cpp -- per-kernel timing with HIP events on the launching stream.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gwn_internal.h"
+
+namespace gwn {
+namespace {
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+};
+struct Acc {
+  double ms = 0;
+  int64_t n = 0;
+};
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Pending> g_pending;
+std::map<std::string, Acc> g_acc;
+
+void resolve_locked() {
+  for (auto& p : g_pending) {
+    float ms = 0;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      g_acc[p.name].ms += ms;
+      g_acc[p.name].n += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+  (void)hipGetLastError();
+}
+}  // namespace
+
+bool profiling_on() {
+  std::lock_guard<std::mutex> l(g_mu);
+  return g_on;
+}
+
+ProfileScope::ProfileScope(const char* name, hipStream_t st) : name_(name), st_(st) {
+  if (!profiling_on()) return;
+  if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (hipEventRecord(a_, st_) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  active_ = true;
+}
+
+ProfileScope::~ProfileScope() {
+  if (!active_) return;
+  if (hipEventRecord(b_, st_) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  std::lock_guard<std::mutex> l(g_mu);
+  g_pending.push_back({name_, a_, b_});
+}
+
+}  // namespace gwn
+
+extern "C" {
+int gwn_profile_enable(int on) {
+  std::lock_guard<std::mutex> l(gwn::g_mu);
+  gwn::g_on = on != 0;
+  return GWN_OK;
+}
+
+int gwn_profile_read(const char* kernel, double* total_ms, int64_t* launches) {
+  std::lock_guard<std::mutex> l(gwn::g_mu);
+  gwn::resolve_locked();
+  if (!kernel || !total_ms || !launches) return GWN_ERR_INVALID;
+  auto it = gwn::g_acc.find(kernel);
+  *total_ms = it == gwn::g_acc.end() ? 0.0 : it->second.ms;
+  *launches = it == gwn::g_acc.end() ? 0 : it->second.n;
+  return GWN_OK;
+}
+
+void gwn_profile_reset(void) {
+  std::lock_guard<std::mutex> l(gwn::g_mu);
+  gwn::resolve_locked();
+  gwn::g_acc.clear();
+}
+}

@@ -1,0 +1,620 @@
+// soup.hip -- triangle-soup fast winding number on gfx950.
+//
+// Pipeline for one mesh (gwn_soup_build):
+//   k_tri_setup     per triangle: float box (the reference's K1 loop,
+//                   FastWindingNumberForSoups.h:6566-6590) and a packed
+//                   a,b,c record for exact evaluation
+//   tree            host SAH4 (reference topology) or device LBVH
+//   k_parent/k_depth, level buckets
+//   k_upward<T>     one launch per tree level, deepest first: per node, the
+//                   reference's item()/post() moment formulas -> BoxData,
+//                   LocalData for the parent, and the subtree slot count
+//   k_positions     one launch per level, root first: first slot of each node
+//                   in the rope (skip-pointer) layout
+//   k_write_slots   one 96-byte record per non-empty child slot
+//
+// Query (gwn_soup_query*): k_morton_q + radix sort orders queries along a
+// Morton curve so a wavefront's 64 queries walk similar paths; k_traverse walks
+// the rope layout with no stack: a far child slot (|q-P|^2 > maxP2*beta^2 and
+// a finite Taylor value) adds its order-0/1/2 term and jumps to its skip
+// pointer, a near internal slot steps into its first child slot (the next
+// record), a near triangle slot adds the exact Van Oosterom-Strackee angle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gwn_internal.h"
+#include "soup_moments.h"
+
+namespace gwn {
+
+constexpr uint32_t LINK_LEAF = 0x80000000u;
+constexpr double PI_D = 3.1415926535897932384626433832795;
+
+// ---------------------------------------------------------------------------
+// setup
+__global__ void k_tri_setup(const float* __restrict__ U, const int32_t* __restrict__ F, int64_t nf,
+                            float* __restrict__ boxes, float* __restrict__ tri) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int32_t* t = F + 3 * i;
+  float p[3][3];
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) p[k][a] = U[3 * (int64_t)t[k] + a];
+  for (int a = 0; a < 3; ++a) {
+    float lo = p[0][a], hi = p[0][a];
+    lo = fmin_h(lo, p[1][a]);
+    hi = fmax_h(hi, p[1][a]);
+    lo = fmin_h(lo, p[2][a]);
+    hi = fmax_h(hi, p[2][a]);
+    boxes[6 * i + 2 * a] = lo;
+    boxes[6 * i + 2 * a + 1] = hi;
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) tri[9 * i + 3 * k + a] = p[k][a];
+}
+
+__global__ void k_parent(const uint32_t* __restrict__ nodes, int64_t nn, int32_t* __restrict__ parent) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  if (i == 0) parent[0] = -1;
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * i + s];
+    if (c != NODE_EMPTY && (c & NODE_INTERNAL)) parent[c & ~NODE_INTERNAL] = (int32_t)i;
+  }
+}
+
+__global__ void k_depth(const int32_t* __restrict__ parent, int64_t nn, int32_t* __restrict__ depth,
+                        uint32_t* __restrict__ hist, int32_t* __restrict__ maxdepth) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  int d = 0;
+  int32_t p = parent[i];
+  while (p >= 0 && d < 4096) {
+    ++d;
+    p = parent[p];
+  }
+  depth[i] = d;
+  atomicAdd(&hist[d], 1u);
+  atomicMax(maxdepth, d);
+}
+
+__global__ void k_bucket(const int32_t* __restrict__ depth, int64_t nn, uint32_t* __restrict__ cursor,
+                         int32_t* __restrict__ order) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const uint32_t pos = atomicAdd(&cursor[depth[i]], 1u);
+  order[pos] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// upward pass: one level
+template <typename T>
+__global__ void k_upward(const int32_t* __restrict__ level_nodes, int64_t count, const uint32_t* __restrict__ nodes,
+                         const float* __restrict__ U, const int32_t* __restrict__ F,
+                         const float* __restrict__ boxes, int order, Local<T>* __restrict__ local,
+                         T* __restrict__ boxdata, uint32_t* __restrict__ nslots_sub) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t node = level_nodes[t];
+  Local<T> ch[4];
+  int nch = 0;
+  uint32_t cnt = 0;
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * (int64_t)node + s];
+    if (c == NODE_EMPTY) break;
+    if (c & NODE_INTERNAL) {
+      const uint32_t ci = c & ~NODE_INTERNAL;
+      ch[s] = local[ci];
+      cnt += 1 + nslots_sub[ci];
+    } else {
+      tri_local<T>(U, F, boxes, c, order, ch[s]);
+      cnt += 1;
+    }
+    ++nch;
+  }
+  T bd[92];
+  Local<T> out;
+  node_post<T>(ch, nch, order, bd, out);
+  local[node] = out;
+  for (int k = 0; k < 92; ++k) boxdata[92 * (int64_t)node + k] = bd[k];
+  nslots_sub[node] = cnt;
+}
+
+// top-down: first slot position of each child node
+__global__ void k_positions(const int32_t* __restrict__ level_nodes, int64_t count,
+                            const uint32_t* __restrict__ nodes, const uint32_t* __restrict__ nslots_sub,
+                            uint32_t* __restrict__ first_slot) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t node = level_nodes[t];
+  uint32_t pos = first_slot[node];
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * (int64_t)node + s];
+    if (c == NODE_EMPTY) break;
+    if (c & NODE_INTERNAL) {
+      const uint32_t ci = c & ~NODE_INTERNAL;
+      first_slot[ci] = pos + 1;
+      pos += 1 + nslots_sub[ci];
+    } else {
+      pos += 1;
+    }
+  }
+}
+
+// Slot record (6 x float4):
+//   0: P.x P.y P.z maxP2          1: N.x N.y N.z link
+//   2: Nd0 Nd1 Nd2 (Nxy+Nyx)      3: (Nyz+Nzy) (Nzx+Nxz) Nkd0 Nkd1
+//   4: Nkd2 SP C2xxy C2xxz        5: C2yyz C2yyx C2zzx C2zzy
+// link = skip index for an internal child, LINK_LEAF|triangle for an item.
+__global__ void k_write_slots(int64_t nn, const uint32_t* __restrict__ nodes, const float* __restrict__ boxdata,
+                              const uint32_t* __restrict__ nslots_sub, const uint32_t* __restrict__ first_slot,
+                              float4* __restrict__ slots) {
+  int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nn) return;
+  const float* bd = boxdata + 92 * node;
+  uint32_t pos = first_slot[node];
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * node + s];
+    if (c == NODE_EMPTY) break;
+    uint32_t link, next;
+    if (c & NODE_INTERNAL) {
+      next = pos + 1 + nslots_sub[c & ~NODE_INTERNAL];
+      link = next;
+    } else {
+      next = pos + 1;
+      link = LINK_LEAF | c;
+    }
+    float4* r = slots + 6 * (int64_t)pos;
+    r[0] = make_float4(bd[BD_P + s], bd[BD_P + 4 + s], bd[BD_P + 8 + s], bd[BD_MAXP2 + s]);
+    r[1] = make_float4(bd[BD_N + s], bd[BD_N + 4 + s], bd[BD_N + 8 + s], __uint_as_float(link));
+    r[2] = make_float4(bd[BD_ND + s], bd[BD_ND + 4 + s], bd[BD_ND + 8 + s], bd[BD_NXY + s]);
+    r[3] = make_float4(bd[BD_NYZ + s], bd[BD_NZX + s], bd[BD_NKD + s], bd[BD_NKD + 4 + s]);
+    r[4] = make_float4(bd[BD_NKD + 8 + s], bd[BD_SP + s], bd[BD_CXXY + s], bd[BD_CXXZ + s]);
+    r[5] = make_float4(bd[BD_CYYZ + s], bd[BD_CYYX + s], bd[BD_CZZX + s], bd[BD_CZZY + s]);
+    pos = next;
+  }
+}
+
+// FP64 slot record: 24 doubles in the same field order (link kept in the FP32 record).
+__global__ void k_write_slots64(int64_t nn, const uint32_t* __restrict__ nodes, const double* __restrict__ bd64,
+                                const uint32_t* __restrict__ first_slot, const uint32_t* __restrict__ nslots_sub,
+                                double* __restrict__ slots64) {
+  int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nn) return;
+  const double* bd = bd64 + 92 * node;
+  uint32_t pos = first_slot[node];
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * node + s];
+    if (c == NODE_EMPTY) break;
+    double* r = slots64 + 24 * (int64_t)pos;
+    r[0] = bd[BD_P + s];
+    r[1] = bd[BD_P + 4 + s];
+    r[2] = bd[BD_P + 8 + s];
+    r[3] = bd[BD_MAXP2 + s];
+    r[4] = bd[BD_N + s];
+    r[5] = bd[BD_N + 4 + s];
+    r[6] = bd[BD_N + 8 + s];
+    r[7] = 0.0;
+    r[8] = bd[BD_ND + s];
+    r[9] = bd[BD_ND + 4 + s];
+    r[10] = bd[BD_ND + 8 + s];
+    r[11] = bd[BD_NXY + s];
+    r[12] = bd[BD_NYZ + s];
+    r[13] = bd[BD_NZX + s];
+    r[14] = bd[BD_NKD + s];
+    r[15] = bd[BD_NKD + 4 + s];
+    r[16] = bd[BD_NKD + 8 + s];
+    r[17] = bd[BD_SP + s];
+    r[18] = bd[BD_CXXY + s];
+    r[19] = bd[BD_CXXZ + s];
+    r[20] = bd[BD_CYYZ + s];
+    r[21] = bd[BD_CYYX + s];
+    r[22] = bd[BD_CZZX + s];
+    r[23] = bd[BD_CZZY + s];
+    pos += (c & NODE_INTERNAL) ? 1 + nslots_sub[c & ~NODE_INTERNAL] : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact triangle solid angle: UTsignedSolidAngleTri (6070-6110)
+template <typename T>
+__device__ __forceinline__ T tri_solid_angle(T ax, T ay, T az, T bx, T by, T bz, T cx, T cy, T cz, T qx, T qy,
+                                             T qz) {
+  T a0 = ax - qx, a1 = ay - qy, a2 = az - qz;
+  T b0 = bx - qx, b1 = by - qy, b2 = bz - qz;
+  T c0 = cx - qx, c1 = cy - qy, c2 = cz - qz;
+  const T al = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+  const T bl = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+  const T cl = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  if (al == T(0) || bl == T(0) || cl == T(0)) return T(0);
+  const T ra = T(1) / al, rb = T(1) / bl, rc = T(1) / cl;
+  a0 *= ra; a1 *= ra; a2 *= ra;
+  b0 *= rb; b1 *= rb; b2 *= rb;
+  c0 *= rc; c1 *= rc; c2 *= rc;
+  const T u0 = b0 - a0, u1 = b1 - a1, u2 = b2 - a2;
+  const T w0 = c0 - a0, w1 = c1 - a1, w2 = c2 - a2;
+  const T num = a0 * (u1 * w2 - u2 * w1) + a1 * (u2 * w0 - u0 * w2) + a2 * (u0 * w1 - u1 * w0);
+  if (num == T(0)) return T(0);
+  const T den = T(1) + (a0 * b0 + a1 * b1 + a2 * b2) + (a0 * c0 + a1 * c1 + a2 * c2) + (b0 * c0 + b1 * c1 + b2 * c2);
+  return T(2) * atan2(num, den);
+}
+
+// Far-field Taylor value of one child slot (SolidAngleFunctors::pre,
+// 7199-7245), for q = query - P already formed and l2 = |q|^2.
+template <int ORDER, typename T, typename V4>
+__device__ __forceinline__ T far_field(T x, T y, T z, T l2, const V4& b, const V4& c, const V4& d, const V4& e,
+                                       const V4& f) {
+  const T m2 = T(1) / l2;
+  const T m1 = sqrt(m2);
+  x *= m1;
+  y *= m1;
+  z *= m1;
+  T o = -m2 * (x * b.x + y * b.y + z * b.z);
+  if (ORDER >= 1) {
+    const T x2 = x * x, y2 = y * y, z2 = z * z;
+    const T m3 = m2 * m1;
+    o += m3 * (c.x + c.y + c.z - T(3) * ((x2 * c.x + y2 * c.y + z2 * c.z) + x * y * c.w + x * z * d.y + y * z * d.x));
+    if (ORDER >= 2) {
+      const T m4 = m2 * m2;
+      // temp0 / temp1 of the reference
+      const T t00 = f.y + f.z, t01 = f.w + e.z, t02 = e.w + f.x;
+      const T t10 = y * e.z + z * e.w, t11 = z * f.x + x * f.y, t12 = x * f.z + y * f.w;
+      const T s0 = T(3) * d.z + t00, s1 = T(3) * d.w + t01, s2 = T(3) * e.x + t02;
+      o += m4 * (T(1.5) * (x * s0 + y * s1 + z * s2) -
+                 T(7.5) * ((x2 * x * d.z + y2 * y * d.w + z2 * z * e.x) + x * y * z * e.y + (x2 * t10 + y2 * t11 + z2 * t12)));
+    }
+  }
+  return o;
+}
+
+struct D4 {
+  double x, y, z, w;
+};
+
+// ---------------------------------------------------------------------------
+// traversal: FP32 (reference arithmetic) and mixed (FP64 near triangles + FP64 sum)
+template <int ORDER, bool MIXED>
+__global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slots, uint32_t nslots,
+                                                  const float* __restrict__ tri, const float* __restrict__ Q,
+                                                  const uint32_t* __restrict__ perm, int64_t nq, float acc2,
+                                                  float near_d, double* __restrict__ W) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = perm ? (int64_t)perm[t] : t;
+  const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+  float sum32 = 0.0f;
+  double sum64 = 0.0;
+  uint32_t i = 0;
+  while (i < nslots) {
+    const float4 a = slots[6 * (size_t)i];
+    const float4 b = slots[6 * (size_t)i + 1];
+    const uint32_t link = __float_as_uint(b.w);
+    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+    const float l2 = x * x + y * y + z * z;
+    const bool leaf = (link & LINK_LEAF) != 0;
+    if (!(l2 <= a.w * acc2)) {
+      float4 c, d, e, f;
+      if (ORDER >= 1) {
+        c = slots[6 * (size_t)i + 2];
+        d = slots[6 * (size_t)i + 3];
+      }
+      if (ORDER >= 2) {
+        e = slots[6 * (size_t)i + 4];
+        f = slots[6 * (size_t)i + 5];
+      }
+      const float o = far_field<ORDER, float, float4>(x, y, z, l2, b, c, d, e, f);
+      if (isfinite(o)) {
+        if (MIXED)
+          sum64 += (double)o;
+        else
+          sum32 += o;
+        i = leaf ? i + 1 : link;
+        continue;
+      }
+    }
+    if (leaf) {
+      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+      if (MIXED) {
+        const float r = sqrtf(a.w) + near_d;
+        if (l2 < r * r) {
+          sum64 += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+        } else {
+          sum64 += (double)tri_solid_angle<float>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+        }
+      } else {
+        sum32 += tri_solid_angle<float>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+      }
+    }
+    ++i;
+  }
+  W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
+}
+
+// all-FP64 form: FP64 moments, far field, atan2 and sum
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
+                                                    uint32_t nslots, const float* __restrict__ tri,
+                                                    const float* __restrict__ Q, const uint32_t* __restrict__ perm,
+                                                    int64_t nq, double acc2, double* __restrict__ W) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = perm ? (int64_t)perm[t] : t;
+  const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+  double sum = 0.0;
+  uint32_t i = 0;
+  while (i < nslots) {
+    const double* r = s64 + 24 * (size_t)i;
+    const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+    const double x = qx - r[0], y = qy - r[1], z = qz - r[2];
+    const double l2 = x * x + y * y + z * z;
+    const bool leaf = (link & LINK_LEAF) != 0;
+    if (!(l2 <= r[3] * acc2)) {
+      const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
+          e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
+      const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
+      if (isfinite(o)) {
+        sum += o;
+        i = leaf ? i + 1 : link;
+        continue;
+      }
+    }
+    if (leaf) {
+      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+      sum += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+    }
+    ++i;
+  }
+  W[qi] = sum / (4.0 * PI_D);
+}
+
+// queries with |w - 0.5| < band are collected for the FP64 re-evaluation
+__global__ void k_collect_ambiguous(const double* __restrict__ W, int64_t nq, double band, uint32_t* __restrict__ list,
+                                    uint32_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  if (fabs(W[i] - 0.5) < band) {
+    const uint32_t p = atomicAdd(count, 1u);
+    list[p] = (uint32_t)i;
+  }
+}
+
+// Morton key of a query (30 bits) over the mesh bbox, for query ordering
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+__global__ void k_morton_q(const float* __restrict__ Q, int64_t nq, float lx, float ly, float lz, float sx, float sy,
+                           float sz, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  float fx = (Q[3 * i] - lx) * sx, fy = (Q[3 * i + 1] - ly) * sy, fz = (Q[3 * i + 2] - lz) * sz;
+  fx = fminf(fmaxf(fx, 0.0f), 1023.0f);
+  fy = fminf(fmaxf(fy, 0.0f), 1023.0f);
+  fz = fminf(fmaxf(fz, 0.0f), 1023.0f);
+  keys[i] = (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) | expand_bits10((uint32_t)fz);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+struct Levels {
+  std::vector<uint32_t> offset;  // per depth: start in order[]
+  DevBuf<int32_t> order;
+  int maxdepth = 0;
+};
+
+static void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st) {
+  DevBuf<int32_t> parent(nn), depth(nn);
+  DevBuf<uint32_t> hist(4097);
+  DevBuf<int32_t> maxd(1);
+  GWN_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), st));
+  GWN_HIP(hipMemsetAsync(maxd.p, 0, sizeof(int32_t), st));
+  const unsigned g = div_up(nn, 256);
+  hipLaunchKernelGGL(k_parent, dim3(g), dim3(256), 0, st, nodes, nn, parent.p);
+  check_launch("k_parent");
+  hipLaunchKernelGGL(k_depth, dim3(g), dim3(256), 0, st, parent.p, nn, depth.p, hist.p, maxd.p);
+  check_launch("k_depth");
+  int32_t md = 0;
+  GWN_HIP(hipMemcpyAsync(&md, maxd.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  GWN_CHECK(md < 4000, GWN_ERR_INVALID, "tree too deep");
+  std::vector<uint32_t> h(md + 1);
+  GWN_HIP(hipMemcpyAsync(h.data(), hist.p, sizeof(uint32_t) * (md + 1), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  L.maxdepth = md;
+  L.offset.assign(md + 2, 0);
+  for (int d = 0; d <= md; ++d) L.offset[d + 1] = L.offset[d] + h[d];
+  GWN_HIP(hipMemcpyAsync(hist.p, L.offset.data(), sizeof(uint32_t) * (md + 1), hipMemcpyHostToDevice, st));
+  L.order.alloc(nn);
+  hipLaunchKernelGGL(k_bucket, dim3(g), dim3(256), 0, st, depth.p, nn, hist.p, L.order.p);
+  check_launch("k_bucket");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+template <typename T>
+static void upward(gwn_soup* s, const Levels& L, T* boxdata, DevBuf<uint32_t>& nsub, hipStream_t st) {
+  const int64_t nn = s->nnodes;
+  DevBuf<Local<T>> local(nn);
+  for (int d = L.maxdepth; d >= 0; --d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_upward<T>, dim3(div_up(cnt, 128)), dim3(128), 0, st, L.order.p + L.offset[d], cnt,
+                       s->nodes.p, s->U.p, s->F.p, s->boxes.p, s->order, local.p, boxdata, nsub.p);
+    check_launch("k_upward");
+  }
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+void soup_finish_build(gwn_soup* s, hipStream_t st) {
+  const int64_t nn = s->nnodes;
+  if (nn == 0) {
+    s->nslots = 0;
+    return;
+  }
+  Levels L;
+  compute_levels(s->nodes.p, nn, L, st);
+  s->depth = L.maxdepth;
+  s->boxdata.alloc(92 * nn);
+  DevBuf<uint32_t> nsub(nn), first(nn);
+  upward<float>(s, L, s->boxdata.p, nsub, st);
+  uint32_t total = 0;
+  GWN_HIP(hipMemcpyAsync(&total, nsub.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipMemsetAsync(first.p, 0, sizeof(uint32_t), st));
+  for (int d = 0; d <= L.maxdepth; ++d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_positions, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt,
+                       s->nodes.p, nsub.p, first.p);
+    check_launch("k_positions");
+  }
+  GWN_HIP(hipStreamSynchronize(st));
+  s->nslots = total;
+  s->slots.alloc(6 * (size_t)total);
+  hipLaunchKernelGGL(k_write_slots, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, s->boxdata.p, nsub.p,
+                     first.p, s->slots.p);
+  check_launch("k_write_slots");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+void soup_ensure_fp64(gwn_soup* s, hipStream_t st) {
+  if (s->slots64.p || s->nnodes == 0) return;
+  const int64_t nn = s->nnodes;
+  Levels L;
+  compute_levels(s->nodes.p, nn, L, st);
+  DevBuf<double> bd64(92 * nn);
+  DevBuf<uint32_t> nsub(nn), first(nn);
+  upward<double>(s, L, bd64.p, nsub, st);
+  GWN_HIP(hipMemsetAsync(first.p, 0, sizeof(uint32_t), st));
+  for (int d = 0; d <= L.maxdepth; ++d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_positions, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt,
+                       s->nodes.p, nsub.p, first.p);
+    check_launch("k_positions");
+  }
+  DevBuf<double> s64(24 * (size_t)s->nslots);
+  hipLaunchKernelGGL(k_write_slots64, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, bd64.p, first.p,
+                     nsub.p, s64.p);
+  check_launch("k_write_slots64");
+  GWN_HIP(hipStreamSynchronize(st));
+  s->slots64 = std::move(s64);
+}
+
+void soup_setup(gwn_soup* s, hipStream_t st) {
+  const int64_t nf = s->nf;
+  s->boxes.alloc(6 * (size_t)nf);
+  s->tri.alloc(9 * (size_t)nf);
+  if (nf > 0) {
+    hipLaunchKernelGGL(k_tri_setup, dim3(div_up(nf, 256)), dim3(256), 0, st, s->U.p, s->F.p, nf, s->boxes.p,
+                       s->tri.p);
+    check_launch("k_tri_setup");
+  }
+}
+
+// Workspace for one query call: Morton keys/permutation + sort temp + ambiguity list.
+static size_t query_work_bytes(int64_t nq) {
+  const size_t a = 256;
+  auto up = [&](size_t b) { return (b + a - 1) / a * a; };
+  return up(4 * (size_t)nq) * 2 + up(radix_sort_temp_bytes(nq, false)) + up(4 * (size_t)nq) + up(4);
+}
+
+void soup_query_device(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                       double* W, hipStream_t st) {
+  gwn_soup* s = const_cast<gwn_soup*>(sc);
+  if (nq <= 0) return;
+  if (s->nslots == 0) {
+    GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * nq, st));
+    return;
+  }
+  const size_t need = query_work_bytes(nq);
+  s->work.reserve(need);
+  uint8_t* w = s->work.p;
+  auto take = [&](size_t b) {
+    uint8_t* p = w;
+    w += (b + 255) / 256 * 256;
+    return p;
+  };
+  uint32_t* keys = (uint32_t*)take(4 * (size_t)nq);
+  uint32_t* perm = (uint32_t*)take(4 * (size_t)nq);
+  const size_t tmpb = radix_sort_temp_bytes(nq, false);
+  void* tmp = take(tmpb);
+  uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
+  uint32_t* ambn = (uint32_t*)take(4);
+
+  const float* bb = s->bbox;
+  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
+              ez = std::max(bb[5] - bb[2], 1e-30f);
+  hipLaunchKernelGGL(k_morton_q, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
+                     1024.0f / ey, 1024.0f / ez, keys, perm);
+  check_launch("k_morton_q");
+  {
+    ProfileScope prof("radix_sort", st);
+    radix_sort_u32(keys, perm, nq, 30, tmp, tmpb, st);
+  }
+
+  const float acc2 = accuracy_scale * accuracy_scale;
+  const unsigned g = div_up(nq, 256);
+  const uint32_t ns = (uint32_t)s->nslots;
+  if (precision == GWN_PREC_FP64) {
+    soup_ensure_fp64(s, st);
+    const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+    ProfileScope prof("soup_traverse64", st);
+    switch (s->order) {
+      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+    }
+    check_launch("k_traverse64");
+    return;
+  }
+  const bool mixed = precision == GWN_PREC_MIXED;
+  const float nd = s->mixed.near_dist;
+#define GWN_LAUNCH_T(O, M) \
+  hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, acc2, nd, W)
+  {
+  ProfileScope prof("soup_traverse", st);
+  if (mixed) {
+    switch (s->order) {
+      case 0: GWN_LAUNCH_T(0, true); break;
+      case 1: GWN_LAUNCH_T(1, true); break;
+      default: GWN_LAUNCH_T(2, true); break;
+    }
+  } else {
+    switch (s->order) {
+      case 0: GWN_LAUNCH_T(0, false); break;
+      case 1: GWN_LAUNCH_T(1, false); break;
+      default: GWN_LAUNCH_T(2, false); break;
+    }
+  }
+  }
+#undef GWN_LAUNCH_T
+  check_launch("k_traverse");
+  if (mixed && s->mixed.ambiguous_band > 0.0f) {
+    soup_ensure_fp64(s, st);
+    GWN_HIP(hipMemsetAsync(ambn, 0, 4, st));
+    hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
+                       ambn);
+    check_launch("k_collect_ambiguous");
+    uint32_t na = 0;
+    GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    if (na > 0) {
+      const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+      const unsigned ga = div_up(na, 256);
+      switch (s->order) {
+        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+      }
+      check_launch("k_traverse64 (ambiguous)");
+    }
+  }
+}
+
+}  // namespace gwn

@@ -1,0 +1,241 @@
+"""Reference-shaped Python interface of the MI355X winding-number library.
+
+Mirrors libigl's public functions on this path (names, argument meaning,
+defaults and output shapes), each call going through the C ABI of libgwn.so:
+
+  winding_number(V, F, O)                       winding_number.h:41-48
+  fast_winding_number(V, F, Q)                  fast_winding_number.h:160-165  (order 2, accuracy_scale 2)
+  fast_winding_number(V, F, order, bvh)         fast_winding_number.h:176-180
+  fast_winding_number(bvh, accuracy_scale, Q)   fast_winding_number.h:191-196 / 203-206
+  fast_winding_number(P, N, A, Q)               fast_winding_number.h:126-131  (order 2, beta 2)
+  fast_winding_number(P, N, A, Q, order, beta)  fast_winding_number.h:111-118
+  fast_winding_number(P, N, A, point_indices, CH, order) -> (CM, R, EC)   .h:44-53
+  fast_winding_number(P, N, A, point_indices, CH, CM, R, EC, Q, beta)     .h:88-99
+  octree(P) -> (point_indices, CH, CN, W)       octree.h
+
+Errors: the reference asserts; here invalid input raises GWNError (or
+ValueError for shape errors caught before the C ABI).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as nat
+from ._native import PREC_FP32, PREC_FP64, PREC_MIXED, TREE_LBVH, TREE_SAH4, GWNError, check, lib
+
+__all__ = ["FastWindingNumberBVH", "PointCloudFWN", "fast_winding_number", "winding_number", "octree",
+           "PREC_FP32", "PREC_MIXED", "PREC_FP64", "TREE_SAH4", "TREE_LBVH", "GWNError"]
+
+
+def _rows3(X, name, dtype):
+    X = np.asarray(X)
+    if X.ndim == 1 and X.shape[0] == 3:
+        X = X.reshape(1, 3)
+    if X.ndim != 2 or X.shape[1] != 3:
+        raise ValueError(f"{name} should be n x 3")
+    return np.ascontiguousarray(X, dtype=dtype)
+
+
+def winding_number(V, F, O):
+    """Exact generalized winding number, FP64 (igl::winding_number(V,F,O,W))."""
+    V = _rows3(V, "V", np.float64)
+    F = _rows3(F, "F", np.int32)
+    O = _rows3(O, "O", np.float64)
+    W = np.empty(O.shape[0], np.float64)
+    check(lib().gwn_exact_winding_number(V, V.shape[0], F, F.shape[0], O, O.shape[0], W))
+    return W
+
+
+class FastWindingNumberBVH:
+    """igl::FastWindingNumberBVH (fast_winding_number.h:140-152): owns the
+    tree and moments on the GPU.  Not copyable; freed with the object."""
+
+    def __init__(self):
+        self._h = None
+        self.order = 2
+
+    def _free(self):
+        if self._h is not None:
+            lib().gwn_soup_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise GWNError(nat.GWN_ERR_INVALID, "FastWindingNumberBVH not built")
+        return self._h
+
+    def build(self, V, F, order=2, tree=TREE_SAH4):
+        V = np.asarray(V)
+        F = _rows3(F, "F", np.int32)
+        if V.ndim != 2 or V.shape[1] != 3:
+            raise ValueError("V should be 3D")
+        self._free()
+        h = nat.vp()
+        if V.dtype == np.float32:
+            check(lib().gwn_soup_build_f32(np.ascontiguousarray(V), V.shape[0], F, F.shape[0], order, tree,
+                                           C_byref(h)))
+        else:
+            check(lib().gwn_soup_build(np.ascontiguousarray(V, dtype=np.float64), V.shape[0], F, F.shape[0],
+                                       order, tree, C_byref(h)))
+        self._h = h.value
+        self.order = order
+        return self
+
+    def info(self):
+        i = nat.SoupInfo()
+        check(lib().gwn_soup_get_info(self.handle, C_byref(i)))
+        return i
+
+    def nodes(self):
+        n = self.info().num_nodes
+        out = np.empty((n, 4), np.uint32)
+        if n:
+            check(lib().gwn_soup_copy_nodes(self.handle, out))
+        return out
+
+    def boxdata(self):
+        n = self.info().num_nodes
+        out = np.empty((n, 92), np.float32)
+        if n:
+            check(lib().gwn_soup_copy_boxdata(self.handle, out))
+        return out
+
+    def set_mixed(self, near_dist=1e-3, ambiguous_band=0.05):
+        p = nat.MixedParams(near_dist, ambiguous_band)
+        check(lib().gwn_soup_set_mixed_params(self.handle, C_byref(p)))
+
+    def query(self, Q, accuracy_scale=2.0, precision=PREC_FP32):
+        Q = _rows3(Q, "Q", np.float64)
+        W = np.empty(Q.shape[0], np.float64)
+        if Q.shape[0]:
+            check(lib().gwn_soup_query(self.handle, Q, Q.shape[0], float(accuracy_scale), precision, W))
+        return W
+
+    def query_device(self, Q_ptr, nq, W_ptr, accuracy_scale=2.0, precision=PREC_FP32, stream=None):
+        """Device pointers (e.g. torch tensor data_ptr()): Q float nq x 3, W double nq."""
+        check(lib().gwn_soup_query_device(self.handle, Q_ptr, nq, float(accuracy_scale), precision, W_ptr, stream))
+
+
+def C_byref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+class PointCloudFWN:
+    """Device octree + expansion coefficients of the point-cloud path."""
+
+    def __init__(self, P, N, A, order=2):
+        P = _rows3(P, "P", np.float32)
+        N = _rows3(N, "N", np.float32)
+        A = np.ascontiguousarray(np.asarray(A).reshape(-1), dtype=np.float32)
+        if not (P.shape[0] == N.shape[0] == A.shape[0]):
+            raise ValueError("P, N, A sizes differ")
+        if order not in (0, 1, 2):
+            raise ValueError("expansion_order must be 0, 1 or 2")
+        h = nat.vp()
+        check(lib().gwn_cloud_build(P, N, A, P.shape[0], order, C_byref(h)))
+        self._h = h.value
+        self.order = order
+        self.n = P.shape[0]
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                lib().gwn_cloud_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def ncells(self):
+        import ctypes
+        n = ctypes.c_int64()
+        check(lib().gwn_cloud_num_cells(self._h, ctypes.byref(n)))
+        return n.value
+
+    def octree(self):
+        m = self.ncells()
+        CH = np.empty((m, 8), np.int32)
+        CN = np.empty((m, 3), np.float32)
+        W = np.empty(m, np.float32)
+        cs = np.empty(m + 1, np.int64)
+        pl = np.empty(max(self.n, 1), np.int32)
+        check(lib().gwn_cloud_copy_octree(self._h, CH.ctypes.data, CN.ctypes.data, W.ctypes.data, cs.ctypes.data,
+                                          pl.ctypes.data))
+        return CH, CN, W, cs, pl[: self.n]
+
+    def expansion(self):
+        m = self.ncells()
+        nt = {0: 3, 1: 12, 2: 39}[self.order]
+        CM = np.empty((m, 3), np.float32)
+        R = np.empty(m, np.float32)
+        EC = np.empty((m, nt), np.float32)
+        check(lib().gwn_cloud_copy_expansion(self._h, CM.ctypes.data, R.ctypes.data, EC.ctypes.data))
+        return CM, R, EC
+
+    def query(self, Q, beta=2.0):
+        Q = _rows3(Q, "Q", np.float32)
+        W = np.empty(Q.shape[0], np.float32)
+        if Q.shape[0]:
+            check(lib().gwn_cloud_query(self._h, Q, Q.shape[0], float(beta), W))
+        return W
+
+
+def octree(P):
+    """igl::octree(P, point_indices, CH, CN, W) computed on the GPU (FP32)."""
+    c = PointCloudFWN(P, np.zeros_like(np.asarray(P, dtype=np.float32)), np.zeros(np.asarray(P).shape[0]), 0)
+    CH, CN, W, cs, pl = c.octree()
+    point_indices = [np.sort(pl[cs[i]:cs[i + 1]]).tolist() for i in range(CH.shape[0])]
+    return point_indices, CH, CN, W
+
+
+def fast_winding_number(*args, **kw):
+    """Overloads of igl::fast_winding_number, dispatched on the arguments."""
+    n = len(args)
+    if n >= 1 and isinstance(args[0], FastWindingNumberBVH):
+        bvh, accuracy_scale, Q = args
+        Qa = np.asarray(Q)
+        single = Qa.ndim == 1
+        W = bvh.query(Qa, accuracy_scale, kw.get("precision", PREC_FP32))
+        return float(W[0]) if single else W
+    if n == 4 and isinstance(args[3], FastWindingNumberBVH):
+        V, F, order, bvh = args
+        bvh.build(V, F, order, kw.get("tree", TREE_SAH4))
+        return bvh
+    if n == 3:
+        V, F, Q = args
+        bvh = FastWindingNumberBVH().build(V, F, 2, kw.get("tree", TREE_SAH4))
+        return bvh.query(Q, 2.0, kw.get("precision", PREC_FP32))
+    if n == 4:
+        P, N, A, Q = args
+        return PointCloudFWN(P, N, A, 2).query(Q, 2.0)
+    if n == 6 and isinstance(args[3], (list, tuple)):
+        P, N, A, point_indices, CH, order = args
+        c = PointCloudFWN(P, N, A, order)
+        return c.expansion()
+    if n == 6:
+        P, N, A, Q, order, beta = args
+        return PointCloudFWN(P, N, A, order).query(Q, beta)
+    if n == 10:
+        P, N, A, point_indices, CH, CM, R, EC, Q, beta = args
+        order = {3: 0, 12: 1, 39: 2}[np.asarray(EC).shape[1]]
+        return PointCloudFWN(P, N, A, order).query(Q, beta)
+    raise TypeError("no fast_winding_number overload takes these arguments")
+
+
+def sah4_tree(V, F):
+    """Host-only reference-topology tree (4 uint32 per node, reference encoding)."""
+    import ctypes
+    V = _rows3(V, "V", np.float32)
+    F = _rows3(F, "F", np.int32)
+    n = ctypes.c_int64()
+    check(lib().gwn_sah4_tree(V, V.shape[0], F, F.shape[0], None, 0, ctypes.byref(n)))
+    out = np.empty((n.value, 4), np.uint32)
+    check(lib().gwn_sah4_tree(V, V.shape[0], F, F.shape[0], out.ctypes.data, n.value, ctypes.byref(n)))
+    return out

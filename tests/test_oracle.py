@@ -1,0 +1,196 @@
+"""CPU tests pinning the oracle (no GPU).
+
+The oracle is the checker for the HIP path, so it is itself checked against
+things that do not depend on it: the compiled reference (oracle/_ref), the
+reference's own golden values, closed forms and invariants.  Each check is
+chosen so that a plausible mistake (dropped term, wrong sign, transposed
+operand, wrong tree split) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from libigl_amd import datagen as D
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+
+
+# ---------------- exact path ----------------
+def test_solid_angle_octant_closed_form():
+    # the triangle (e_x, e_y, e_z) subtends 1/8 of the sphere from the origin:
+    # w = 1/8 (sign: counter-clockwise seen from the origin's side is negative)
+    a, b, c = np.eye(3)
+    w = O.solid_angle(a, b, c, np.zeros(3))
+    assert abs(abs(w) - 0.125) < 1e-15
+    # orientation flip negates; a point beyond the plane sees the opposite sign
+    assert abs(O.solid_angle(a, c, b, np.zeros(3)) + w) < 1e-15
+    assert np.sign(O.solid_angle(a, b, c, np.ones(3))) == -np.sign(w)
+
+
+def test_solid_angle_square_pyramid():
+    # a unit square at height h seen from the origin: Omega = 4 asin(1/(1+4h^2)) ... use
+    # the exact formula for a rectangle a x b at distance d: 4 atan(ab / (2d sqrt(4d^2+a^2+b^2)))
+    d = 0.37
+    V = np.array([[-0.5, -0.5, d], [0.5, -0.5, d], [0.5, 0.5, d], [-0.5, 0.5, d]])
+    F = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    w = O.winding_number_bf(V, F, np.zeros((1, 3)))[0]
+    omega = 4 * math.atan(1.0 / (2 * d * math.sqrt(4 * d * d + 2)))
+    assert abs(abs(w) - omega / (4 * math.pi)) < 1e-14
+
+
+def test_exact_closed_mesh_inside_outside():
+    V, F = D.noisy_sphere(3, 0.01, 7)
+    O_in = np.array([[0, 0, 0], [0.3, -0.2, 0.1], [-0.5, 0.4, 0.2]])
+    O_out = np.array([[1.5, 0, 0], [0, -2, 0.3], [3, 3, 3]])
+    w = O.winding_number_bf(V, F, np.concatenate([O_in, O_out]))
+    assert np.allclose(w[:3], 1.0, atol=1e-12)
+    assert np.allclose(w[3:], 0.0, atol=1e-12)
+    w_flip = O.winding_number_bf(V, F[:, ::-1].copy(), O_in)
+    assert np.allclose(w_flip, -1.0, atol=1e-12)
+
+
+def test_single_tet_outside_point():
+    # tests/include/igl/signed_distance.cpp:4-44: the tet, P=(0.5,0.5,0.5) is outside (w = 0)
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    F = np.array([[0, 1, 3], [0, 2, 1], [0, 3, 2], [1, 2, 3]], np.int32)
+    assert abs(O.winding_number_bf(V, F, np.array([[0.5, 0.5, 0.5]]))[0]) < 1e-15
+    assert abs(O.winding_number_bf(V, F, np.array([[0.1, 0.1, 0.1]]))[0] - 1.0) < 1e-15
+
+
+@needs_ref
+def test_tri_angle_f64_matches_reference_formula():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        a, b, c, q = rng.normal(size=(4, 3))
+        # the VOS det/atan2 form and the reference's normalised form agree
+        w1 = O.solid_angle(a, b, c, q)
+        w2 = O.ref().ref_solid_angle_tri_f64(a, b, c, q) / (4 * math.pi)
+        assert abs(w1 - w2) < 1e-14
+        assert abs(O.lib().oracle_solid_angle_tri_f64(a, b, c, q) / (4 * math.pi) - w1) < 1e-14
+
+
+# ---------------- soup path ----------------
+@needs_ref
+@pytest.mark.parametrize("level,nb,seed", [(0, 0, 1), (1, 2, 2), (3, 8, 3), (5, 16, 4)])
+def test_soup_oracle_bit_exact_vs_reference(level, nb, seed):
+    V, F = D.bumpy_sphere(level, nb, seed) if nb else D.noisy_sphere(level, 0.01, seed)
+    o, r = O.Soup(V, F), O.RefSoup(V, F)
+    assert np.array_equal(o.nodes(), r.nodes())
+    assert np.array_equal(o.boxdata().view(np.uint32), r.boxdata().view(np.uint32))
+    rng = np.random.default_rng(seed)
+    Q = np.concatenate([rng.uniform(-1.6, 1.6, (300, 3)), D.near_surface_queries(V, F, 300, 1e-3, rng)])
+    for beta in (1.0, 2.0, 4.0):
+        assert np.array_equal(o.omega(Q, beta).view(np.uint32), r.omega(Q, beta).view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("order", [0, 1])
+def test_soup_oracle_lower_orders_vs_reference(order):
+    V, F = D.bumpy_sphere(3, 8, 9)
+    o, r = O.Soup(V, F, order), O.RefSoup(V, F, order)
+    rng = np.random.default_rng(1)
+    Q = rng.uniform(-1.6, 1.6, (400, 3))
+    assert np.array_equal(o.omega(Q, 2.0).view(np.uint32), r.omega(Q, 2.0).view(np.uint32))
+
+
+@needs_ref
+def test_soup_oracle_degenerate_and_nan_inputs():
+    V, F = D.noisy_sphere(2, 0.01, 5)
+    V = V.copy()
+    F = np.concatenate([F, [[0, 0, 1], [2, 2, 2]]]).astype(np.int32)  # zero-area triangles
+    Vn = np.concatenate([V, [[np.nan, 0, 0]]])
+    F2 = np.concatenate([F, [[0, 1, V.shape[0]]]]).astype(np.int32)  # a triangle with a NaN vertex
+    for VV, FF in ((V, F), (Vn, F2)):
+        o, r = O.Soup(VV, FF), O.RefSoup(VV, FF)
+        assert np.array_equal(o.nodes(), r.nodes())
+        Q = np.random.default_rng(2).uniform(-1.5, 1.5, (200, 3))
+        oo, rr = o.omega(Q), r.omega(Q)
+        assert np.array_equal(oo.view(np.uint32), rr.view(np.uint32))
+
+
+def test_soup_oracle_closed_mesh_and_large_beta_is_exact():
+    V, F = D.bumpy_sphere(3, 8, 12)
+    s = O.Soup(V, F)
+    rng = np.random.default_rng(4)
+    Q = rng.uniform(-1.2, 1.2, (200, 3))
+    w_exact = O.winding_number_bf(V, F, Q)
+    # accuracy_scale -> infinity: every triangle is evaluated exactly (FP32)
+    assert np.abs(s.query(Q, 1e6) - w_exact).max() < 2e-5
+    # default accuracy: the Taylor far field is within a few 1e-3
+    assert np.abs(s.query(Q, 2.0) - w_exact).max() < 1e-2
+
+
+# ---------------- cloud path ----------------
+def test_cloud_one_point_golden():
+    # tests/include/igl/fast_winding_number.cpp:17-53
+    P = np.array([[0.1, 0.2, 0.3]])
+    N = np.array([[4.0, 5.0, 6.0]])
+    N /= np.linalg.norm(N)
+    A = np.array([0.7])
+    c = O.Cloud(P, N, A, 2)
+    Q = np.array([[0, 0, 0], [4, -3, 2], [-1, 1, 3], [0, 5, -2]], float)
+    gold = np.array([0.38779369004261133, -0.00041235296362485, -0.00362978253577090, -0.00041235296362485])
+    assert np.abs(c.eval(Q, 2) - gold).max() < 1e-15
+
+
+def test_octree_tiny_by_hand():
+    # two points on a diagonal: root splits once; octants by (x>=c)+2(y>=c)+4(z>=c)
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]])
+    t = O.Octree(P)
+    assert t.t.ncells == 9
+    assert list(t.CH[0]) == list(range(1, 9))
+    assert np.allclose(t.CN[0], [0.5, 0.5, 0.5]) and t.W[0] == 1.0
+    assert list(t.point_indices(1)) == [0] and list(t.point_indices(8)) == [1]
+    assert np.allclose(t.CN[8], [0.75, 0.75, 0.75]) and t.W[8] == 0.5
+    # three points, two close: centres along the x axis are 0.5, 0.25, 0.125
+    # (0.1 stays in octant 0 each time) and 0.0625, where x=0.1 >= c moves
+    # point 1 to octant 1: three nested splits below the root -> 1 + 4*8 cells
+    P = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [1.0, 1.0, 1.0]])
+    t = O.Octree(P)
+    assert t.t.ncells == 33
+    assert list(t.CH[1]) == list(range(9, 17))
+    assert list(t.CH[9]) == list(range(17, 25))
+    assert list(t.CH[17]) == list(range(25, 33))
+    assert list(t.point_indices(25)) == [0] and list(t.point_indices(26)) == [1]
+    assert list(t.point_indices(8)) == [2]
+    assert np.allclose(t.CN[17], [0.0625] * 3) and t.W[17] == 0.125
+
+
+def test_cloud_taylor_terms_closed_form():
+    # a cluster of dipoles far away: expansion (order 2) ~ brute force, order 0 worse
+    rng = np.random.default_rng(8)
+    P = 0.05 * rng.normal(size=(64, 3))
+    N = rng.normal(size=(64, 3))
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    A = rng.uniform(0.1, 1.0, 64)
+    Q = np.array([[2.0, 0.3, -0.1], [-1.0, 1.5, 0.7]])
+    brute = O.Cloud(P, N, A, 2).eval(Q, 0)
+    e = {}
+    for order in (0, 1, 2):
+        w = O.Cloud(P, N, A, order).eval(Q, 2.0)
+        e[order] = np.abs(w - brute).max()
+    # each order removes one power of (cell radius / distance) ~ 1/20 here
+    assert e[2] < 0.1 * e[1] and e[1] < 0.5 * e[0]
+
+
+def test_cloud_large_beta_equals_brute_force():
+    V, F = D.noisy_sphere(2, 0.01, 3)
+    rng = np.random.default_rng(5)
+    P, N, _ = D.sample_surface(V, F, 500, rng)
+    A = np.full(500, 4 * np.pi / 500)
+    c = O.Cloud(P, N, A, 2)
+    Q = rng.uniform(-1.3, 1.3, (50, 3))
+    assert np.abs(c.eval(Q, 1e9) - c.eval(Q, 0)).max() < 1e-12
+    w = c.eval(Q, 2.0)
+    inside = np.linalg.norm(Q, axis=1) < 0.8
+    outside = np.linalg.norm(Q, axis=1) > 1.2
+    assert np.all(np.abs(w[inside] - 1) < 0.2) and np.all(np.abs(w[outside]) < 0.2)
+
+
+def test_parallel_for_slicing_matches_serial():
+    V, F = D.noisy_sphere(2, 0.01, 3)
+    s = O.Soup(V, F)
+    Q = np.random.default_rng(1).uniform(-1.5, 1.5, (3000, 3))
+    assert np.array_equal(s.query(Q, 2.0, nthreads=1), s.query(Q, 2.0, nthreads=7))
