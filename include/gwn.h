@@ -142,11 +142,14 @@ int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, i
 void gwn_cloud_free(gwn_cloud* c);
 /* number of octree cells (CH rows) */
 int gwn_cloud_num_cells(const gwn_cloud* c, int64_t* ncells);
-/* CH ncells x 8 (int32, -1 for leaves), CN ncells x 3, W ncells,
- * CM ncells x 3, R ncells, EC ncells x (3/12/39), and the points of each cell
- * (CSR: cell_start[ncells+1], point_list) in ascending point order. */
+/* CH ncells x 8 (int32, -1 for leaves), CN ncells x 3, W ncells, and the
+ * points of each cell (point_indices in the reference): cell c holds
+ * point_list[cell_start[c] .. cell_start[c] + cell_count[c]), point_list being
+ * the n point indices in octree order (sort a cell's slice to get the
+ * reference's ascending point_indices list).
+ * Any output pointer may be NULL. */
 int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, int64_t* cell_start,
-                          int32_t* point_list);
+                          int32_t* cell_count, int32_t* point_list);
 int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC);
 /* Replaces igl::fast_winding_number(P,N,A,point_indices,CH,CM,R,EC,Q,beta,WN),
  * fast_winding_number.h:88-99; beta <= 0 is the reference's brute-force

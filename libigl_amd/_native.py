@@ -56,7 +56,7 @@ SIGNATURES = {
     "gwn_cloud_build": (C.c_int, [f32p, f32p, f32p, C.c_int64, C.c_int, C.POINTER(vp)]),
     "gwn_cloud_free": (None, [vp]),
     "gwn_cloud_num_cells": (C.c_int, [vp, C.POINTER(C.c_int64)]),
-    "gwn_cloud_copy_octree": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "gwn_cloud_copy_octree": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
     "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
     "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),

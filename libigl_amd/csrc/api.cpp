@@ -348,14 +348,15 @@ int gwn_cloud_num_cells(const gwn_cloud* c, int64_t* ncells) {
 }
 
 int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, int64_t* cell_start,
-                          int32_t* point_list) {
+                          int32_t* cell_count, int32_t* point_list) {
   return guard([&] {
     GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
     const size_t nc = (size_t)c->ncells;
     if (CH && nc) GWN_HIP(hipMemcpy(CH, c->CH.p, sizeof(int32_t) * 8 * nc, hipMemcpyDeviceToHost));
     if (CN && nc) GWN_HIP(hipMemcpy(CN, c->CN.p, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost));
     if (W && nc) GWN_HIP(hipMemcpy(W, c->Wd.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
-    if (cell_start && nc) GWN_HIP(hipMemcpy(cell_start, c->cstart.p, sizeof(int64_t) * (nc + 1), hipMemcpyDeviceToHost));
+    if (cell_start && nc) GWN_HIP(hipMemcpy(cell_start, c->cstart.p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost));
+    if (cell_count && nc) GWN_HIP(hipMemcpy(cell_count, c->ccount.p, sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
     if (point_list && c->n) GWN_HIP(hipMemcpy(point_list, c->perm.p, sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToHost));
   });
 }

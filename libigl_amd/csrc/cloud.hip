@@ -1,11 +1,800 @@
-// cloud.hip -- point-cloud path (placeholder until the octree kernels land).
+// cloud.hip -- point-cloud fast winding number on gfx950.
+//
+// Octree (the reference's igl::octree, octree.cpp:7-169, cell for cell):
+//   k_oct_keys    each point's octant path, 42 levels, computed with the
+//                 reference's float centre recursion (child centre = parent
+//                 centre +- width/4, octant bits x>=c | 2(y>=c) | 4(z>=c)),
+//                 packed as two 63-bit words (3 bits per level, level 0 first)
+//   radix sort    stable LDS radix sort on (low word, then high word): points
+//                 of one cell become contiguous and keep ascending indices,
+//                 which is the order of the reference's point_indices lists
+//   k_oct_lcp     common-prefix length (in levels) of neighbouring keys, and
+//                 the number of split cells each neighbour boundary opens
+//   scan          -> the depth-first rank of every split cell: the reference
+//                 appends the 8 children of the k-th split cell (in depth-first
+//                 order) as cells 1+8k .. 8+8k, so rank gives the cell ids
+//   k_oct_level   one launch per level: split each cell's point range by the
+//                 next octant digit, write CH, centres, widths, point ranges,
+//                 and queue the children that hold two or more points
+// Cells deeper than 42 levels are not split (the reference keeps splitting up
+// to depth 30000, which only matters for points equal to ~1 float ulp).
+//
+// Precompute (fast_winding_number.cpp:8-124): per non-empty cell, CM = area-
+// weighted centre, R = max |p - CM|, EC = [sum A n, sum A (p-CM) n^T,
+// sum 1/2 (p-CM)_k A (p-CM) n^T], accumulated in FP64 (a thread per small
+// cell, a wavefront per large one) and stored in FP32.
+//
+// Evaluation (fast_winding_number.cpp:126-311): the reference's near lists
+// become a rope over the non-empty non-root cells in depth-first order: a
+// non-leaf cell with |CM-q| > beta R adds its Taylor expansion and jumps over
+// its subtree, otherwise the walk steps into its children; a leaf adds the
+// direct dipole sum of its points (which the reference does whether the leaf
+// is near or far).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "gwn_internal.h"
 
 namespace gwn {
-void cloud_build_device(gwn_cloud*, hipStream_t) { throw Error{GWN_ERR_INVALID, "point cloud path not available in this build"}; }
-void cloud_query_device(const gwn_cloud*, const float*, int64_t, double, float*, hipStream_t) {
-  throw Error{GWN_ERR_INVALID, "point cloud path not available in this build"};
+
+constexpr int OCT_LEVELS = 42;  // octant digits per point (two words of 21)
+constexpr int OCT_MAX_SPLIT_DEPTH = OCT_LEVELS - 1;
+constexpr uint32_t REC_LEAF = 0x80000000u;
+constexpr double CL_PI = 3.1415926535897932384626433832795;
+
+__global__ void k_oct_keys(const float* __restrict__ P, int64_t n, float cx, float cy, float cz, float w0,
+                           uint64_t* __restrict__ hi, uint64_t* __restrict__ lo, uint32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = P[3 * i], y = P[3 * i + 1], z = P[3 * i + 2];
+  float c0 = cx, c1 = cy, c2 = cz, w = w0;
+  uint64_t kh = 0, kl = 0;
+  for (int d = 0; d < OCT_LEVELS; ++d) {
+    uint32_t o = 0;
+    if (x >= c0) o += 1;
+    if (y >= c1) o += 2;
+    if (z >= c2) o += 4;
+    if (d < 21)
+      kh |= (uint64_t)o << (60 - 3 * d);
+    else
+      kl |= (uint64_t)o << (60 - 3 * (d - 21));
+    const float h = w / 2;   // child width
+    const float hh = h / 2;  // child centre offset
+    c0 = c0 + ((o & 1) ? hh : -hh);
+    c1 = c1 + ((o & 2) ? hh : -hh);
+    c2 = c2 + ((o & 4) ? hh : -hh);
+    w = h;
+  }
+  hi[i] = kh;
+  lo[i] = kl;
+  idx[i] = (uint32_t)i;
 }
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ perm, int64_t n,
+                             uint64_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+__device__ __forceinline__ int oct_lcp(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
+  const uint64_t x = h0 ^ h1;
+  if (x) return (__clzll(x) - 1) / 3;
+  const uint64_t y = l0 ^ l1;
+  if (y) return 21 + (__clzll(y) - 1) / 3;
+  return OCT_LEVELS;
+}
+
+// lcp[a] for boundary a (between sorted points a and a+1), a in [0, n-1);
+// opens[a] = number of split cells whose first point is a.
+__global__ void k_oct_lcp(const uint64_t* __restrict__ hi, const uint64_t* __restrict__ lo, int64_t n,
+                          int32_t* __restrict__ lcp, uint32_t* __restrict__ opens) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n - 1) return;
+  const int cur = oct_lcp(hi[a], lo[a], hi[a + 1], lo[a + 1]);
+  const int prev = a == 0 ? -1 : oct_lcp(hi[a - 1], lo[a - 1], hi[a], lo[a]);
+  lcp[a] = cur;
+  const int top = cur < OCT_MAX_SPLIT_DEPTH ? cur : OCT_MAX_SPLIT_DEPTH;
+  opens[a] = top > prev ? (uint32_t)(top - prev) : 0u;
+}
+
+__device__ __forceinline__ uint32_t oct_digit(const uint64_t* hi, const uint64_t* lo, int64_t k, int d) {
+  return d < 21 ? (uint32_t)((hi[k] >> (60 - 3 * d)) & 7u) : (uint32_t)((lo[k] >> (60 - 3 * (d - 21))) & 7u);
+}
+
+struct SplitCell {
+  int32_t cell;
+  int32_t depth;
+  int64_t start, end;
+  uint32_t rank;
+  uint32_t pad;
+};
+
+__device__ __forceinline__ uint32_t split_rank(const int32_t* lcp, const uint32_t* open_scan, int64_t s, int d) {
+  const int prev = s == 0 ? -1 : lcp[s - 1];
+  return open_scan[s] + (uint32_t)(d - prev - 1);
+}
+
+__global__ void k_oct_level(const SplitCell* __restrict__ cur, uint32_t ncur, const uint64_t* __restrict__ hi,
+                            const uint64_t* __restrict__ lo, const int32_t* __restrict__ lcp,
+                            const uint32_t* __restrict__ open_scan, int32_t* __restrict__ CH, float* __restrict__ CN,
+                            float* __restrict__ Wd, int64_t* __restrict__ cstart, int32_t* __restrict__ ccount,
+                            SplitCell* __restrict__ next, uint32_t* __restrict__ nnext, uint32_t next_cap,
+                            int64_t ncells, uint32_t* __restrict__ err) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncur) return;
+  const SplitCell sc = cur[t];
+  const float c0 = CN[3 * (int64_t)sc.cell], c1 = CN[3 * (int64_t)sc.cell + 1], c2 = CN[3 * (int64_t)sc.cell + 2];
+  const float h = Wd[sc.cell] / 2;
+  const float hh = h / 2;
+  int64_t s = sc.start;
+  for (uint32_t o = 0; o < 8; ++o) {
+    // first point of the range whose digit at this depth exceeds o
+    int64_t a = s, b = sc.end;
+    while (a < b) {
+      const int64_t m = (a + b) >> 1;
+      if (oct_digit(hi, lo, m, sc.depth) <= o)
+        a = m + 1;
+      else
+        b = m;
+    }
+    const int64_t e = a;
+    const int64_t child64 = 1 + 8 * (int64_t)sc.rank + o;
+    if (child64 >= ncells) {  // inconsistent ranks: report, never write out of range
+      atomicOr(err, 1u);
+      return;
+    }
+    const int32_t child = (int32_t)child64;
+    CH[8 * (int64_t)sc.cell + o] = child;
+    CN[3 * (int64_t)child] = c0 + ((o & 1) ? hh : -hh);
+    CN[3 * (int64_t)child + 1] = c1 + ((o & 2) ? hh : -hh);
+    CN[3 * (int64_t)child + 2] = c2 + ((o & 4) ? hh : -hh);
+    Wd[child] = h;
+    cstart[child] = s;
+    ccount[child] = (int32_t)(e - s);
+    if (e - s >= 2 && sc.depth + 1 <= OCT_MAX_SPLIT_DEPTH) {
+      SplitCell nc;
+      nc.cell = child;
+      nc.depth = sc.depth + 1;
+      nc.start = s;
+      nc.end = e;
+      nc.rank = split_rank(lcp, open_scan, s, sc.depth + 1);
+      nc.pad = 0;
+      const uint32_t slot = atomicAdd(nnext, 1u);
+      if (slot < next_cap)
+        next[slot] = nc;
+      else
+        atomicOr(err, 2u);
+    }
+    s = e;
+  }
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_permute_points(const float* __restrict__ P, const float* __restrict__ N, const float* __restrict__ A,
+                                 const uint32_t* __restrict__ perm, int64_t n, float* __restrict__ Ps,
+                                 float* __restrict__ NAs, int32_t* __restrict__ perm_out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = perm[k];
+  const float a = A[i];
+  for (int d = 0; d < 3; ++d) {
+    Ps[3 * k + d] = P[3 * i + d];
+    NAs[3 * k + d] = N[3 * i + d] * a;  // N.row * A as the reference forms it
+  }
+  perm_out[k] = (int32_t)i;
+}
+
+// ---------------- precompute ----------------
+struct CellSums {
+  double area, ap[3], an[3];
+};
+
+template <int NT>
+__device__ __forceinline__ void cell_moments_serial(const float* __restrict__ P, const float* __restrict__ N,
+                                                    const float* __restrict__ A, const int32_t* __restrict__ perm,
+                                                    int64_t s, int32_t cnt, float* __restrict__ cm,
+                                                    float* __restrict__ r, float* __restrict__ ec) {
+  double area = 0, ap[3] = {0, 0, 0}, an[3] = {0, 0, 0};
+  for (int32_t j = 0; j < cnt; ++j) {
+    const int64_t p = perm[s + j];
+    const double a = A[p];
+    area += a;
+    for (int d = 0; d < 3; ++d) {
+      ap[d] += a * (double)P[3 * p + d];
+      an[d] += a * (double)N[3 * p + d];
+    }
+  }
+  float c[3];
+  for (int d = 0; d < 3; ++d) c[d] = (float)(ap[d] / area);
+  double e[NT > 3 ? NT - 3 : 1];
+  for (int k = 0; k < NT - 3; ++k) e[k] = 0;
+  float rmax = 0.0f;
+  for (int32_t j = 0; j < cnt; ++j) {
+    const int64_t p = perm[s + j];
+    const double a = A[p];
+    float q[3];
+    for (int d = 0; d < 3; ++d) q[d] = P[3 * p + d] - c[d];
+    const float nrm = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    rmax = nrm > rmax ? nrm : rmax;
+    if (NT >= 12) {
+      const double n3[3] = {N[3 * p], N[3 * p + 1], N[3 * p + 2]};
+      for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) {
+          const double t = a * (double)q[u] * n3[v];
+          e[u + 3 * v] += t;
+          if (NT == 39)
+            for (int k = 0; k < 3; ++k) e[9 + 9 * k + u + 3 * v] += 0.5 * (double)q[k] * t;
+        }
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    cm[d] = c[d];
+    ec[d] = (float)an[d];
+  }
+  for (int k = 0; k < NT - 3; ++k) ec[3 + k] = (float)e[k];
+  *r = rmax;
+}
+
+// one thread per small cell
+template <int NT>
+__global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, const int64_t* __restrict__ cstart,
+                             const int32_t* __restrict__ ccount, const float* __restrict__ P,
+                             const float* __restrict__ N, const float* __restrict__ A,
+                             const int32_t* __restrict__ perm, float* __restrict__ CM, float* __restrict__ R,
+                             float* __restrict__ EC) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t c = cells[t];
+  cell_moments_serial<NT>(P, N, A, perm, cstart[c], ccount[c], CM + 3 * (int64_t)c, R + c, EC + (int64_t)NT * c);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// one wavefront per large cell
+template <int NT>
+__global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ cells, uint32_t count,
+                                                    const int64_t* __restrict__ cstart,
+                                                    const int32_t* __restrict__ ccount, const float* __restrict__ P,
+                                                    const float* __restrict__ N, const float* __restrict__ A,
+                                                    const int32_t* __restrict__ perm, float* __restrict__ CM,
+                                                    float* __restrict__ R, float* __restrict__ EC) {
+  const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (w >= count) return;
+  const int32_t c = cells[w];
+  const int64_t s = cstart[c];
+  const int32_t cnt = ccount[c];
+  double area = 0, ap0 = 0, ap1 = 0, ap2 = 0, an0 = 0, an1 = 0, an2 = 0;
+  for (int32_t j = lane; j < cnt; j += 64) {
+    const int64_t p = perm[s + j];
+    const double a = A[p];
+    area += a;
+    ap0 += a * (double)P[3 * p];
+    ap1 += a * (double)P[3 * p + 1];
+    ap2 += a * (double)P[3 * p + 2];
+    an0 += a * (double)N[3 * p];
+    an1 += a * (double)N[3 * p + 1];
+    an2 += a * (double)N[3 * p + 2];
+  }
+  area = wave_sum(area);
+  ap0 = wave_sum(ap0);
+  ap1 = wave_sum(ap1);
+  ap2 = wave_sum(ap2);
+  an0 = wave_sum(an0);
+  an1 = wave_sum(an1);
+  an2 = wave_sum(an2);
+  const float c3[3] = {(float)(ap0 / area), (float)(ap1 / area), (float)(ap2 / area)};
+  double e[NT > 3 ? NT - 3 : 1];
+  for (int k = 0; k < NT - 3; ++k) e[k] = 0;
+  float rmax = 0.0f;
+  for (int32_t j = lane; j < cnt; j += 64) {
+    const int64_t p = perm[s + j];
+    const double a = A[p];
+    float q[3];
+    for (int d = 0; d < 3; ++d) q[d] = P[3 * p + d] - c3[d];
+    const float nrm = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    rmax = nrm > rmax ? nrm : rmax;
+    if (NT >= 12) {
+      const double n3[3] = {N[3 * p], N[3 * p + 1], N[3 * p + 2]};
+      for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) {
+          const double t = a * (double)q[u] * n3[v];
+          e[u + 3 * v] += t;
+          if (NT == 39)
+            for (int k = 0; k < 3; ++k) e[9 + 9 * k + u + 3 * v] += 0.5 * (double)q[k] * t;
+        }
+    }
+  }
+  rmax = wave_max(rmax);
+  for (int k = 0; k < NT - 3; ++k) e[k] = wave_sum(e[k]);
+  if (lane == 0) {
+    for (int d = 0; d < 3; ++d) CM[3 * (int64_t)c + d] = c3[d];
+    R[c] = rmax;
+    float* ec = EC + (int64_t)NT * c;
+    ec[0] = (float)an0;
+    ec[1] = (float)an1;
+    ec[2] = (float)an2;
+    for (int k = 0; k < NT - 3; ++k) ec[3 + k] = (float)e[k];
+  }
+}
+
+__global__ void k_cell_empty(int64_t ncells, const int32_t* __restrict__ ccount, int nt, float* __restrict__ CM,
+                             float* __restrict__ R, float* __restrict__ EC) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells || ccount[c] > 0) return;
+  for (int d = 0; d < 3; ++d) CM[3 * c + d] = __int_as_float(0x7FC00000);  // NaN, as the reference
+  R[c] = 0.0f;
+  for (int k = 0; k < nt; ++k) EC[nt * c + k] = 0.0f;
+}
+
+// classify non-empty cells into small / large lists
+__global__ void k_cell_lists(int64_t ncells, const int32_t* __restrict__ ccount, int32_t* __restrict__ small,
+                             uint32_t* __restrict__ nsmall, int32_t* __restrict__ large, uint32_t* __restrict__ nlarge) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int32_t k = ccount[c];
+  if (k <= 0) return;
+  if (k <= 32)
+    small[atomicAdd(nsmall, 1u)] = (int32_t)c;
+  else
+    large[atomicAdd(nlarge, 1u)] = (int32_t)c;
+}
+
+// ---------------- rope over non-empty non-root cells ----------------
+// bottom-up: rc[cell] = records in the subtree of a non-empty cell (itself included)
+__global__ void k_rec_count(const SplitCell* __restrict__ lvl, uint32_t n, const int32_t* __restrict__ CH,
+                            const int32_t* __restrict__ ccount, uint32_t* __restrict__ rc) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t c = lvl[t].cell;
+  uint32_t s = 1;
+  for (int o = 0; o < 8; ++o) {
+    const int32_t ch = CH[8 * (int64_t)c + o];
+    if (ccount[ch] <= 0) continue;
+    s += (CH[8 * (int64_t)ch] >= 0) ? rc[ch] : 1u;
+  }
+  rc[c] = s;
+}
+
+// top-down: record positions of the children of each split cell
+__global__ void k_rec_pos(const SplitCell* __restrict__ lvl, uint32_t n, const int32_t* __restrict__ CH,
+                          const int32_t* __restrict__ ccount, const uint32_t* __restrict__ rc,
+                          uint32_t* __restrict__ pos) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t c = lvl[t].cell;
+  uint32_t p = (c == 0) ? 0u : pos[c] + 1;
+  for (int o = 0; o < 8; ++o) {
+    const int32_t ch = CH[8 * (int64_t)c + o];
+    if (ccount[ch] <= 0) continue;
+    pos[ch] = p;
+    p += (CH[8 * (int64_t)ch] >= 0) ? rc[ch] : 1u;
+  }
+}
+
+__global__ void k_rec_write(int64_t ncells, const int32_t* __restrict__ CH, const int32_t* __restrict__ ccount,
+                            const int64_t* __restrict__ cstart, const uint32_t* __restrict__ rc,
+                            const uint32_t* __restrict__ pos, const float* __restrict__ CM,
+                            const float* __restrict__ R, uint32_t nrec, float4* __restrict__ head,
+                            uint2* __restrict__ meta) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0 || c >= ncells || ccount[c] <= 0) return;
+  const uint32_t p = pos[c];
+  if (p >= nrec) return;
+  head[p] = make_float4(CM[3 * c], CM[3 * c + 1], CM[3 * c + 2], R[c]);
+  if (CH[8 * c] >= 0)
+    meta[p] = make_uint2(p + rc[c], (uint32_t)c);
+  else
+    meta[p] = make_uint2(REC_LEAF | (uint32_t)cstart[c], (uint32_t)ccount[c]);
+}
+
+// ---------------- evaluation ----------------
+__device__ __forceinline__ float dipole(float lx, float ly, float lz, float ax, float ay, float az) {
+  const float PI_4 = (float)(4.0 * CL_PI);
+  const float n = sqrtf(lx * lx + ly * ly + lz * lz);
+  if (n == 0.0f) return 0.5f;
+  return (lx * ax + ly * ay + lz * az) / (PI_4 * (n * n * n));
+}
+
+template <int NT>
+__device__ __forceinline__ float expansion(float lx, float ly, float lz, const float* __restrict__ ec) {
+  const float PI_4 = (float)(4.0 * CL_PI);
+  float wn = dipole(lx, ly, lz, ec[0], ec[1], ec[2]);
+  const float loc[3] = {lx, ly, lz};
+  const float r = sqrtf(lx * lx + ly * ly + lz * lz);
+  float r3 = 0, r5 = 0;
+  if (NT > 3) {
+    r3 = PI_4 * r * r * r;
+    r5 = r3 * r * r;
+    const float d = (float)(1.0 / (double)r3);
+    const float s = (float)(-3.0 / (double)r5);
+    float acc = 0.0f;
+    for (int v = 0; v < 3; ++v)
+      for (int u = 0; u < 3; ++u) {
+        float S = (loc[u] * loc[v]) * s;
+        if (u == v) S += d;
+        acc += S * ec[3 + u + 3 * v];
+      }
+    wn += acc;
+  }
+  if (NT > 12) {
+    const float r7 = r5 * r * r;
+    const float s = (float)(-3.0 / (double)r5);
+    for (int i = 0; i < 3; ++i) {
+      const float f15 = (float)(15.0 * (double)loc[i]);
+      float acc = 0.0f;
+      for (int v = 0; v < 3; ++v)
+        for (int u = 0; u < 3; ++u) {
+          float rcd = 0.0f;
+          if (u == v) rcd += loc[i];
+          if (u == i) rcd += loc[v];
+          if (v == i) rcd += loc[u];
+          const float L = loc[u] * (loc[v] / r7);
+          acc += (f15 * L + s * rcd) * ec[12 + 9 * i + u + 3 * v];
+        }
+      wn += acc;
+    }
+  }
+  return wn;
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict__ head, const uint2* __restrict__ meta,
+                                                        uint32_t nrec, const float* __restrict__ EC,
+                                                        const float* __restrict__ Ps, const float* __restrict__ NAs,
+                                                        int64_t npts, bool root_leaf, const float* __restrict__ Q,
+                                                        const uint32_t* __restrict__ perm, int64_t nq, double beta,
+                                                        float* __restrict__ WN) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = perm ? (int64_t)perm[t] : t;
+  const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+  float wn = 0.0f;
+  if (root_leaf) {
+    for (int64_t j = 0; j < npts; ++j)
+      wn += dipole(Ps[3 * j] - qx, Ps[3 * j + 1] - qy, Ps[3 * j + 2] - qz, NAs[3 * j], NAs[3 * j + 1], NAs[3 * j + 2]);
+  } else {
+    uint32_t i = 0;
+    while (i < nrec) {
+      const uint2 m = meta[i];
+      if (m.x & REC_LEAF) {
+        const int64_t s = m.x & ~REC_LEAF;
+        for (uint32_t j = 0; j < m.y; ++j) {
+          const int64_t k = s + j;
+          wn += dipole(Ps[3 * k] - qx, Ps[3 * k + 1] - qy, Ps[3 * k + 2] - qz, NAs[3 * k], NAs[3 * k + 1],
+                       NAs[3 * k + 2]);
+        }
+        ++i;
+        continue;
+      }
+      const float4 h = head[i];
+      const float lx = h.x - qx, ly = h.y - qy, lz = h.z - qz;
+      const float nrm = sqrtf(lx * lx + ly * ly + lz * lz);
+      if ((double)nrm > beta * (double)h.w) {
+        wn += expansion<NT>(lx, ly, lz, EC + (int64_t)NT * m.y);
+        i = m.x;
+      } else {
+        ++i;
+      }
+    }
+  }
+  WN[qi] = wn;
+}
+
+// beta <= 0: brute-force dipole sum with a double accumulator (300-310)
+__global__ __launch_bounds__(256) void k_cloud_brute(const float* __restrict__ Ps, const float* __restrict__ NAs,
+                                                     int64_t npts, const float* __restrict__ Q, int64_t nq,
+                                                     float* __restrict__ WN) {
+  __shared__ float tile[6 * 256];
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = q < nq;
+  const float qx = ok ? Q[3 * q] : 0.f, qy = ok ? Q[3 * q + 1] : 0.f, qz = ok ? Q[3 * q + 2] : 0.f;
+  double wn = 0.0;
+  for (int64_t b = 0; b < npts; b += 256) {
+    const int cnt = (int)((npts - b) < 256 ? (npts - b) : 256);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt)
+      for (int d = 0; d < 3; ++d) {
+        tile[6 * threadIdx.x + d] = Ps[3 * (b + threadIdx.x) + d];
+        tile[6 * threadIdx.x + 3 + d] = NAs[3 * (b + threadIdx.x) + d];
+      }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const float* t = tile + 6 * j;
+      wn += (double)dipole(t[0] - qx, t[1] - qy, t[2] - qz, t[3], t[4], t[5]);
+    }
+  }
+  if (ok) WN[q] = (float)wn;
+}
+
+// queries along a Morton curve (shared with the soup path's ordering)
+__device__ __forceinline__ uint32_t cl_expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+__global__ void k_cloud_morton_q(const float* __restrict__ Q, int64_t nq, float lx, float ly, float lz, float sx,
+                                 float sy, float sz, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  float fx = fminf(fmaxf((Q[3 * i] - lx) * sx, 0.0f), 1023.0f);
+  float fy = fminf(fmaxf((Q[3 * i + 1] - ly) * sy, 0.0f), 1023.0f);
+  float fz = fminf(fmaxf((Q[3 * i + 2] - lz) * sz, 0.0f), 1023.0f);
+  keys[i] = (cl_expand10((uint32_t)fx) << 2) | (cl_expand10((uint32_t)fy) << 1) | cl_expand10((uint32_t)fz);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------- host orchestration ----------------
+void cloud_build_device(gwn_cloud* c, hipStream_t st) {
+  const int64_t n = c->n;
+  if (n == 0) {
+    // a single empty root cell
+    c->ncells = 1;
+    c->CH.alloc(8);
+    c->CN.alloc(3);
+    c->Wd.alloc(1);
+    c->cstart.alloc(1);
+    c->ccount.alloc(1);
+    c->CM.alloc(3);
+    c->R.alloc(1);
+    c->EC.alloc(c->nterms);
+    c->perm.alloc(1);
+    c->Ps.alloc(3);
+    c->NAs.alloc(3);
+    const int32_t ch[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    const float zero3[3] = {0, 0, 0};
+    const float nan3[3] = {NAN, NAN, NAN};
+    const int64_t s0 = 0;
+    const int32_t c0 = 0;
+    const float z = 0.0f;
+    std::vector<float> ec(c->nterms, 0.0f);
+    GWN_HIP(hipMemcpyAsync(c->CH.p, ch, sizeof(ch), hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->CN.p, zero3, sizeof(zero3), hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->Wd.p, &z, 4, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->cstart.p, &s0, 8, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->ccount.p, &c0, 4, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->CM.p, nan3, sizeof(nan3), hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->R.p, &z, 4, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->EC.p, ec.data(), 4 * ec.size(), hipMemcpyHostToDevice, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    c->nrec = 0;
+    return;
+  }
+  // root cube: bbox centre and longest side (octree.cpp:139-150), in float
+  std::vector<float> hp(3 * (size_t)n);
+  GWN_HIP(hipMemcpyAsync(hp.data(), c->P.p, sizeof(float) * hp.size(), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  float mn[3] = {hp[0], hp[1], hp[2]}, mx[3] = {hp[0], hp[1], hp[2]};
+  for (int64_t i = 1; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const float v = hp[3 * i + d];
+      if (v < mn[d]) mn[d] = v;
+      if (v > mx[d]) mx[d] = v;
+    }
+  float ctr[3];
+  for (int d = 0; d < 3; ++d) ctr[d] = (mn[d] + mx[d]) / 2.0f;
+  float w0 = mx[0] - mn[0];
+  for (int d = 1; d < 3; ++d)
+    if (mx[d] - mn[d] > w0) w0 = mx[d] - mn[d];
+  for (int d = 0; d < 3; ++d) {
+    c->bbox[d] = mn[d];
+    c->bbox[3 + d] = mx[d];
+  }
+
+  // keys and sort: LSD over the two words (low word, then high word), stable
+  DevBuf<uint64_t> hi(n), lo(n), lo0(n), key2(n), ls(n);
+  DevBuf<uint32_t> idx(n);
+  const unsigned g = div_up(n, 256);
+  hipLaunchKernelGGL(k_oct_keys, dim3(g), dim3(256), 0, st, c->P.p, n, ctr[0], ctr[1], ctr[2], w0, hi.p, lo.p, idx.p);
+  check_launch("k_oct_keys");
+  GWN_HIP(hipMemcpyAsync(lo0.p, lo.p, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+  {
+    const size_t tb = radix_sort_temp_bytes(n, true);
+    DevBuf<uint8_t> tmp(tb);
+    radix_sort_u64(lo.p, idx.p, n, 63, tmp.p, tb, st);
+    hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, st, hi.p, idx.p, n, key2.p);
+    check_launch("k_gather_u64");
+    radix_sort_u64(key2.p, idx.p, n, 63, tmp.p, tb, st);
+  }
+  hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, st, lo0.p, idx.p, n, ls.p);
+  check_launch("k_gather_u64");
+  const uint64_t* HS = key2.p;  // high words in octree order
+  const uint64_t* LS = ls.p;    // low words in octree order
+
+  // points in octree order
+  c->Ps.alloc(3 * (size_t)n);
+  c->NAs.alloc(3 * (size_t)n);
+  c->perm.alloc(n);
+  hipLaunchKernelGGL(k_permute_points, dim3(g), dim3(256), 0, st, c->P.p, c->N.p, c->A.p, idx.p, n, c->Ps.p, c->NAs.p,
+                     c->perm.p);
+  check_launch("k_permute_points");
+
+  // split-cell ranks
+  const int64_t nb = n - 1;
+  DevBuf<int32_t> lcp(nb > 0 ? nb : 1);
+  DevBuf<uint32_t> opens(nb > 0 ? nb : 1), oscan(nb > 0 ? nb : 1), total(1);
+  uint32_t S = 0;
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_oct_lcp, dim3(div_up(nb, 256)), dim3(256), 0, st, HS, LS, n, lcp.p, opens.p);
+    check_launch("k_oct_lcp");
+    DevBuf<uint8_t> sc(scan_temp_bytes(nb));
+    exclusive_scan_u32(opens.p, oscan.p, nb, total.p, sc.p, st);
+    GWN_HIP(hipMemcpyAsync(&S, total.p, 4, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  }
+  const int64_t ncells = 1 + 8 * (int64_t)S;
+  c->ncells = ncells;
+  c->CH.alloc(8 * (size_t)ncells);
+  c->CN.alloc(3 * (size_t)ncells);
+  c->Wd.alloc(ncells);
+  c->cstart.alloc(ncells);
+  c->ccount.alloc(ncells);
+  hipLaunchKernelGGL(k_fill_i32, dim3(div_up(8 * ncells, 256)), dim3(256), 0, st, c->CH.p, 8 * ncells, -1);
+  check_launch("k_fill_i32");
+  {
+    const int64_t s0 = 0;
+    const int32_t cnt0 = (int32_t)n;
+    GWN_HIP(hipMemcpyAsync(c->CN.p, ctr, sizeof(ctr), hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->Wd.p, &w0, 4, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->cstart.p, &s0, 8, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->ccount.p, &cnt0, 4, hipMemcpyHostToDevice, st));
+  }
+  // level-synchronous split; all split cells kept, level by level
+  DevBuf<SplitCell> all(S > 0 ? S : 1);
+  DevBuf<uint32_t> cntd(1), errd(1);
+  GWN_HIP(hipMemsetAsync(errd.p, 0, 4, st));
+  std::vector<uint32_t> lvl_off{0};
+  if (S > 0) {
+    SplitCell root{0, 0, 0, n, 0, 0};
+    GWN_HIP(hipMemcpyAsync(all.p, &root, sizeof(root), hipMemcpyHostToDevice, st));
+    lvl_off.push_back(1);
+    for (int d = 0; d <= OCT_MAX_SPLIT_DEPTH; ++d) {
+      const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+      if (e == b) break;
+      GWN_CHECK(e <= S, GWN_ERR_INVALID, "octree split count mismatch");
+      GWN_HIP(hipMemsetAsync(cntd.p, 0, 4, st));
+      hipLaunchKernelGGL(k_oct_level, dim3(div_up(e - b, 128)), dim3(128), 0, st, all.p + b, e - b, HS, LS, lcp.p,
+                         oscan.p, c->CH.p, c->CN.p, c->Wd.p, c->cstart.p, c->ccount.p, all.p + e, cntd.p, S - e,
+                         ncells, errd.p);
+      check_launch("k_oct_level");
+      uint32_t nn = 0, ef = 0;
+      GWN_HIP(hipMemcpyAsync(&nn, cntd.p, 4, hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipMemcpyAsync(&ef, errd.p, 4, hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipStreamSynchronize(st));
+      GWN_CHECK(ef == 0, GWN_ERR_INVALID, "octree build: split ranks out of range");
+      GWN_CHECK(e + nn <= S, GWN_ERR_INVALID, "octree split count mismatch");
+      lvl_off.push_back(e + nn);
+    }
+    GWN_CHECK(lvl_off.back() == S, GWN_ERR_INVALID, "octree split cells != scan total");
+  }
+
+  // precompute
+  const int nt = c->nterms;
+  c->CM.alloc(3 * (size_t)ncells);
+  c->R.alloc(ncells);
+  c->EC.alloc((size_t)nt * ncells);
+  {
+    DevBuf<int32_t> small(ncells), large(ncells);
+    DevBuf<uint32_t> ns(2);
+    GWN_HIP(hipMemsetAsync(ns.p, 0, 8, st));
+    hipLaunchKernelGGL(k_cell_lists, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, small.p, ns.p,
+                       large.p, ns.p + 1);
+    check_launch("k_cell_lists");
+    uint32_t cnts[2];
+    GWN_HIP(hipMemcpyAsync(cnts, ns.p, 8, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_cell_empty, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, nt, c->CM.p,
+                       c->R.p, c->EC.p);
+    check_launch("k_cell_empty");
+#define GWN_CELL_LAUNCH(NTV)                                                                                        \
+  do {                                                                                                              \
+    if (cnts[0])                                                                                                    \
+      hipLaunchKernelGGL(k_cell_small<NTV>, dim3(div_up(cnts[0], 128)), dim3(128), 0, st, small.p, cnts[0],        \
+                         c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p, c->EC.p);    \
+    if (cnts[1])                                                                                                    \
+      hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)cnts[1] * 64, 256)), dim3(256), 0, st, large.p,    \
+                         cnts[1], c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
+                         c->EC.p);                                                                                  \
+  } while (0)
+    if (nt == 3)
+      GWN_CELL_LAUNCH(3);
+    else if (nt == 12)
+      GWN_CELL_LAUNCH(12);
+    else
+      GWN_CELL_LAUNCH(39);
+#undef GWN_CELL_LAUNCH
+    check_launch("k_cell_moments");
+  }
+
+  // rope records
+  if (S == 0) {
+    c->nrec = 0;
+    GWN_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  DevBuf<uint32_t> rc(ncells), pos(ncells);
+  const int nlev = (int)lvl_off.size() - 1;
+  for (int d = nlev - 1; d >= 0; --d) {
+    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+    if (e > b)
+      hipLaunchKernelGGL(k_rec_count, dim3(div_up(e - b, 256)), dim3(256), 0, st, all.p + b, e - b, c->CH.p,
+                         c->ccount.p, rc.p);
+  }
+  check_launch("k_rec_count");
+  uint32_t root_rc = 0;
+  GWN_HIP(hipMemcpyAsync(&root_rc, rc.p, 4, hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  c->nrec = root_rc - 1;  // the root itself has no record
+  for (int d = 0; d < nlev; ++d) {
+    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+    if (e > b)
+      hipLaunchKernelGGL(k_rec_pos, dim3(div_up(e - b, 256)), dim3(256), 0, st, all.p + b, e - b, c->CH.p,
+                         c->ccount.p, rc.p, pos.p);
+  }
+  check_launch("k_rec_pos");
+  c->head.alloc(c->nrec);
+  c->meta.alloc(c->nrec);
+  hipLaunchKernelGGL(k_rec_write, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->CH.p, c->ccount.p,
+                     c->cstart.p, rc.p, pos.p, c->CM.p, c->R.p, (uint32_t)c->nrec, c->head.p, c->meta.p);
+  check_launch("k_rec_write");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st) {
+  gwn_cloud* c = const_cast<gwn_cloud*>(cc);
+  if (nq <= 0) return;
+  if (c->n == 0) {
+    GWN_HIP(hipMemsetAsync(WN, 0, sizeof(float) * nq, st));
+    return;
+  }
+  const unsigned g = div_up(nq, 256);
+  if (!(beta > 0)) {
+    ProfileScope prof("cloud_brute", st);
+    hipLaunchKernelGGL(k_cloud_brute, dim3(g), dim3(256), 0, st, c->Ps.p, c->NAs.p, c->n, Q, nq, WN);
+    check_launch("k_cloud_brute");
+    return;
+  }
+  const size_t tb = radix_sort_temp_bytes(nq, false);
+  const size_t need = 2 * ((4 * (size_t)nq + 255) / 256 * 256) + (tb + 255) / 256 * 256;
+  c->work.reserve(need);
+  uint32_t* keys = (uint32_t*)c->work.p;
+  uint32_t* perm = keys + (4 * (size_t)nq + 255) / 256 * 64;
+  void* tmp = (uint8_t*)perm + (4 * (size_t)nq + 255) / 256 * 256;
+  const float* bb = c->bbox;
+  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
+              ez = std::max(bb[5] - bb[2], 1e-30f);
+  hipLaunchKernelGGL(k_cloud_morton_q, dim3(g), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
+                     1024.0f / ey, 1024.0f / ez, keys, perm);
+  check_launch("k_cloud_morton_q");
+  radix_sort_u32(keys, perm, nq, 30, tmp, tb, st);
+  const bool root_leaf = c->nrec == 0;
+  ProfileScope prof("cloud_traverse", st);
+  if (c->nterms == 3)
+    hipLaunchKernelGGL(k_cloud_traverse<3>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
+                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+  else if (c->nterms == 12)
+    hipLaunchKernelGGL(k_cloud_traverse<12>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
+                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+  else
+    hipLaunchKernelGGL(k_cloud_traverse<39>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
+                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+  check_launch("k_cloud_traverse");
+}
+
 }  // namespace gwn

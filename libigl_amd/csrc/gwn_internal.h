@@ -154,14 +154,18 @@ struct gwn_soup {
 };
 
 struct gwn_cloud {
-  int64_t n = 0, ncells = 0;
+  int64_t n = 0, ncells = 0, nrec = 0;
   int order = 2, nterms = 39;
+  float bbox[6] = {0, 0, 0, 0, 0, 0};
   gwn::DevBuf<float> P, N, A;          // n x 3, n x 3, n (original order)
-  gwn::DevBuf<float> Ps, NAs;          // points sorted by octree key: xyz, n*A
-  gwn::DevBuf<int32_t> perm;           // sorted position -> original index
+  gwn::DevBuf<float> Ps, NAs;          // points in octree order: xyz and N*A
+  gwn::DevBuf<int32_t> perm;           // octree position -> original point index
   gwn::DevBuf<int32_t> CH;             // ncells x 8
   gwn::DevBuf<float> CN, Wd;           // ncells x 3, ncells
-  gwn::DevBuf<int64_t> cstart;         // ncells + 1 (into sorted points)
+  gwn::DevBuf<int64_t> cstart;         // ncells: first point (octree order)
+  gwn::DevBuf<int32_t> ccount;         // ncells: number of points
   gwn::DevBuf<float> CM, R, EC;        // ncells x 3, ncells, ncells x nterms
+  gwn::DevBuf<float4> head;            // nrec: CM.xyz, R of each non-empty non-root cell (rope order)
+  gwn::DevBuf<uint2> meta;             // nrec: (skip, cell) or (LEAF|first point, count)
   gwn::DevBuf<uint8_t> work;
 };

@@ -164,11 +164,12 @@ class PointCloudFWN:
         CH = np.empty((m, 8), np.int32)
         CN = np.empty((m, 3), np.float32)
         W = np.empty(m, np.float32)
-        cs = np.empty(m + 1, np.int64)
+        cs = np.empty(m, np.int64)
+        cc = np.empty(m, np.int32)
         pl = np.empty(max(self.n, 1), np.int32)
         check(lib().gwn_cloud_copy_octree(self._h, CH.ctypes.data, CN.ctypes.data, W.ctypes.data, cs.ctypes.data,
-                                          pl.ctypes.data))
-        return CH, CN, W, cs, pl[: self.n]
+                                          cc.ctypes.data, pl.ctypes.data))
+        return CH, CN, W, cs, cc, pl[: self.n]
 
     def expansion(self):
         m = self.ncells()
@@ -190,8 +191,8 @@ class PointCloudFWN:
 def octree(P):
     """igl::octree(P, point_indices, CH, CN, W) computed on the GPU (FP32)."""
     c = PointCloudFWN(P, np.zeros_like(np.asarray(P, dtype=np.float32)), np.zeros(np.asarray(P).shape[0]), 0)
-    CH, CN, W, cs, pl = c.octree()
-    point_indices = [np.sort(pl[cs[i]:cs[i + 1]]).tolist() for i in range(CH.shape[0])]
+    CH, CN, W, cs, cc, pl = c.octree()
+    point_indices = [np.sort(pl[cs[i]:cs[i] + cc[i]]).tolist() for i in range(CH.shape[0])]
     return point_indices, CH, CN, W
 
 

@@ -1,0 +1,64 @@
+"""GPU LBVH build: node array bit-exact against a CPU build from the same
+Morton codes (tests/lbvh_cpu.py), and accuracy of the resulting winding
+numbers against the FP64 exact sum and the reference-topology tree."""
+import numpy as np
+import pytest
+
+import lbvh_cpu
+import oracle_py as O
+from libigl_amd import datagen as D
+from libigl_amd import gwn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("level,nb,seed", [(0, 0, 1), (1, 2, 2), (3, 8, 3), (4, 16, 4)])
+def test_lbvh_nodes_bit_exact_vs_cpu_build(level, nb, seed):
+    V, F = D.bumpy_sphere(level, nb, seed) if nb else D.noisy_sphere(level, 0.01, seed)
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2, gwn.TREE_LBVH)
+    ours = bvh.nodes()
+    cpu = lbvh_cpu.lbvh_nodes(V, F)
+    assert ours.shape == cpu.shape
+    assert np.array_equal(ours, cpu)
+    # every triangle appears exactly once as an item
+    items = ours[(ours != lbvh_cpu.EMPTY) & ((ours & lbvh_cpu.INTERNAL) == 0)]
+    assert np.array_equal(np.sort(items), np.arange(F.shape[0]))
+
+
+def test_lbvh_duplicate_codes_and_nonfinite():
+    # many triangles share a centre (equal Morton codes) and some are dropped
+    V, F = D.noisy_sphere(2, 0.01, 5)
+    F = np.concatenate([F, np.repeat(F[:3], 20, axis=0)]).astype(np.int32)
+    Vn = np.concatenate([V, [[np.nan, 0, 0]]])
+    F = np.concatenate([F, [[V.shape[0], 0, 1]]]).astype(np.int32)
+    bvh = gwn.FastWindingNumberBVH().build(Vn, F, 2, gwn.TREE_LBVH)
+    assert np.array_equal(bvh.nodes(), lbvh_cpu.lbvh_nodes(Vn, F))
+    Q = np.random.default_rng(1).uniform(-1.3, 1.3, (2000, 3))
+    assert np.all(np.isfinite(bvh.query(Q)))
+
+
+def test_lbvh_accuracy_vs_exact_and_reference():
+    V, F = D.bumpy_sphere(4, 16, 9)
+    rng = np.random.default_rng(3)
+    Q = np.concatenate([rng.uniform(-1.4, 1.4, (3000, 3)), D.near_surface_queries(V, F, 3000, 1e-3, rng)])
+    Wx = O.winding_number_bf(V, F, Q)
+    lb = gwn.FastWindingNumberBVH().build(V, F, 2, gwn.TREE_LBVH)
+    sa = gwn.FastWindingNumberBVH().build(V, F, 2, gwn.TREE_SAH4)
+    # all triangles exact: both trees give the FP32 exact sum
+    assert np.abs(lb.query(Q, 1e6) - sa.query(Q, 1e6)).max() < 1e-5
+    for beta in (2.0, 5.0):
+        el = np.abs(lb.query(Q, beta) - Wx).max()
+        es = np.abs(sa.query(Q, beta) - Wx).max()
+        print(f"beta {beta}: LBVH max err {el:.3e}, SAH4 max err {es:.3e}")
+        assert el < 3 * es + 1e-5
+
+
+def test_lbvh_tiny_meshes():
+    V, F = D.noisy_sphere(1, 0.01, 5)
+    Q = np.random.default_rng(2).uniform(-1.4, 1.4, (500, 3))
+    for k in (1, 2, 3, 5, 9):
+        b = gwn.FastWindingNumberBVH().build(V, F[:k].copy(), 2, gwn.TREE_LBVH)
+        assert np.array_equal(b.nodes(), lbvh_cpu.lbvh_nodes(V, F[:k].copy()))
+        W = b.query(Q, 1e6)
+        Wx = O.winding_number_bf(V, F[:k].copy(), Q)
+        assert np.abs(W - Wx).max() < 1e-5
