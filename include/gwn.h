@@ -157,6 +157,47 @@ int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC)
 int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN);
 int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of soup queries against resident meshes (no
+ * counterpart in the reference, whose callers such as
+ * signed_distance.cpp:50-121 query one point at a time on the host).
+ *
+ * A server owns `slots` HIP streams, each served by its own host thread, and a
+ * device-memory budget.  Meshes are built once and stay resident (their
+ * device bytes count against the budget; add_mesh fails with GWN_ERR_NOMEM
+ * when they do not fit).  gwn_server_submit returns at once; a request is
+ * admitted, first come first served, when a slot is idle and its device
+ * footprint (queries, results, sort workspace) fits in the remaining budget.
+ * Admitted requests run concurrently on different streams, so their kernels
+ * overlap on the device.  A request's result is written to the caller's W
+ * (which must stay valid until the request is done) and is bit-identical to
+ * gwn_soup_query on the same mesh and queries.  Timestamps are seconds on a
+ * steady clock shared by all requests of a server.
+ */
+typedef struct gwn_server gwn_server;
+typedef struct {
+  double t_submit, t_start, t_done; /* arrival, admission, result in host memory */
+  int64_t num_queries;
+  int32_t mesh, status;             /* status: GWN_OK or the request's error */
+} gwn_request_info;
+
+int gwn_server_create(int slots, int64_t device_budget_bytes, gwn_server** out);
+void gwn_server_destroy(gwn_server* s);
+/* builds the mesh's tree (and FP64 moments) and keeps it resident */
+int gwn_server_add_mesh(gwn_server* s, const double* V, int64_t nv, const int32_t* F, int64_t nf, int order,
+                        int tree_kind, int32_t* mesh_id);
+int gwn_server_submit(gwn_server* s, int32_t mesh_id, const double* Q, int64_t nq, float accuracy_scale,
+                      int precision, double* W, int64_t* request_id);
+/* blocks until the request is done; returns its status */
+int gwn_server_wait(gwn_server* s, int64_t request_id);
+/* blocks until every submitted request is done */
+int gwn_server_drain(gwn_server* s);
+int gwn_server_request_info(const gwn_server* s, int64_t request_id, gwn_request_info* info);
+/* device bytes: resident meshes, slot buffers in use, budget */
+int gwn_server_memory(const gwn_server* s, int64_t* resident, int64_t* in_flight, int64_t* budget);
+/* seconds on the server clock (for callers that time arrivals) */
+double gwn_server_now(const gwn_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,11 @@ class MixedParams(C.Structure):
     _fields_ = [("near_dist", C.c_float), ("ambiguous_band", C.c_float)]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("num_queries", C.c_int64), ("mesh", C.c_int32), ("status", C.c_int32)]
+
+
 # name -> (restype, argtypes); mirrors include/gwn.h
 SIGNATURES = {
     "gwn_last_error": (C.c_char_p, []),
@@ -60,6 +65,16 @@ SIGNATURES = {
     "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
     "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
     "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),
+    "gwn_server_create": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
+    "gwn_server_destroy": (None, [vp]),
+    "gwn_server_add_mesh": (C.c_int, [vp, f64p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int32)]),
+    "gwn_server_submit": (C.c_int, [vp, C.c_int32, vp, C.c_int64, C.c_float, C.c_int, vp, C.POINTER(C.c_int64)]),
+    "gwn_server_wait": (C.c_int, [vp, C.c_int64]),
+    "gwn_server_drain": (C.c_int, [vp]),
+    "gwn_server_request_info": (C.c_int, [vp, C.c_int64, C.POINTER(RequestInfo)]),
+    "gwn_server_memory": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "gwn_server_now": (C.c_double, [vp]),
 }
 
 _lib = None

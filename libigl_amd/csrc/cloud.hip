@@ -172,6 +172,11 @@ __global__ void k_oct_level(const SplitCell* __restrict__ cur, uint32_t ncur, co
   }
 }
 
+__global__ void k_split_counts(const SplitCell* __restrict__ lvl, uint32_t m, uint32_t* __restrict__ cnt) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) cnt[j] = (uint32_t)(lvl[j].end - lvl[j].start);
+}
+
 __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -192,6 +197,42 @@ __global__ void k_permute_points(const float* __restrict__ P, const float* __res
 }
 
 // ---------------- precompute ----------------
+// Centre of mass of split cells, summed as the reference sums it: FP32,
+// sequentially, over the cell's points in ascending index order
+// (fast_winding_number.cpp:60-73).  Per level, the (cell, point) pairs are
+// sorted by (cell ordinal, point index) and one thread folds each cell.
+__global__ void k_lvl_emit(const SplitCell* __restrict__ lvl, const uint32_t* __restrict__ off,
+                           const int32_t* __restrict__ perm, uint64_t* __restrict__ keys) {
+  const uint32_t j = blockIdx.x;
+  const SplitCell sc = lvl[j];
+  const int64_t cnt = sc.end - sc.start;
+  const uint32_t o = off[j];
+  for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x)
+    keys[o + t] = ((uint64_t)j << 32) | (uint32_t)perm[sc.start + t];
+}
+
+__global__ void k_lvl_fold(const SplitCell* __restrict__ lvl, uint32_t ncell, const uint32_t* __restrict__ off,
+                           const uint64_t* __restrict__ keys, const float* __restrict__ P,
+                           const float* __restrict__ A, float* __restrict__ CM) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncell) return;
+  const SplitCell sc = lvl[j];
+  const int64_t cnt = sc.end - sc.start;
+  const uint32_t o = off[j];
+  float at = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+  for (int64_t t = 0; t < cnt; ++t) {
+    const int64_t i = (int64_t)(keys[o + t] & 0xFFFFFFFFull);
+    const float a = A[i];
+    at += a;
+    m0 += a * P[3 * i];
+    m1 += a * P[3 * i + 1];
+    m2 += a * P[3 * i + 2];
+  }
+  CM[3 * (int64_t)sc.cell] = m0 / at;
+  CM[3 * (int64_t)sc.cell + 1] = m1 / at;
+  CM[3 * (int64_t)sc.cell + 2] = m2 / at;
+}
+
 struct CellSums {
   double area, ap[3], an[3];
 };
@@ -199,20 +240,21 @@ struct CellSums {
 template <int NT>
 __device__ __forceinline__ void cell_moments_serial(const float* __restrict__ P, const float* __restrict__ N,
                                                     const float* __restrict__ A, const int32_t* __restrict__ perm,
-                                                    int64_t s, int32_t cnt, float* __restrict__ cm,
+                                                    int64_t s, int32_t cnt, bool split, float* __restrict__ cm,
                                                     float* __restrict__ r, float* __restrict__ ec) {
-  double area = 0, ap[3] = {0, 0, 0}, an[3] = {0, 0, 0};
+  double an[3] = {0, 0, 0};
+  float at = 0.0f, mc[3] = {0.0f, 0.0f, 0.0f};
   for (int32_t j = 0; j < cnt; ++j) {
     const int64_t p = perm[s + j];
-    const double a = A[p];
-    area += a;
+    const float af = A[p];
+    at += af;
     for (int d = 0; d < 3; ++d) {
-      ap[d] += a * (double)P[3 * p + d];
-      an[d] += a * (double)N[3 * p + d];
+      mc[d] += af * P[3 * p + d];
+      an[d] += (double)af * (double)N[3 * p + d];
     }
   }
   float c[3];
-  for (int d = 0; d < 3; ++d) c[d] = (float)(ap[d] / area);
+  for (int d = 0; d < 3; ++d) c[d] = split ? cm[d] : mc[d] / at;
   double e[NT > 3 ? NT - 3 : 1];
   for (int k = 0; k < NT - 3; ++k) e[k] = 0;
   float rmax = 0.0f;
@@ -244,7 +286,8 @@ __device__ __forceinline__ void cell_moments_serial(const float* __restrict__ P,
 
 // one thread per small cell
 template <int NT>
-__global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, const int64_t* __restrict__ cstart,
+__global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, const int32_t* __restrict__ CH,
+                             const int64_t* __restrict__ cstart,
                              const int32_t* __restrict__ ccount, const float* __restrict__ P,
                              const float* __restrict__ N, const float* __restrict__ A,
                              const int32_t* __restrict__ perm, float* __restrict__ CM, float* __restrict__ R,
@@ -252,7 +295,8 @@ __global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, 
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
   const int32_t c = cells[t];
-  cell_moments_serial<NT>(P, N, A, perm, cstart[c], ccount[c], CM + 3 * (int64_t)c, R + c, EC + (int64_t)NT * c);
+  cell_moments_serial<NT>(P, N, A, perm, cstart[c], ccount[c], CH[8 * (int64_t)c] >= 0, CM + 3 * (int64_t)c, R + c,
+                          EC + (int64_t)NT * c);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -267,7 +311,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // one wavefront per large cell
 template <int NT>
 __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ cells, uint32_t count,
-                                                    const int64_t* __restrict__ cstart,
+                                                    const int32_t* __restrict__ CH, const int64_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ ccount, const float* __restrict__ P,
                                                     const float* __restrict__ N, const float* __restrict__ A,
                                                     const int32_t* __restrict__ perm, float* __restrict__ CM,
@@ -278,26 +322,35 @@ __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ 
   const int32_t c = cells[w];
   const int64_t s = cstart[c];
   const int32_t cnt = ccount[c];
-  double area = 0, ap0 = 0, ap1 = 0, ap2 = 0, an0 = 0, an1 = 0, an2 = 0;
+  // split cells take their centre from k_lvl_fold; an unsplittable leaf (more
+  // than 32 points with equal 42-level keys) sums it here, serially, in
+  // index order (its points are index-ordered)
+  float lc[3] = {0.0f, 0.0f, 0.0f};
+  const bool split = CH[8 * (int64_t)c] >= 0;
+  if (!split && lane == 0) {
+    float at = 0.0f;
+    for (int32_t j = 0; j < cnt; ++j) {
+      const int64_t p = perm[s + j];
+      const float af = A[p];
+      at += af;
+      for (int d = 0; d < 3; ++d) lc[d] += af * P[3 * p + d];
+    }
+    for (int d = 0; d < 3; ++d) lc[d] = lc[d] / at;
+  }
+  for (int d = 0; d < 3; ++d) lc[d] = __shfl(lc[d], 0, 64);
+  double an0 = 0, an1 = 0, an2 = 0;
   for (int32_t j = lane; j < cnt; j += 64) {
     const int64_t p = perm[s + j];
     const double a = A[p];
-    area += a;
-    ap0 += a * (double)P[3 * p];
-    ap1 += a * (double)P[3 * p + 1];
-    ap2 += a * (double)P[3 * p + 2];
     an0 += a * (double)N[3 * p];
     an1 += a * (double)N[3 * p + 1];
     an2 += a * (double)N[3 * p + 2];
   }
-  area = wave_sum(area);
-  ap0 = wave_sum(ap0);
-  ap1 = wave_sum(ap1);
-  ap2 = wave_sum(ap2);
   an0 = wave_sum(an0);
   an1 = wave_sum(an1);
   an2 = wave_sum(an2);
-  const float c3[3] = {(float)(ap0 / area), (float)(ap1 / area), (float)(ap2 / area)};
+  const float c3[3] = {split ? CM[3 * (int64_t)c] : lc[0], split ? CM[3 * (int64_t)c + 1] : lc[1],
+                       split ? CM[3 * (int64_t)c + 2] : lc[2]};
   double e[NT > 3 ? NT - 3 : 1];
   for (int k = 0; k < NT - 3; ++k) e[k] = 0;
   float rmax = 0.0f;
@@ -322,7 +375,8 @@ __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ 
   rmax = wave_max(rmax);
   for (int k = 0; k < NT - 3; ++k) e[k] = wave_sum(e[k]);
   if (lane == 0) {
-    for (int d = 0; d < 3; ++d) CM[3 * (int64_t)c + d] = c3[d];
+    if (!split)
+      for (int d = 0; d < 3; ++d) CM[3 * (int64_t)c + d] = c3[d];
     R[c] = rmax;
     float* ec = EC + (int64_t)NT * c;
     ec[0] = (float)an0;
@@ -702,14 +756,42 @@ void cloud_build_device(gwn_cloud* c, hipStream_t st) {
     hipLaunchKernelGGL(k_cell_empty, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, nt, c->CM.p,
                        c->R.p, c->EC.p);
     check_launch("k_cell_empty");
+    // split-cell centres, level by level
+    for (size_t d = 0; d + 1 < lvl_off.size(); ++d) {
+      const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+      if (e == b) continue;
+      const uint32_t m = e - b;
+      DevBuf<uint32_t> cnt(m), off(m), tot(1);
+      hipLaunchKernelGGL(k_split_counts, dim3(div_up(m, 256)), dim3(256), 0, st, all.p + b, m, cnt.p);
+      check_launch("k_split_counts");
+      DevBuf<uint8_t> sct(scan_temp_bytes(m));
+      exclusive_scan_u32(cnt.p, off.p, m, tot.p, sct.p, st);
+      uint32_t T = 0;
+      GWN_HIP(hipMemcpyAsync(&T, tot.p, 4, hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipStreamSynchronize(st));
+      DevBuf<uint64_t> keys(T);
+      DevBuf<uint32_t> dummy(T);
+      hipLaunchKernelGGL(k_lvl_emit, dim3(m), dim3(64), 0, st, all.p + b, off.p, c->perm.p, keys.p);
+      check_launch("k_lvl_emit");
+      int obits = 1;
+      while ((1u << obits) < m) ++obits;
+      const size_t tb = radix_sort_temp_bytes(T, true);
+      DevBuf<uint8_t> tmp(tb);
+      radix_sort_u64(keys.p, dummy.p, T, 32 + obits, tmp.p, tb, st);
+      hipLaunchKernelGGL(k_lvl_fold, dim3(div_up(m, 64)), dim3(64), 0, st, all.p + b, m, off.p, keys.p, c->P.p,
+                         c->A.p, c->CM.p);
+      check_launch("k_lvl_fold");
+      GWN_HIP(hipStreamSynchronize(st));
+    }
 #define GWN_CELL_LAUNCH(NTV)                                                                                        \
   do {                                                                                                              \
     if (cnts[0])                                                                                                    \
       hipLaunchKernelGGL(k_cell_small<NTV>, dim3(div_up(cnts[0], 128)), dim3(128), 0, st, small.p, cnts[0],        \
-                         c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p, c->EC.p);    \
+                         c->CH.p, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
+                         c->EC.p);                                                                                  \
     if (cnts[1])                                                                                                    \
       hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)cnts[1] * 64, 256)), dim3(256), 0, st, large.p,    \
-                         cnts[1], c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
+                         cnts[1], c->CH.p, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
                          c->EC.p);                                                                                  \
   } while (0)
     if (nt == 3)

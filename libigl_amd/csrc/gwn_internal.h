@@ -9,6 +9,8 @@
 
 #include "../../include/gwn.h"
 
+struct gwn_soup;
+
 namespace gwn {
 
 // ---- error reporting (thread-local last error, returned as gwn_status) ----
@@ -131,6 +133,12 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* 
 // tri_boxes: nf x 6 floats on device. Writes nodes (device, 4 per node) and returns count.
 int64_t lbvh_build(const float* tri_boxes_dev, int64_t nf, const float* bbox_host, DevBuf<uint32_t>& nodes,
                    hipStream_t st);
+
+// soup queries (soup.hip)
+size_t soup_query_work_bytes(int64_t nq);
+void soup_ensure_fp64(gwn_soup* s, hipStream_t st);
+void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
 
 }  // namespace gwn
 

@@ -518,23 +518,37 @@ void soup_setup(gwn_soup* s, hipStream_t st) {
 }
 
 // Workspace for one query call: Morton keys/permutation + sort temp + ambiguity list.
-static size_t query_work_bytes(int64_t nq) {
+size_t soup_query_work_bytes(int64_t nq) {
   const size_t a = 256;
   auto up = [&](size_t b) { return (b + a - 1) / a * a; };
   return up(4 * (size_t)nq) * 2 + up(radix_sort_temp_bytes(nq, false)) + up(4 * (size_t)nq) + up(4);
 }
 
+void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
+
 void soup_query_device(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
                        double* W, hipStream_t st) {
+  gwn_soup* s = const_cast<gwn_soup*>(sc);
+  if (nq <= 0) return;
+  const size_t need = soup_query_work_bytes(nq);
+  s->work.reserve(need);
+  soup_query_device_ws(sc, Q, nq, accuracy_scale, precision, W, st, s->work.p, s->work.bytes());
+}
+
+// Query with caller-provided device workspace (soup_query_work_bytes(nq)
+// bytes); calls with distinct workspaces may run concurrently on distinct
+// streams against the same tree once its FP64 moments exist.
+void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes) {
   gwn_soup* s = const_cast<gwn_soup*>(sc);
   if (nq <= 0) return;
   if (s->nslots == 0) {
     GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * nq, st));
     return;
   }
-  const size_t need = query_work_bytes(nq);
-  s->work.reserve(need);
-  uint8_t* w = s->work.p;
+  GWN_CHECK(ws_bytes >= soup_query_work_bytes(nq), GWN_ERR_INVALID, "query workspace too small");
+  uint8_t* w = workspace;
   auto take = [&](size_t b) {
     uint8_t* p = w;
     w += (b + 255) / 256 * 256;

@@ -104,17 +104,19 @@ def uv_sphere(ntris_target: int, nbumps: int = 8, seed: int = 3):
     ring = np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], -1).reshape(-1, 3)
     V = np.concatenate([[[0, 0, 1.0]], ring, [[0, 0, -1.0]]], axis=0)
     top, bot = 0, V.shape[0] - 1
-    idx = lambda i, j: 1 + i * nlon + (j % nlon)
-    F = []
-    for j in range(nlon):
-        F.append((top, idx(0, j), idx(0, j + 1)))
-    for i in range(nlat - 2):
-        for j in range(nlon):
-            a, b, c, d = idx(i, j), idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)
-            F.append((a, b, c))
-            F.append((a, c, d))
-    for j in range(nlon):
-        F.append((bot, idx(nlat - 2, j + 1), idx(nlat - 2, j)))
+    j = np.arange(nlon)
+    jn = (j + 1) % nlon
+    caps_top = np.stack([np.full(nlon, top), 1 + j, 1 + jn], 1)
+    i = np.arange(nlat - 2)[:, None]
+    a = 1 + i * nlon + j[None, :]
+    b = 1 + (i + 1) * nlon + j[None, :]
+    c = 1 + (i + 1) * nlon + jn[None, :]
+    d = 1 + i * nlon + jn[None, :]
+    quads = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 1, 3), np.stack([a, c, d], -1).reshape(-1, 1, 3)],
+                           axis=1).reshape(-1, 3)
+    last = 1 + (nlat - 2) * nlon
+    caps_bot = np.stack([np.full(nlon, bot), last + jn, last + j], 1)
+    F = np.concatenate([caps_top, quads, caps_bot], 0)
     F = np.asarray(F, dtype=np.int32)
     rng = np.random.default_rng(seed)
     return _bump(V, nbumps, rng), F

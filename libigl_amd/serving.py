@@ -1,0 +1,126 @@
+"""Python handle on the native continuous-batching server (gwn_server_*)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+from ._native import check, lib
+
+
+class Server:
+    """Resident meshes + continuous batching over `slots` HIP streams within a
+    device-memory budget.  submit() returns at once; the result array is
+    filled when wait()/drain() says the request is done."""
+
+    def __init__(self, slots=4, device_budget_bytes=32 << 30):
+        h = nat.vp()
+        check(lib().gwn_server_create(slots, int(device_budget_bytes), C.byref(h)))
+        self._h = h.value
+        self._keep = {}  # request id -> (Q, W) kept alive until done
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().gwn_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_mesh(self, V, F, order=2, tree=nat.TREE_SAH4):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.int32)
+        mid = C.c_int32()
+        check(lib().gwn_server_add_mesh(self._h, V, V.shape[0], F, F.shape[0], order, tree, C.byref(mid)))
+        return mid.value
+
+    def submit(self, mesh, Q, accuracy_scale=2.0, precision=nat.PREC_FP32):
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
+        W = np.empty(Q.shape[0], np.float64)
+        rid = C.c_int64()
+        check(lib().gwn_server_submit(self._h, mesh, Q.ctypes.data, Q.shape[0], float(accuracy_scale), precision,
+                                      W.ctypes.data, C.byref(rid)))
+        self._keep[rid.value] = (Q, W)
+        return rid.value
+
+    def wait(self, rid):
+        rc = lib().gwn_server_wait(self._h, rid)
+        Q, W = self._keep.pop(rid)
+        check(rc)
+        return W
+
+    def result(self, rid):
+        return self._keep[rid][1]
+
+    def drain(self):
+        check(lib().gwn_server_drain(self._h))
+
+    def info(self, rid):
+        i = nat.RequestInfo()
+        check(lib().gwn_server_request_info(self._h, rid, C.byref(i)))
+        return i
+
+    def memory(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().gwn_server_memory(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def now(self):
+        return lib().gwn_server_now(self._h)
+
+
+def poisson_trace(server, mesh_ids, batches, lam, duration, seed=0, accuracy_scale=2.0,
+                  precision=nat.PREC_FP32, nq_lo=1_000, nq_hi=1_000_000, max_requests=None):
+    """Replays a Poisson arrival trace against a server.
+
+    Requests arrive at rate `lam` per second for `duration` seconds; each
+    names a random mesh and carries a batch of size log-uniform in
+    [nq_lo, nq_hi], a window of that mesh's pre-generated queries (batches[m],
+    uniform in its bbox).  Submission happens at the arrival time on the
+    server clock; latency is arrival -> result in host memory.
+    Returns a dict of throughput and latency percentiles."""
+    import time
+
+    rng = np.random.default_rng(seed)
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if t >= duration or (max_requests and len(arrivals) >= max_requests):
+            break
+        arrivals.append(t)
+    n = len(arrivals)
+    meshes = rng.integers(0, len(mesh_ids), size=n)
+    sizes = np.exp(rng.uniform(np.log(nq_lo), np.log(nq_hi), size=n)).astype(np.int64)
+    starts = [int(rng.integers(0, batches[m].shape[0] - s + 1)) for m, s in zip(meshes, sizes)]
+    rids = []
+    t0 = server.now() + 0.01
+    late = 0.0
+    for k in range(n):
+        target = t0 + arrivals[k]
+        while True:
+            d = target - server.now()
+            if d <= 0:
+                late = max(late, -d)
+                break
+            time.sleep(min(d, 0.002))
+        m = meshes[k]
+        Q = batches[m][starts[k]: starts[k] + sizes[k]]
+        rids.append(server.submit(mesh_ids[m], Q, accuracy_scale, precision))
+    server.drain()
+    infos = [server.info(r) for r in rids]
+    for r in rids:
+        server.wait(r)
+    lat = np.array([i.t_done - i.t_submit for i in infos])
+    q = np.array([i.num_queries for i in infos], dtype=np.float64)
+    span = max(i.t_done for i in infos) - min(i.t_submit for i in infos)
+    ok = all(i.status == 0 for i in infos)
+    return {"requests": n, "queries": int(q.sum()), "span_s": span, "queries_per_s": q.sum() / span,
+            "offered_queries_per_s": q.sum() / duration, "p50_ms": 1e3 * float(np.quantile(lat, 0.5)),
+            "p99_ms": 1e3 * float(np.quantile(lat, 0.99)), "max_ms": 1e3 * float(lat.max()),
+            "submit_lag_ms": 1e3 * late, "all_ok": ok, "lambda": lam}
