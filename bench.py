@@ -47,7 +47,24 @@ def main():
     ap.add_argument("--nq", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--beta", type=float, default=0.0,
+                    help="skip the accuracy sweep and time this accuracy_scale only (profiling runs)")
+    ap.add_argument("--tree", choices=["sah4", "lbvh"], default="sah4")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 1.3M-triangle mixed vs FP64 comparison")
+    ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes": 4 * n, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries would shard by "
                                    "Morton range with a replicated tree; see DESIGN.md)"}))
@@ -68,7 +85,8 @@ def main():
     bvh = gwn.FastWindingNumberBVH()
     torch.cuda.synchronize()
     tb = time.perf_counter()
-    bvh.build(V, F, 2, gwn.TREE_SAH4)
+    tree = gwn.TREE_SAH4 if args.tree == "sah4" else gwn.TREE_LBVH
+    bvh.build(V, F, 2, tree)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - tb
     info = bvh.info()
@@ -82,6 +100,19 @@ def main():
 
     def run(beta, prec=gwn.PREC_FP32):
         bvh.query_device(dQ.data_ptr(), nq, dW.data_ptr(), beta, prec, stream.cuda_stream)
+
+    if args.beta > 0:  # profiling mode: only the timed steps at one beta
+        for _ in range(args.warmup):
+            run(args.beta)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            run(args.beta)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        print(json.dumps({"mode": "profile", "beta": args.beta, "tree": args.tree, "ms_per_step": 1e3 * dt,
+                          "queries_per_s": nq / dt}), flush=True)
+        return
 
     # FP64 exact reference on the GPU for all queries
     # the oracle sees the original FP64 mesh and queries (the FWN path narrows both to float)
@@ -218,6 +249,20 @@ def main():
         except Exception as e:
             log(f"[bench] cpu baseline skipped: {e}")
 
+    mixed_cfg4 = None
+    if not args.no_mixed:
+        try:
+            mixed_cfg4 = run_mixed_workload(BETA_GRID)
+        except Exception as e:
+            log(f"[bench] mixed workload skipped: {e}")
+
+    serving = None
+    if args.serving_duration > 0:
+        try:
+            serving = run_serving(args.serving_duration, beta_acc)
+        except Exception as e:
+            log(f"[bench] serving skipped: {e}")
+
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -233,9 +278,108 @@ def main():
         "fp64": {"queries_per_s": round(nq / (dtd / max(3, args.steps // 4)), 1), "max_abs_dw": err_fp64,
                  "kernel_ms": round(kd_ms, 4)},
         "build_s": round(build_s, 4), "sort_ms": round(sort_ms, 4),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "mixed_vs_fp64": mixed_cfg4,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_mixed_workload(beta_grid):
+    """Mixed-precision vs all-FP64 soup queries on the level-8 icosphere
+    (1,310,720 triangles, 64 bumps), 10M queries (80% uniform, 20% within 1e-4
+    of the surface).  |dw| against the FP64 exact sum on a 100k-query sample."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+    from libigl_amd import _native as nat
+
+    t0 = time.time()
+    V, F, Q = D.mixed_workload()
+    nq = Q.shape[0]
+    bvh = gwn.FastWindingNumberBVH()
+    tb = time.perf_counter()
+    bvh.build(V, F, 2, gwn.TREE_SAH4)
+    build_s = time.perf_counter() - tb
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev)
+    dQ = torch.from_numpy(Q.astype(np.float32)).to(dev)
+    dW = torch.empty(nq, dtype=torch.float64, device=dev)
+    sub = np.arange(0, nq, nq // 100_000)
+    dV = torch.from_numpy(np.ascontiguousarray(V, dtype=np.float64)).to(dev)
+    dF = torch.from_numpy(F).to(dev)
+    dQs = torch.from_numpy(np.ascontiguousarray(Q[sub], dtype=np.float64)).to(dev)
+    dWx = torch.empty(sub.size, dtype=torch.float64, device=dev)
+    nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr(), V.shape[0], dF.data_ptr(), F.shape[0],
+                                                       dQs.data_ptr(), sub.size, dWx.data_ptr(), st.cuda_stream))
+    torch.cuda.synchronize()
+    Wx = dWx.cpu().numpy()
+    log(f"[bench] mixed workload: {F.shape[0]} tris, {nq} queries, build {build_s:.2f}s, setup {time.time() - t0:.1f}s")
+
+    def run(beta, prec):
+        bvh.query_device(dQ.data_ptr(), nq, dW.data_ptr(), beta, prec, st.cuda_stream)
+
+    def err(beta, prec):
+        run(beta, prec)
+        torch.cuda.synchronize()
+        e = np.abs(dW.cpu().numpy()[sub] - Wx)
+        return float(e.max()), float(np.quantile(e, 0.99))
+
+    beta = beta_grid[-1]
+    for b in beta_grid:
+        if err(b, gwn.PREC_MIXED)[0] <= ACC_TARGET:
+            beta = b
+            break
+
+    def rate(prec, steps):
+        run(beta, prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            run(beta, prec)
+        torch.cuda.synchronize()
+        return nq * steps / (time.perf_counter() - t)
+
+    out = {"workload": "level-8 icosphere 1,310,720 tris, 64 bumps, 10M queries (80% uniform, 20% within 1e-4)",
+           "accuracy_scale": beta, "build_s": round(build_s, 3), "error_sample": int(sub.size)}
+    for name, prec in (("fp32", gwn.PREC_FP32), ("mixed", gwn.PREC_MIXED), ("fp64", gwn.PREC_FP64)):
+        mx, p99 = err(beta, prec)
+        out[name] = {"queries_per_s": round(rate(prec, 3), 1), "max_abs_dw": mx, "p99_abs_dw": p99}
+        log(f"[bench] cfg4 {name}: {out[name]}")
+    out["mixed_over_fp64"] = round(out["mixed"]["queries_per_s"] / out["fp64"]["queries_per_s"], 3)
+    del dQ, dW, dV, dF, dQs, dWx
+    return out
+
+
+def run_serving(duration, beta_acc):
+    """64 resident meshes (log-uniform 1k-500k triangles), Poisson arrivals at
+    2000/s, batches log-uniform in [1k, 1M] uniform in the mesh's bbox; mixed
+    precision.  Latency = arrival -> result in host memory."""
+    import numpy as np
+
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+    from libigl_amd.serving import Server, poisson_trace
+
+    t0 = time.time()
+    meshes = D.serving_meshes()
+    srv = Server(slots=8, device_budget_bytes=64 << 30)
+    ids = [srv.add_mesh(V, F, 2, gwn.TREE_SAH4) for V, F in meshes]
+    rng = np.random.default_rng(77)
+    batches = [D.uniform_queries(V.min(0), V.max(0), 1_000_000, rng) for V, _ in meshes]
+    ntris = sum(F.shape[0] for _, F in meshes)
+    log(f"[bench] serving: 64 meshes, {ntris} tris resident, setup {time.time() - t0:.1f}s")
+    res = {}
+    for beta in (2.0, beta_acc):
+        r = poisson_trace(srv, ids, batches, lam=2000.0, duration=duration, seed=9, accuracy_scale=beta,
+                          precision=gwn.PREC_MIXED)
+        log(f"[bench] serving beta {beta}: {r}")
+        res[f"beta_{beta}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+    resident, held, budget = srv.memory()
+    srv.close()
+    res.update({"meshes": 64, "resident_triangles": int(ntris), "resident_bytes": int(resident), "slots": 8,
+                "lambda_per_s": 2000.0, "duration_s": duration, "precision": "mixed"})
+    return res
 
 
 if __name__ == "__main__":

@@ -513,7 +513,7 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
                                                         int64_t npts, bool root_leaf, const float* __restrict__ Q,
                                                         const uint32_t* __restrict__ perm, int64_t nq, double beta,
                                                         float* __restrict__ WN) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];

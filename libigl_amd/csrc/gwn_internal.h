@@ -106,6 +106,17 @@ class ProfileScope {
   bool active_ = false;
 };
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by
+// blockIdx.x.  xcd_block() renumbers them so that XCD x runs a contiguous range
+// of logical blocks: with queries in Morton order each XCD then walks one
+// spatial region and its own L2 holds that region's part of the tree.
+__device__ __forceinline__ uint32_t xcd_block() {
+  constexpr uint32_t kXcd = 8;
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb / kXcd, r = nb % kXcd, x = b % kXcd, k = b / kXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // ---- tree encoding shared with the reference (FastWindingNumberForSoups.h:4217-4232) ----
 constexpr uint32_t NODE_EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t NODE_INTERNAL = 0x80000000u;

@@ -40,6 +40,27 @@ struct Request {
 
 size_t footprint(int64_t nq) { return (size_t)nq * (12 + 8) + soup_query_work_bytes(nq) + 3 * 256; }
 
+// Host-side copies of large batches (narrowing into pinned memory, results
+// back to the caller) split over a few threads.
+template <typename Fn>
+void host_parallel(int64_t n, Fn fn) {
+  constexpr int64_t kMin = 1 << 17;
+  constexpr int kThreads = 4;
+  if (n < kMin) {
+    fn(0, n);
+    return;
+  }
+  std::thread th[kThreads - 1];
+  const int64_t chunk = (n + kThreads - 1) / kThreads;
+  for (int t = 1; t < kThreads; ++t) {
+    const int64_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b < e) th[t - 1] = std::thread(fn, b, e);
+  }
+  fn(0, std::min(n, chunk));
+  for (auto& x : th)
+    if (x.joinable()) x.join();
+}
+
 struct Slot {
   hipStream_t stream = nullptr;
   DevBuf<uint8_t> dev;  // dQ | dW | workspace
@@ -131,7 +152,11 @@ void gwn_server::worker(gwn::Slot* sl) {
         sl->host_cap = (size_t)r->nq;
       }
       // fast_winding_number.cpp:431-434: queries narrowed to float
-      for (int64_t i = 0; i < 3 * r->nq; ++i) sl->hQ[i] = (float)r->Q[i];
+      float* hQ = sl->hQ;
+      const double* Qs = r->Q;
+      host_parallel(3 * r->nq, [hQ, Qs](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) hQ[i] = (float)Qs[i];
+      });
       uint8_t* base = sl->dev.p;
       auto a256 = [](size_t b) { return (b + 255) / 256 * 256; };
       float* dQ = reinterpret_cast<float*>(base);
@@ -142,7 +167,9 @@ void gwn_server::worker(gwn::Slot* sl) {
       soup_query_device_ws(m, dQ, r->nq, r->beta, r->prec, dW, sl->stream, ws, wsb);
       GWN_HIP(hipMemcpyAsync(sl->hW, dW, 8 * (size_t)r->nq, hipMemcpyDeviceToHost, sl->stream));
       GWN_HIP(hipStreamSynchronize(sl->stream));
-      std::memcpy(r->W, sl->hW, 8 * (size_t)r->nq);
+      double* Wd = r->W;
+      const double* hW = sl->hW;
+      host_parallel(r->nq, [Wd, hW](int64_t b, int64_t e) { std::memcpy(Wd + b, hW + b, 8 * (size_t)(e - b)); });
     } catch (const Error& e) {
       status = e.code;
       err = e.msg;
