@@ -33,6 +33,11 @@ BETA_GRID = [2.0, 3.0, 3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 7.0, 8.0]
 FLOP_TEST = 10    # |q-P|^2 <= maxP2 * beta^2 per child slot (7179-7191)
 FLOP_FAR = 83     # accepted order-2 far-field slot (7202-7241)
 FLOP_EXACT = 76   # exact triangle (6078-6109)
+# HBM bytes moved per k_traverse launch on the default workload (1M queries,
+# SAH4 tree), keyed by accuracy_scale: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
+# for a 256 MiB device copy (fetch x2.000, write x1.000).
+MEASURED_TRAFFIC = {5.0: int((155.0 + 33.3) * 2**20)}
 
 
 def log(*a):
@@ -211,7 +216,9 @@ def main():
     if flops_per_q is not None and k_ms > 0:
         achieved = flops_per_q * nq / (k_ms * 1e-3) / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
-                    "frac": round(achieved / 157.3, 4), "traffic": None,
+                    "frac": round(achieved / 157.3, 4),
+                    "traffic": (MEASURED_TRAFFIC.get(beta_acc)
+                                if (nq == 1_000_000 and args.tree == "sah4") else None),
                     "kernel": "k_traverse<2,false>", "kernel_ms": round(k_ms, 4),
                     "flop_per_query": round(flops_per_q, 1),
                     "note": "FP32 VALU; no MFMA on this path; packed FP32 disabled so 78.6 TFLOP/s is the "
