@@ -167,7 +167,9 @@ int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, doubl
  * device bytes count against the budget; add_mesh fails with GWN_ERR_NOMEM
  * when they do not fit).  gwn_server_submit returns at once; a request is
  * admitted, first come first served, when a slot is idle and its device
- * footprint (queries, results, sort workspace) fits in the remaining budget.
+ * footprint (gwn_server_request_bytes: queries, results and sort workspace of
+ * one chunk of at most 262,144 queries, doubled for larger requests, which run
+ * as a pipeline of such chunks) fits in the remaining budget.
  * Admitted requests run concurrently on different streams, so their kernels
  * overlap on the device.  A request's result is written to the caller's W
  * (which must stay valid until the request is done) and is bit-identical to
@@ -197,6 +199,12 @@ int gwn_server_request_info(const gwn_server* s, int64_t request_id, gwn_request
 int gwn_server_memory(const gwn_server* s, int64_t* resident, int64_t* in_flight, int64_t* budget);
 /* seconds on the server clock (for callers that time arrivals) */
 double gwn_server_now(const gwn_server* s);
+/* device bytes a request of num_queries needs while it runs */
+int64_t gwn_server_request_bytes(int64_t num_queries);
+/* sizes every slot's device buffer for requests of num_queries now, so that
+ * no slot allocates while serving; needs an idle server (no request in
+ * flight); GWN_ERR_NOMEM when the buffers do not fit in the budget */
+int gwn_server_reserve(gwn_server* s, int64_t num_queries);
 
 #ifdef __cplusplus
 }

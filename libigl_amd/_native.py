@@ -75,6 +75,8 @@ SIGNATURES = {
     "gwn_server_request_info": (C.c_int, [vp, C.c_int64, C.POINTER(RequestInfo)]),
     "gwn_server_memory": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gwn_server_now": (C.c_double, [vp]),
+    "gwn_server_request_bytes": (C.c_int64, [C.c_int64]),
+    "gwn_server_reserve": (C.c_int, [vp, C.c_int64]),
 }
 
 _lib = None

@@ -1,13 +1,16 @@
 // serving.cpp -- continuous batching of winding-number queries on resident meshes.
 //
-// One host thread per slot, each with its own HIP stream, pinned staging
-// buffers and grow-only device buffers.  Requests queue FIFO; an idle slot
-// takes the head request when the request's device footprint fits in the
-// budget left after the resident meshes and the buffers other slots hold (a
-// slot that cannot fit the head releases its own idle buffers so another can).
-// Requests on different slots overlap on the device; each is bit-identical to
-// a standalone gwn_soup_query because it runs the same kernels on the same
-// inputs.
+// One host thread per slot, each with its own HIP stream, two pinned chunk
+// buffers and a device buffer that grows in size classes.  Requests queue
+// FIFO; an idle slot takes the head request when the request's device
+// footprint fits in the budget left after the resident meshes and the buffers
+// other slots hold (a slot that cannot fit the head releases its own idle
+// buffer so another can).  Device buffers are (re)allocated outside the
+// server lock, so a growing slot never stalls submissions or other slots.
+// A request is processed in chunks, pipelined so that host-side narrowing and
+// copy-out overlap device work.  Requests on different slots overlap on the
+// device; each is bit-identical to a standalone gwn_soup_query because every
+// query's result depends only on the query, the tree and the precision mode.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +18,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -24,6 +28,31 @@
 
 namespace gwn {
 namespace {
+
+// A request runs in chunks of at most kChunk queries.  Two chunk buffers per
+// slot (pinned host and device) let the host narrow chunk c+1 and copy out
+// chunk c-1 while the device works on chunk c; per-query results do not
+// depend on which other queries share a launch, so chunking leaves every
+// result bit-identical to one gwn_soup_query over the whole batch.
+constexpr int64_t kChunk = 1 << 18;
+
+size_t a256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Device bytes of one chunk buffer for n queries: queries, results, workspace.
+size_t region_bytes(int64_t n) { return a256(12 * (size_t)n) + a256(8 * (size_t)n) + soup_query_work_bytes(n); }
+
+// Device bytes a slot needs for a request of nq queries: one buffer up to a
+// chunk, two (double-buffered) above.
+size_t need_bytes(int64_t nq) { return nq <= kChunk ? region_bytes(nq) : 2 * region_bytes(kChunk); }
+
+// Slots grow their device buffer in power-of-two classes (at least 1 MiB, at
+// most the two-chunk maximum) so a slot reallocates a handful of times at most.
+size_t class_bytes(int64_t nq) {
+  const size_t need = need_bytes(nq), top = 2 * region_bytes(kChunk);
+  size_t c = (size_t)1 << 20;
+  while (c < need) c <<= 1;
+  return std::max(need, std::min(c, top));
+}
 
 struct Request {
   int64_t id = 0;
@@ -38,36 +67,83 @@ struct Request {
   bool done = false;
 };
 
-size_t footprint(int64_t nq) { return (size_t)nq * (12 + 8) + soup_query_work_bytes(nq) + 3 * 256; }
+// Persistent host threads for the narrowing and copy-out loops of large
+// chunks (shared by all slots of all servers; callers run one part inline).
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool pool;
+    return pool;
+  }
+  template <typename Fn>
+  void parallel(int64_t n, Fn fn) {
+    constexpr int64_t kMin = 1 << 16;
+    constexpr int kParts = 4;
+    if (n < kMin) {
+      fn(0, n);
+      return;
+    }
+    const int64_t chunk = (n + kParts - 1) / kParts;
+    std::mutex m;
+    std::condition_variable cv;
+    int left = 0;
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      for (int t = 1; t < kParts; ++t) {
+        const int64_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) continue;
+        ++left;
+        tasks_.push_back([&, b, e] {
+          fn(b, e);
+          std::lock_guard<std::mutex> g(m);
+          if (--left == 0) cv.notify_one();
+        });
+      }
+    }
+    cv_.notify_all();
+    fn(0, std::min(n, chunk));
+    std::unique_lock<std::mutex> g(m);
+    cv.wait(g, [&] { return left == 0; });
+  }
 
-// Host-side copies of large batches (narrowing into pinned memory, results
-// back to the caller) split over a few threads.
-template <typename Fn>
-void host_parallel(int64_t n, Fn fn) {
-  constexpr int64_t kMin = 1 << 17;
-  constexpr int kThreads = 4;
-  if (n < kMin) {
-    fn(0, n);
-    return;
+ private:
+  HostPool() {
+    for (int i = 0; i < 8; ++i) th_.emplace_back([this] { run(); });
   }
-  std::thread th[kThreads - 1];
-  const int64_t chunk = (n + kThreads - 1) / kThreads;
-  for (int t = 1; t < kThreads; ++t) {
-    const int64_t b = t * chunk, e = std::min(n, b + chunk);
-    if (b < e) th[t - 1] = std::thread(fn, b, e);
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
   }
-  fn(0, std::min(n, chunk));
-  for (auto& x : th)
-    if (x.joinable()) x.join();
-}
+  void run() {
+    std::unique_lock<std::mutex> l(mu_);
+    for (;;) {
+      cv_.wait(l, [&] { return stop_ || !tasks_.empty(); });
+      if (tasks_.empty()) return;
+      auto t = std::move(tasks_.front());
+      tasks_.pop_front();
+      l.unlock();
+      t();
+      l.lock();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::function<void()>> tasks_;
+  std::vector<std::thread> th_;
+  bool stop_ = false;
+};
 
 struct Slot {
   hipStream_t stream = nullptr;
-  DevBuf<uint8_t> dev;  // dQ | dW | workspace
-  size_t dev_cap = 0;
-  float* hQ = nullptr;
-  double* hW = nullptr;
-  size_t host_cap = 0;  // queries
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  DevBuf<uint8_t> dev;  // one or two chunk regions: dQ | dW | workspace
+  size_t dev_cap = 0;   // bytes reserved against the budget (== dev.n once allocated)
+  float* hQ[2] = {nullptr, nullptr};
+  double* hW[2] = {nullptr, nullptr};
   std::thread th;
 };
 
@@ -89,8 +165,49 @@ struct gwn_server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   void worker(gwn::Slot* sl);
+  void run(gwn::Slot* sl, gwn::Request* r);
   ~gwn_server();
 };
+
+// Runs one admitted request on a slot whose device buffer holds
+// need_bytes(r->nq); called without the server lock.
+void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
+  using namespace gwn;
+  gwn_soup* m = meshes[r->mesh];
+  const int64_t nchunk = (r->nq + kChunk - 1) / kChunk;
+  const int64_t cn = std::min<int64_t>(r->nq, kChunk);
+  const size_t reg = region_bytes(cn);
+  auto copy_out = [&](int64_t c) {
+    const int b = (int)(c & 1);
+    GWN_HIP(hipEventSynchronize(sl->ev[b]));
+    const int64_t off = c * kChunk, n = std::min(kChunk, r->nq - off);
+    double* Wd = r->W + off;
+    const double* hW = sl->hW[b];
+    HostPool::get().parallel(n, [Wd, hW](int64_t lo, int64_t hi) {
+      std::memcpy(Wd + lo, hW + lo, 8 * (size_t)(hi - lo));
+    });
+  };
+  for (int64_t c = 0; c < nchunk; ++c) {
+    const int b = (int)(c & 1);
+    if (c >= 2) copy_out(c - 2);  // frees host buffer b
+    const int64_t off = c * kChunk, n = std::min(kChunk, r->nq - off);
+    // fast_winding_number.cpp:431-434: queries narrowed to float
+    float* hQ = sl->hQ[b];
+    const double* Qs = r->Q + 3 * off;
+    HostPool::get().parallel(3 * n, [hQ, Qs](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) hQ[i] = (float)Qs[i];
+    });
+    uint8_t* base = sl->dev.p + (size_t)b * reg;
+    float* dQ = reinterpret_cast<float*>(base);
+    double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)cn));
+    uint8_t* ws = base + a256(12 * (size_t)cn) + a256(8 * (size_t)cn);
+    GWN_HIP(hipMemcpyAsync(dQ, hQ, 12 * (size_t)n, hipMemcpyHostToDevice, sl->stream));
+    soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn));
+    GWN_HIP(hipMemcpyAsync(sl->hW[b], dW, 8 * (size_t)n, hipMemcpyDeviceToHost, sl->stream));
+    GWN_HIP(hipEventRecord(sl->ev[b], sl->stream));
+  }
+  for (int64_t c = std::max<int64_t>(0, nchunk - 2); c < nchunk; ++c) copy_out(c);
+}
 
 void gwn_server::worker(gwn::Slot* sl) {
   using namespace gwn;
@@ -98,10 +215,11 @@ void gwn_server::worker(gwn::Slot* sl) {
   std::unique_lock<std::mutex> lk(mu);
   for (;;) {
     Request* r = nullptr;
+    size_t grow_to = 0;  // nonzero: reallocate the device buffer (outside the lock)
     while (!stop) {
       if (!pending.empty()) {
         Request* head = pending.front();
-        const size_t need = footprint(head->nq);
+        const size_t need = need_bytes(head->nq);
         if ((int64_t)need > budget - resident) {  // can never fit: fail it
           pending.pop_front();
           head->status = GWN_ERR_NOMEM;
@@ -111,12 +229,22 @@ void gwn_server::worker(gwn::Slot* sl) {
           cv_done.notify_all();
           continue;
         }
-        if (sl->dev_cap >= need || resident + held - (int64_t)sl->dev_cap + (int64_t)need <= budget) {
+        if (sl->dev_cap >= need) {
           pending.pop_front();
           r = head;
           break;
         }
-        if (sl->dev_cap > 0) {  // make room for another slot
+        const size_t cls = class_bytes(head->nq);
+        const size_t want = resident + held - (int64_t)sl->dev_cap + (int64_t)cls <= budget ? cls : need;
+        if (resident + held - (int64_t)sl->dev_cap + (int64_t)want <= budget) {
+          pending.pop_front();
+          r = head;
+          held += (int64_t)want - (int64_t)sl->dev_cap;
+          sl->dev_cap = want;
+          grow_to = want;
+          break;
+        }
+        if (sl->dev_cap > 0) {  // make room for another slot (rare: memory pressure)
           held -= (int64_t)sl->dev_cap;
           sl->dev.release();
           sl->dev_cap = 0;
@@ -126,50 +254,16 @@ void gwn_server::worker(gwn::Slot* sl) {
       cv_work.wait(lk);
     }
     if (!r) return;
-    const size_t need = footprint(r->nq);
     r->t_start = now();
+    lk.unlock();
     int status = GWN_OK;
     std::string err;
     try {
-      if (sl->dev_cap < need) {
-        held -= (int64_t)sl->dev_cap;
+      if (grow_to) {
         sl->dev.release();
-        sl->dev_cap = 0;
-        sl->dev.alloc(need);
-        sl->dev_cap = need;
-        held += (int64_t)need;
+        sl->dev.alloc(grow_to);
       }
-      gwn_soup* m = meshes[r->mesh];
-      lk.unlock();
-      if ((size_t)r->nq > sl->host_cap) {
-        if (sl->hQ) (void)hipHostFree(sl->hQ);
-        if (sl->hW) (void)hipHostFree(sl->hW);
-        sl->hQ = nullptr;
-        sl->hW = nullptr;
-        sl->host_cap = 0;
-        GWN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl->hQ), sizeof(float) * 3 * (size_t)r->nq));
-        GWN_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl->hW), sizeof(double) * (size_t)r->nq));
-        sl->host_cap = (size_t)r->nq;
-      }
-      // fast_winding_number.cpp:431-434: queries narrowed to float
-      float* hQ = sl->hQ;
-      const double* Qs = r->Q;
-      host_parallel(3 * r->nq, [hQ, Qs](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) hQ[i] = (float)Qs[i];
-      });
-      uint8_t* base = sl->dev.p;
-      auto a256 = [](size_t b) { return (b + 255) / 256 * 256; };
-      float* dQ = reinterpret_cast<float*>(base);
-      double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)r->nq));
-      uint8_t* ws = base + a256(12 * (size_t)r->nq) + a256(8 * (size_t)r->nq);
-      const size_t wsb = need - (size_t)(ws - base);
-      GWN_HIP(hipMemcpyAsync(dQ, sl->hQ, 12 * (size_t)r->nq, hipMemcpyHostToDevice, sl->stream));
-      soup_query_device_ws(m, dQ, r->nq, r->beta, r->prec, dW, sl->stream, ws, wsb);
-      GWN_HIP(hipMemcpyAsync(sl->hW, dW, 8 * (size_t)r->nq, hipMemcpyDeviceToHost, sl->stream));
-      GWN_HIP(hipStreamSynchronize(sl->stream));
-      double* Wd = r->W;
-      const double* hW = sl->hW;
-      host_parallel(r->nq, [Wd, hW](int64_t b, int64_t e) { std::memcpy(Wd + b, hW + b, 8 * (size_t)(e - b)); });
+      run(sl, r);
     } catch (const Error& e) {
       status = e.code;
       err = e.msg;
@@ -177,7 +271,12 @@ void gwn_server::worker(gwn::Slot* sl) {
       status = GWN_ERR_INVALID;
       err = e.what();
     }
-    if (!lk.owns_lock()) lk.lock();
+    if (status != GWN_OK) (void)hipStreamSynchronize(sl->stream);  // no copy may still read the buffers
+    lk.lock();
+    if (grow_to && sl->dev.p == nullptr) {  // allocation failed: return the reservation
+      held -= (int64_t)sl->dev_cap;
+      sl->dev_cap = 0;
+    }
     if (status != GWN_OK) set_error(err);
     r->status = status;
     r->t_done = now();
@@ -196,8 +295,13 @@ gwn_server::~gwn_server() {
   cv_work.notify_all();
   for (auto& s : slots) {
     if (s->th.joinable()) s->th.join();
-    if (s->hQ) (void)hipHostFree(s->hQ);
-    if (s->hW) (void)hipHostFree(s->hW);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int b = 0; b < 2; ++b) {
+      if (s->hQ[b]) (void)hipHostFree(s->hQ[b]);
+      if (s->hW[b]) (void)hipHostFree(s->hW[b]);
+      if (s->ev[b]) (void)hipEventDestroy(s->ev[b]);
+    }
+    s->dev.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
   }
   for (auto* m : meshes) gwn_soup_free(m);
@@ -219,13 +323,19 @@ int gwn_server_create(int nslots, int64_t budget, gwn_server** out) {
     s->budget = budget;
     for (int i = 0; i < nslots; ++i) {
       auto sl = std::make_unique<Slot>();
-      if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipGetLastError();
-        delete s;
-        set_error("hipStreamCreate failed");
-        return GWN_ERR_HIP;
+      bool ok = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) == hipSuccess;
+      for (int b = 0; b < 2 && ok; ++b) {
+        ok = hipEventCreateWithFlags(&sl->ev[b], hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void**>(&sl->hQ[b]), 12 * (size_t)kChunk) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void**>(&sl->hW[b]), 8 * (size_t)kChunk) == hipSuccess;
       }
       s->slots.push_back(std::move(sl));
+      if (!ok) {
+        (void)hipGetLastError();
+        delete s;
+        set_error("stream, event or pinned buffer creation failed");
+        return GWN_ERR_HIP;
+      }
     }
     for (auto& sl : s->slots) sl->th = std::thread(&gwn_server::worker, s, sl.get());
     *out = s;
@@ -352,5 +462,42 @@ int gwn_server_memory(const gwn_server* cs, int64_t* resident, int64_t* in_fligh
 }
 
 double gwn_server_now(const gwn_server* s) { return s ? s->now() : 0.0; }
+
+int64_t gwn_server_request_bytes(int64_t num_queries) {
+  return num_queries <= 0 ? 0 : (int64_t)need_bytes(num_queries);
+}
+
+int gwn_server_reserve(gwn_server* s, int64_t num_queries) {
+  if (!s || num_queries <= 0) {
+    set_error("bad reserve arguments");
+    return GWN_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> l(s->mu);
+  if (s->outstanding != 0) {
+    set_error("gwn_server_reserve needs an idle server");
+    return GWN_ERR_INVALID;
+  }
+  // No request is in flight and the lock is held, so no worker touches its
+  // slot's buffer while it is replaced here.
+  const size_t want = class_bytes(num_queries);
+  try {
+    for (auto& sl : s->slots) {
+      if (sl->dev_cap >= want) continue;
+      if (s->resident + s->held - (int64_t)sl->dev_cap + (int64_t)want > s->budget) {
+        set_error("slot buffers do not fit in the device budget");
+        return GWN_ERR_NOMEM;
+      }
+      s->held -= (int64_t)sl->dev_cap;
+      sl->dev_cap = 0;
+      sl->dev.alloc(want);
+      sl->dev_cap = want;
+      s->held += (int64_t)want;
+    }
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+  return GWN_OK;
+}
 
 }  // extern "C"

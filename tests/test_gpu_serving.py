@@ -46,10 +46,17 @@ def test_mix_equals_alone():
 
 def test_admission_by_memory():
     V, F = D.uv_sphere(4000, 4, 7)
-    # budget fits the mesh and one small request at a time
     probe = gwn.FastWindingNumberBVH().build(V, F)
-    mesh_bytes = probe.info().device_bytes
-    srv = Server(slots=4, device_budget_bytes=mesh_bytes * 2 + (16 << 20))
+    tmp = Server(slots=1, device_budget_bytes=1 << 30)
+    tmp.add_mesh(V, F)
+    mesh_bytes = tmp.memory()[0]  # resident bytes include the FP64 moments
+    tmp.close()
+    small = Server.request_bytes(100_000)
+    # a request above one chunk runs double-buffered: its footprint stays bounded
+    assert Server.request_bytes(2_000_000) == Server.request_bytes(10_000_000) > Server.request_bytes(262_144)
+    # budget: the mesh plus room for one 100k-query request at a time
+    budget = mesh_bytes + small + small // 2
+    srv = Server(slots=4, device_budget_bytes=budget)
     mid = srv.add_mesh(V, F)
     rng = np.random.default_rng(1)
     Qs = [D.uniform_queries(V.min(0), V.max(0), 100_000, rng) for _ in range(6)]
@@ -57,19 +64,46 @@ def test_admission_by_memory():
     srv.drain()
     for rid, Q in zip(rids, Qs):
         assert np.array_equal(srv.wait(rid), probe.query(Q))
-    resident, held, budget = srv.memory()
-    assert resident + held <= budget
-    # a request that can never fit is failed, not hung
+    resident, held, budget_r = srv.memory()
+    assert resident == mesh_bytes and budget_r == budget and resident + held <= budget
+    # a request whose footprint can never fit is failed, not hung
     big = D.uniform_queries(V.min(0), V.max(0), 2_000_000, rng)
+    assert Server.request_bytes(big.shape[0]) > budget - mesh_bytes
     rid = srv.submit(mid, big)
     srv.drain()
     assert srv.info(rid).status == gwn.GWNError(3, "").code
     with pytest.raises(gwn.GWNError):
         srv.wait(rid)
+    # reserving slot buffers beyond the budget is refused
+    with pytest.raises(gwn.GWNError):
+        srv.reserve(1_000_000)
     # a mesh that does not fit is refused
     V2, F2 = D.uv_sphere(200_000, 4, 8)
     with pytest.raises(gwn.GWNError):
         srv.add_mesh(V2, F2)
+    srv.close()
+
+
+def test_chunked_requests_equal_alone():
+    """Requests above one chunk (262,144 queries) run as a pipeline of chunks
+    over two buffers; results equal one standalone query of the whole batch,
+    also after reserve() pre-sized the slots."""
+    V, F = D.uv_sphere(30_000, 6, 17)
+    alone = gwn.FastWindingNumberBVH().build(V, F)
+    srv = Server(slots=2, device_budget_bytes=4 << 30)
+    mid = srv.add_mesh(V, F)
+    srv.reserve(1_000_000)
+    _, held, _ = srv.memory()
+    assert held == 2 * Server.request_bytes(1_000_000)
+    rng = np.random.default_rng(4)
+    reqs = []
+    for nq, prec in ((262_144, gwn.PREC_FP32), (262_145, gwn.PREC_MIXED), (600_001, gwn.PREC_FP32),
+                     (1_000_000, gwn.PREC_MIXED), (5, gwn.PREC_FP64)):
+        Q = D.uniform_queries(V.min(0) - 0.1, V.max(0) + 0.1, nq, rng)
+        reqs.append((srv.submit(mid, Q, 2.0, prec), Q, prec))
+    srv.drain()
+    for rid, Q, prec in reqs:
+        assert np.array_equal(srv.wait(rid), alone.query(Q, 2.0, prec)), (Q.shape, prec)
     srv.close()
 
 
