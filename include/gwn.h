@@ -136,9 +136,26 @@ int gwn_soup_copy_boxdata(const gwn_soup* s, float* out);
  * gwn_cloud_build replaces igl::octree(P,point_indices,CH,CN,W) (octree.h,
  * octree.cpp:7-169) followed by the precompute
  * igl::fast_winding_number(P,N,A,point_indices,CH,order,CM,R,EC),
- * fast_winding_number.h:44-53.  P, N n x 3, A n; FP32.
+ * fast_winding_number.h:44-53.  P, N n x 3, A n; FP32.  With N and A both
+ * NULL it builds the octree alone (igl::octree): such a cloud has no
+ * expansion and cannot be queried.
  */
 int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, int order, gwn_cloud** out);
+/* The precompute (fast_winding_number.h:44-53, .cpp:8-124) and the
+ * evaluation inputs (fast_winding_number.h:88-99) over a caller's octree:
+ * CH ncells x 8 (int32, -1 for leaves), and point_indices flattened as
+ * cell_start (ncells + 1 entries, cell_start[0] = 0) and point_list (the
+ * lists back to back, in the caller's order).  CM, R, EC (ncells x 3, ncells,
+ * ncells x (3 | 12 | 39)) are used as given when all three are non-NULL and
+ * computed as the reference's precompute computes them when all are NULL
+ * (centres summed in FP32 over each cell's list in list order).  The cells
+ * reachable from cell 0 must form a tree whose split cells have 8 children
+ * (GWN_ERR_INVALID otherwise).  gwn_cloud_copy_octree is refused on such a
+ * cloud (the caller holds the octree); gwn_cloud_copy_expansion returns CM,
+ * R, EC. */
+int gwn_cloud_build_from_octree(const float* P, const float* N, const float* A, int64_t n, const int32_t* CH,
+                                int64_t ncells, const int64_t* cell_start, const int32_t* point_list, int order,
+                                const float* CM, const float* R, const float* EC, gwn_cloud** out);
 void gwn_cloud_free(gwn_cloud* c);
 /* number of octree cells (CH rows) */
 int gwn_cloud_num_cells(const gwn_cloud* c, int64_t* ncells);

@@ -2,6 +2,12 @@
 //
 // Replaces, with the same names, argument meaning and output resizing:
 //   igl::winding_number(V,F,O,W)                      winding_number.h:41-48
+//   igl::winding_number(V,F,p)                        winding_number.h:49-60
+//   igl::octree(P,point_indices,CH,CN,W)              octree.h:36-41
+//   igl::fast_winding_number(P,N,A,point_indices,CH,expansion_order,CM,R,EC)
+//                                                     fast_winding_number.h:44-53
+//   igl::fast_winding_number(P,N,A,point_indices,CH,CM,R,EC,Q,beta,WN)
+//                                                     fast_winding_number.h:88-99
 //   igl::fast_winding_number(V,F,Q,W)                 fast_winding_number.h:160-165
 //   igl::fast_winding_number(V,F,order,bvh)           fast_winding_number.h:176-180
 //   igl::fast_winding_number(bvh,accuracy_scale,Q,W)  fast_winding_number.h:191-196
@@ -19,7 +25,9 @@
 // with rows(), cols(), operator()(i,j) and resize(r,c); the Eigen overloads
 // are declared when <Eigen/Core> is available.
 #pragma once
+#include <algorithm>
 #include <cstdint>
+#include <numeric>
 #include <stdexcept>
 #include <type_traits>
 #include <string>
@@ -78,6 +86,16 @@ void winding_number(const MV& V, const MF& F, const MO& O, MW& W) {
   for (int64_t i = 0; i < (int64_t)O.rows(); ++i) W(i, 0) = static_cast<typename std::decay<decltype(W(0, 0))>::type>(w[i]);
 }
 
+template <typename MV, typename MF, typename Mp>
+double winding_number_point(const MV& V, const MF& F, const Mp& p) {
+  const auto v = rows3<double>(V, "V");
+  const auto f = faces(F);
+  const double o[3] = {static_cast<double>(p(0, 0)), static_cast<double>(p(0, 1)), static_cast<double>(p(0, 2))};
+  double w = 0;
+  check(gwn_exact_winding_number(v.data(), V.rows(), f.data(), F.rows(), o, 1, &w));
+  return w;
+}
+
 template <typename MV, typename MF>
 void build(const MV& V, const MF& F, int order, FastWindingNumberBVH& bvh) {
   const auto v = rows3<double>(V, "V");
@@ -125,6 +143,135 @@ void cloud(const MP& P, const MN& N, const MA& A, const MQ& Q, int order, double
   for (int64_t i = 0; i < (int64_t)Q.rows(); ++i) WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
 }
 
+// point_indices (vector of vectors) -> cell_start (ncells + 1) / point_list
+template <typename Index>
+void flatten_lists(const std::vector<std::vector<Index>>& point_indices, std::vector<int64_t>& start,
+                   std::vector<int32_t>& list) {
+  start.assign(point_indices.size() + 1, 0);
+  for (size_t i = 0; i < point_indices.size(); ++i) start[i + 1] = start[i] + (int64_t)point_indices[i].size();
+  list.clear();
+  list.reserve((size_t)start.back());
+  for (const auto& cell : point_indices)
+    for (const auto& k : cell) list.push_back(static_cast<int32_t>(k));
+}
+
+template <typename MCH>
+std::vector<int32_t> children(const MCH& CH) {
+  if (CH.rows() > 0 && CH.cols() != 8) throw std::runtime_error("CH should have 8 columns");
+  std::vector<int32_t> out(8 * (size_t)CH.rows());
+  for (int64_t i = 0; i < (int64_t)CH.rows(); ++i)
+    for (int j = 0; j < 8; ++j) out[8 * i + j] = static_cast<int32_t>(CH(i, j));
+  return out;
+}
+
+template <typename MP, typename Index, typename MCH, typename MCN, typename MW>
+void octree(const MP& P, std::vector<std::vector<Index>>& point_indices, MCH& CH, MCN& CN, MW& W) {
+  const auto p = rows3<float>(P, "P");
+  gwn_cloud* c = nullptr;
+  check(gwn_cloud_build(p.data(), nullptr, nullptr, P.rows(), 0, &c));
+  int64_t m = 0;
+  std::vector<int32_t> ch, list;
+  std::vector<float> cn, w;
+  std::vector<int64_t> cs;
+  std::vector<int32_t> cc;
+  int rc = gwn_cloud_num_cells(c, &m);
+  if (rc == GWN_OK) {
+    ch.resize(8 * (size_t)m);
+    cn.resize(3 * (size_t)m);
+    w.resize((size_t)m);
+    cs.resize((size_t)m);
+    cc.resize((size_t)m);
+    list.resize((size_t)std::max<int64_t>(1, P.rows()));
+    rc = gwn_cloud_copy_octree(c, ch.data(), cn.data(), w.data(), cs.data(), cc.data(), list.data());
+  }
+  gwn_cloud_free(c);
+  check(rc);
+  // octree.cpp lists each cell's points in ascending index order
+  point_indices.assign((size_t)m, {});
+  for (int64_t i = 0; i < m; ++i) {
+    auto& cell = point_indices[i];
+    cell.reserve((size_t)cc[i]);
+    for (int32_t k = 0; k < cc[i]; ++k) cell.push_back(static_cast<Index>(list[cs[i] + k]));
+    std::sort(cell.begin(), cell.end());
+  }
+  CH.resize(m, 8);
+  CN.resize(m, 3);
+  W.resize(m, 1);
+  for (int64_t i = 0; i < m; ++i) {
+    for (int j = 0; j < 8; ++j) CH(i, j) = ch[8 * i + j];
+    for (int j = 0; j < 3; ++j) CN(i, j) = cn[3 * i + j];
+    W(i, 0) = w[i];
+  }
+}
+
+template <typename MP, typename MN, typename MA, typename Index, typename MCH>
+gwn_cloud* cloud_over(const MP& P, const MN& N, const MA& A, const std::vector<std::vector<Index>>& point_indices,
+                      const MCH& CH, int order, const float* CM, const float* R, const float* EC) {
+  const auto p = rows3<float>(P, "P");
+  const auto n = rows3<float>(N, "N");
+  std::vector<float> a((size_t)P.rows());
+  for (int64_t i = 0; i < (int64_t)P.rows(); ++i) a[i] = static_cast<float>(A(i, 0));
+  const auto ch = children(CH);
+  if ((int64_t)point_indices.size() != (int64_t)CH.rows())
+    throw std::runtime_error("point_indices and CH disagree on the number of cells");
+  std::vector<int64_t> start;
+  std::vector<int32_t> list;
+  flatten_lists(point_indices, start, list);
+  gwn_cloud* c = nullptr;
+  check(gwn_cloud_build_from_octree(p.data(), n.data(), a.data(), P.rows(), ch.data(), CH.rows(), start.data(),
+                                    list.data(), order, CM, R, EC, &c));
+  return c;
+}
+
+template <typename MP, typename MN, typename MA, typename Index, typename MCH, typename MCM, typename MR,
+          typename MEC>
+void cloud_precompute(const MP& P, const MN& N, const MA& A, const std::vector<std::vector<Index>>& point_indices,
+                      const MCH& CH, int order, MCM& CM, MR& R, MEC& EC) {
+  if (order < 0 || order > 2) throw std::runtime_error("expansion_order must be 0, 1 or 2");
+  gwn_cloud* c = cloud_over(P, N, A, point_indices, CH, order, nullptr, nullptr, nullptr);
+  const int64_t m = CH.rows();
+  const int nt = order == 0 ? 3 : (order == 1 ? 12 : 39);
+  std::vector<float> cm(3 * (size_t)m), r((size_t)m), ec((size_t)nt * m);
+  const int rc = m > 0 ? gwn_cloud_copy_expansion(c, cm.data(), r.data(), ec.data()) : GWN_OK;
+  gwn_cloud_free(c);
+  check(rc);
+  CM.resize(m, 3);
+  R.resize(m, 1);
+  EC.resize(m, nt);
+  for (int64_t i = 0; i < m; ++i) {
+    for (int j = 0; j < 3; ++j) CM(i, j) = cm[3 * i + j];
+    R(i, 0) = r[i];
+    for (int j = 0; j < nt; ++j) EC(i, j) = ec[(size_t)nt * i + j];
+  }
+}
+
+template <typename MP, typename MN, typename MA, typename Index, typename MCH, typename MCM, typename MR,
+          typename MEC, typename MQ, typename MW>
+void cloud_eval(const MP& P, const MN& N, const MA& A, const std::vector<std::vector<Index>>& point_indices,
+                const MCH& CH, const MCM& CM, const MR& R, const MEC& EC, const MQ& Q, double beta, MW& WN) {
+  const int64_t m = CH.rows();
+  const int nt = (int)EC.cols();
+  if (nt != 3 && nt != 12 && nt != 39) throw std::runtime_error("EC should have 3, 12 or 39 columns");
+  if ((int64_t)CM.rows() != m || (int64_t)R.rows() * (int64_t)R.cols() != m || (int64_t)EC.rows() != m)
+    throw std::runtime_error("CM, R, EC and CH disagree on the number of cells");
+  const auto cm = rows3<float>(CM, "CM");
+  std::vector<float> r((size_t)m), ec((size_t)nt * m);
+  for (int64_t i = 0; i < m; ++i) {
+    r[i] = static_cast<float>(R(i, 0));
+    for (int j = 0; j < nt; ++j) ec[(size_t)nt * i + j] = static_cast<float>(EC(i, j));
+  }
+  const int order = nt == 3 ? 0 : (nt == 12 ? 1 : 2);
+  gwn_cloud* c = cloud_over(P, N, A, point_indices, CH, order, cm.data(), r.data(), ec.data());
+  const auto q = rows3<float>(Q, "Q");
+  std::vector<float> w((size_t)Q.rows());
+  const int rc = Q.rows() > 0 ? gwn_cloud_query(c, q.data(), Q.rows(), beta, w.data()) : GWN_OK;
+  gwn_cloud_free(c);
+  check(rc);
+  WN.resize(Q.rows(), 1);
+  for (int64_t i = 0; i < (int64_t)Q.rows(); ++i)
+    WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
+}
+
 }  // namespace gwn_detail
 }  // namespace igl
 
@@ -137,6 +284,43 @@ template <typename DerivedV, typename DerivedF, typename DerivedO, typename Deri
 void winding_number(const Eigen::MatrixBase<DerivedV>& V, const Eigen::MatrixBase<DerivedF>& F,
                     const Eigen::MatrixBase<DerivedO>& O, Eigen::PlainObjectBase<DerivedW>& W) {
   gwn_detail::winding_number(V.derived(), F.derived(), O.derived(), W.derived());
+}
+
+template <typename DerivedV, typename DerivedF, typename Derivedp>
+typename DerivedV::Scalar winding_number(const Eigen::MatrixBase<DerivedV>& V, const Eigen::MatrixBase<DerivedF>& F,
+                                         const Eigen::MatrixBase<Derivedp>& p) {
+  return static_cast<typename DerivedV::Scalar>(gwn_detail::winding_number_point(V.derived(), F.derived(), p.derived()));
+}
+
+template <typename DerivedP, typename IndexType, typename DerivedCH, typename DerivedCN, typename DerivedW>
+void octree(const Eigen::MatrixBase<DerivedP>& P, std::vector<std::vector<IndexType>>& point_indices,
+            Eigen::PlainObjectBase<DerivedCH>& CH, Eigen::PlainObjectBase<DerivedCN>& CN,
+            Eigen::PlainObjectBase<DerivedW>& W) {
+  gwn_detail::octree(P.derived(), point_indices, CH.derived(), CN.derived(), W.derived());
+}
+
+template <typename DerivedP, typename DerivedA, typename DerivedN, typename Index, typename DerivedCH,
+          typename DerivedCM, typename DerivedR, typename DerivedEC>
+void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedN>& N,
+                         const Eigen::MatrixBase<DerivedA>& A, const std::vector<std::vector<Index>>& point_indices,
+                         const Eigen::MatrixBase<DerivedCH>& CH, const int expansion_order,
+                         Eigen::PlainObjectBase<DerivedCM>& CM, Eigen::PlainObjectBase<DerivedR>& R,
+                         Eigen::PlainObjectBase<DerivedEC>& EC) {
+  gwn_detail::cloud_precompute(P.derived(), N.derived(), A.derived(), point_indices, CH.derived(), expansion_order,
+                               CM.derived(), R.derived(), EC.derived());
+}
+
+template <typename DerivedP, typename DerivedA, typename DerivedN, typename Index, typename DerivedCH,
+          typename DerivedCM, typename DerivedR, typename DerivedEC, typename DerivedQ, typename BetaType,
+          typename DerivedWN>
+void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedN>& N,
+                         const Eigen::MatrixBase<DerivedA>& A, const std::vector<std::vector<Index>>& point_indices,
+                         const Eigen::MatrixBase<DerivedCH>& CH, const Eigen::MatrixBase<DerivedCM>& CM,
+                         const Eigen::MatrixBase<DerivedR>& R, const Eigen::MatrixBase<DerivedEC>& EC,
+                         const Eigen::MatrixBase<DerivedQ>& Q, const BetaType beta,
+                         Eigen::PlainObjectBase<DerivedWN>& WN) {
+  gwn_detail::cloud_eval(P.derived(), N.derived(), A.derived(), point_indices, CH.derived(), CM.derived(),
+                         R.derived(), EC.derived(), Q.derived(), (double)beta, WN.derived());
 }
 
 template <typename DerivedV, typename DerivedF>

@@ -58,7 +58,9 @@ SIGNATURES = {
     "gwn_sah4_tree": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64)]),
     "gwn_soup_copy_nodes": (C.c_int, [vp, u32p]),
     "gwn_soup_copy_boxdata": (C.c_int, [vp, f32p]),
-    "gwn_cloud_build": (C.c_int, [f32p, f32p, f32p, C.c_int64, C.c_int, C.POINTER(vp)]),
+    "gwn_cloud_build": (C.c_int, [f32p, vp, vp, C.c_int64, C.c_int, C.POINTER(vp)]),
+    "gwn_cloud_build_from_octree": (C.c_int, [f32p, f32p, f32p, C.c_int64, i32p, C.c_int64, i64p, i32p, C.c_int,
+                                              vp, vp, vp, C.POINTER(vp)]),
     "gwn_cloud_free": (None, [vp]),
     "gwn_cloud_num_cells": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     "gwn_cloud_copy_octree": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),

@@ -33,7 +33,10 @@ void soup_setup(gwn_soup* s, hipStream_t st);
 void soup_finish_build(gwn_soup* s, hipStream_t st);
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
                        hipStream_t st);
-void cloud_build_device(gwn_cloud* c, hipStream_t st);
+void cloud_build_device(gwn_cloud* c, hipStream_t st, bool moments);
+void cloud_build_from_octree_device(gwn_cloud* c, const int32_t* CH, int64_t ncells, const int64_t* cell_start,
+                                    const int32_t* point_list, const float* CM, const float* R, const float* EC,
+                                    hipStream_t st);
 void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st);
 
 template <typename Fn>
@@ -317,6 +320,44 @@ int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, i
     *out = nullptr;
     GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
     GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+    const bool octree_only = !N && !A;
+    GWN_CHECK(n == 0 || (P && (octree_only || (N && A))), GWN_ERR_INVALID, "null input");
+    ensure_device();
+    hipStream_t st = nullptr;
+    gwn_cloud* c = new gwn_cloud();
+    try {
+      c->n = n;
+      c->order = order;
+      c->nterms = order == 0 ? 3 : (order == 1 ? 12 : 39);
+      c->octree_only = octree_only;
+      upload(c->P, P, 3 * (size_t)n, st);
+      if (octree_only) {
+        c->N.alloc(3 * (size_t)(n > 0 ? n : 1));
+        c->A.alloc((size_t)(n > 0 ? n : 1));
+        GWN_HIP(hipMemsetAsync(c->N.p, 0, c->N.bytes(), st));
+        GWN_HIP(hipMemsetAsync(c->A.p, 0, c->A.bytes(), st));
+      } else {
+        upload(c->N, N, 3 * (size_t)n, st);
+        upload(c->A, A, (size_t)n, st);
+      }
+      cloud_build_device(c, st, !octree_only);
+      GWN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int gwn_cloud_build_from_octree(const float* P, const float* N, const float* A, int64_t n, const int32_t* CH,
+                                int64_t ncells, const int64_t* cell_start, const int32_t* point_list, int order,
+                                const float* CM, const float* R, const float* EC, gwn_cloud** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
+    GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
     GWN_CHECK(n == 0 || (P && N && A), GWN_ERR_INVALID, "null input");
     ensure_device();
     hipStream_t st = nullptr;
@@ -328,7 +369,7 @@ int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, i
       upload(c->P, P, 3 * (size_t)n, st);
       upload(c->N, N, 3 * (size_t)n, st);
       upload(c->A, A, (size_t)n, st);
-      cloud_build_device(c, st);
+      cloud_build_from_octree_device(c, CH, ncells, cell_start, point_list, CM, R, EC, st);
       GWN_HIP(hipStreamSynchronize(st));
     } catch (...) {
       delete c;
@@ -351,6 +392,7 @@ int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, 
                           int32_t* cell_count, int32_t* point_list) {
   return guard([&] {
     GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(!c->user_octree, GWN_ERR_INVALID, "cloud was built over a caller's octree");
     const size_t nc = (size_t)c->ncells;
     if (CH && nc) GWN_HIP(hipMemcpy(CH, c->CH.p, sizeof(int32_t) * 8 * nc, hipMemcpyDeviceToHost));
     if (CN && nc) GWN_HIP(hipMemcpy(CN, c->CN.p, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost));
@@ -364,6 +406,7 @@ int gwn_cloud_copy_octree(const gwn_cloud* c, int32_t* CH, float* CN, float* W, 
 int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC) {
   return guard([&] {
     GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(!c->octree_only, GWN_ERR_INVALID, "cloud was built without normals and areas");
     const size_t nc = (size_t)c->ncells;
     if (CM && nc) GWN_HIP(hipMemcpy(CM, c->CM.p, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost));
     if (R && nc) GWN_HIP(hipMemcpy(R, c->R.p, sizeof(float) * nc, hipMemcpyDeviceToHost));
@@ -376,6 +419,7 @@ int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, doubl
     GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
     GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
     GWN_CHECK(nq == 0 || (Q && WN), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(!c->octree_only, GWN_ERR_INVALID, "cloud was built without normals and areas");
     cloud_query_device(c, Q, nq, beta, WN, (hipStream_t)stream);
   });
 }
@@ -385,6 +429,7 @@ int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta,
     GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null handle");
     GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
     GWN_CHECK(nq == 0 || (Q && WN), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(!c->octree_only, GWN_ERR_INVALID, "cloud was built without normals and areas");
     if (nq == 0) return;
     hipStream_t st = nullptr;
     DevBuf<float> dQ, dW(nq);

@@ -287,7 +287,7 @@ __device__ __forceinline__ void cell_moments_serial(const float* __restrict__ P,
 // one thread per small cell
 template <int NT>
 __global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, const int32_t* __restrict__ CH,
-                             const int64_t* __restrict__ cstart,
+                             bool fold_cm, const int64_t* __restrict__ cstart,
                              const int32_t* __restrict__ ccount, const float* __restrict__ P,
                              const float* __restrict__ N, const float* __restrict__ A,
                              const int32_t* __restrict__ perm, float* __restrict__ CM, float* __restrict__ R,
@@ -295,7 +295,7 @@ __global__ void k_cell_small(const int32_t* __restrict__ cells, uint32_t count, 
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
   const int32_t c = cells[t];
-  cell_moments_serial<NT>(P, N, A, perm, cstart[c], ccount[c], CH[8 * (int64_t)c] >= 0, CM + 3 * (int64_t)c, R + c,
+  cell_moments_serial<NT>(P, N, A, perm, cstart[c], ccount[c], fold_cm && CH[8 * (int64_t)c] >= 0, CM + 3 * (int64_t)c, R + c,
                           EC + (int64_t)NT * c);
 }
 
@@ -311,7 +311,8 @@ __device__ __forceinline__ float wave_max(float v) {
 // one wavefront per large cell
 template <int NT>
 __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ cells, uint32_t count,
-                                                    const int32_t* __restrict__ CH, const int64_t* __restrict__ cstart,
+                                                    const int32_t* __restrict__ CH, bool fold_cm,
+                                                    const int64_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ ccount, const float* __restrict__ P,
                                                     const float* __restrict__ N, const float* __restrict__ A,
                                                     const int32_t* __restrict__ perm, float* __restrict__ CM,
@@ -322,11 +323,12 @@ __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ 
   const int32_t c = cells[w];
   const int64_t s = cstart[c];
   const int32_t cnt = ccount[c];
-  // split cells take their centre from k_lvl_fold; an unsplittable leaf (more
-  // than 32 points with equal 42-level keys) sums it here, serially, in
-  // index order (its points are index-ordered)
+  // with fold_cm, split cells take their centre from k_lvl_fold; every other
+  // cell (an unsplittable leaf of more than 32 points with equal 42-level
+  // keys, or any cell of a caller's octree) sums it here, serially, in the
+  // order of its point list
   float lc[3] = {0.0f, 0.0f, 0.0f};
-  const bool split = CH[8 * (int64_t)c] >= 0;
+  const bool split = fold_cm && CH[8 * (int64_t)c] >= 0;
   if (!split && lane == 0) {
     float at = 0.0f;
     for (int32_t j = 0; j < cnt; ++j) {
@@ -431,6 +433,7 @@ __global__ void k_rec_pos(const SplitCell* __restrict__ lvl, uint32_t n, const i
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int32_t c = lvl[t].cell;
+  if (c != 0 && pos[c] == 0xFFFFFFFFu) return;  // not reached (an empty ancestor)
   uint32_t p = (c == 0) ? 0u : pos[c] + 1;
   for (int o = 0; o < 8; ++o) {
     const int32_t ch = CH[8 * (int64_t)c + o];
@@ -595,7 +598,124 @@ __global__ void k_cloud_morton_q(const float* __restrict__ Q, int64_t nq, float 
 }
 
 // ---------------- host orchestration ----------------
-void cloud_build_device(gwn_cloud* c, hipStream_t st) {
+// CM, R, EC of every cell.  fold_cm: split cells of this library's own
+// octree take their centres from the per-level folds (their point lists are
+// ranges of the key order, the reference sums in ascending index order);
+// otherwise each cell sums its centre over its own list, in list order.
+static void cell_moments(gwn_cloud* c, bool fold_cm, const SplitCell* all, const std::vector<uint32_t>& lvl_off,
+                         hipStream_t st) {
+  const int64_t ncells = c->ncells;
+  const int nt = c->nterms;
+  c->CM.alloc(3 * (size_t)ncells);
+  c->R.alloc(ncells);
+  c->EC.alloc((size_t)nt * ncells);
+  {
+    DevBuf<int32_t> small(ncells), large(ncells);
+    DevBuf<uint32_t> ns(2);
+    GWN_HIP(hipMemsetAsync(ns.p, 0, 8, st));
+    hipLaunchKernelGGL(k_cell_lists, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, small.p, ns.p,
+                       large.p, ns.p + 1);
+    check_launch("k_cell_lists");
+    uint32_t cnts[2];
+    GWN_HIP(hipMemcpyAsync(cnts, ns.p, 8, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_cell_empty, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, nt, c->CM.p,
+                       c->R.p, c->EC.p);
+    check_launch("k_cell_empty");
+    // split-cell centres, level by level
+    for (size_t d = 0; fold_cm && d + 1 < lvl_off.size(); ++d) {
+      const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+      if (e == b) continue;
+      const uint32_t m = e - b;
+      DevBuf<uint32_t> cnt(m), off(m), tot(1);
+      hipLaunchKernelGGL(k_split_counts, dim3(div_up(m, 256)), dim3(256), 0, st, all + b, m, cnt.p);
+      check_launch("k_split_counts");
+      DevBuf<uint8_t> sct(scan_temp_bytes(m));
+      exclusive_scan_u32(cnt.p, off.p, m, tot.p, sct.p, st);
+      uint32_t T = 0;
+      GWN_HIP(hipMemcpyAsync(&T, tot.p, 4, hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipStreamSynchronize(st));
+      DevBuf<uint64_t> keys(T);
+      DevBuf<uint32_t> dummy(T);
+      hipLaunchKernelGGL(k_lvl_emit, dim3(m), dim3(64), 0, st, all + b, off.p, c->perm.p, keys.p);
+      check_launch("k_lvl_emit");
+      int obits = 1;
+      while ((1u << obits) < m) ++obits;
+      const size_t tb = radix_sort_temp_bytes(T, true);
+      DevBuf<uint8_t> tmp(tb);
+      radix_sort_u64(keys.p, dummy.p, T, 32 + obits, tmp.p, tb, st);
+      hipLaunchKernelGGL(k_lvl_fold, dim3(div_up(m, 64)), dim3(64), 0, st, all + b, m, off.p, keys.p, c->P.p,
+                         c->A.p, c->CM.p);
+      check_launch("k_lvl_fold");
+      GWN_HIP(hipStreamSynchronize(st));
+    }
+#define GWN_CELL_LAUNCH(NTV)                                                                                        \
+  do {                                                                                                              \
+    if (cnts[0])                                                                                                    \
+      hipLaunchKernelGGL(k_cell_small<NTV>, dim3(div_up(cnts[0], 128)), dim3(128), 0, st, small.p, cnts[0],        \
+                         c->CH.p, fold_cm, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
+                         c->EC.p);                                                                                  \
+    if (cnts[1])                                                                                                    \
+      hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)cnts[1] * 64, 256)), dim3(256), 0, st, large.p,    \
+                         cnts[1], c->CH.p, fold_cm, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
+                         c->EC.p);                                                                                  \
+  } while (0)
+    if (nt == 3)
+      GWN_CELL_LAUNCH(3);
+    else if (nt == 12)
+      GWN_CELL_LAUNCH(12);
+    else
+      GWN_CELL_LAUNCH(39);
+#undef GWN_CELL_LAUNCH
+    check_launch("k_cell_moments");
+  }
+}
+
+// Rope records over the non-empty non-root cells (split cells given level by
+// level in `all`); a leaf record points at leaf_start[cell] in Ps/NAs.
+static void rope_records(gwn_cloud* c, const SplitCell* all, const std::vector<uint32_t>& lvl_off,
+                         const int64_t* leaf_start, hipStream_t st) {
+  const int64_t ncells = c->ncells;
+  if (c->root_leaf || lvl_off.size() < 2) {
+    c->nrec = 0;
+    GWN_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  DevBuf<uint32_t> rc(ncells), pos(ncells);
+  // positions start unset (all ones): cells never reached through non-empty
+  // parents keep it and get no record
+  hipLaunchKernelGGL(k_fill_i32, dim3(div_up(ncells, 256)), dim3(256), 0, st, reinterpret_cast<int32_t*>(pos.p),
+                     ncells, -1);
+  check_launch("k_fill_i32");
+  const int nlev = (int)lvl_off.size() - 1;
+  for (int d = nlev - 1; d >= 0; --d) {
+    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+    if (e > b)
+      hipLaunchKernelGGL(k_rec_count, dim3(div_up(e - b, 256)), dim3(256), 0, st, all + b, e - b, c->CH.p,
+                         c->ccount.p, rc.p);
+  }
+  check_launch("k_rec_count");
+  uint32_t root_rc = 0;
+  GWN_HIP(hipMemcpyAsync(&root_rc, rc.p, 4, hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  c->nrec = root_rc - 1;  // the root itself has no record
+  if (c->nrec == 0) return;
+  for (int d = 0; d < nlev; ++d) {
+    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
+    if (e > b)
+      hipLaunchKernelGGL(k_rec_pos, dim3(div_up(e - b, 256)), dim3(256), 0, st, all + b, e - b, c->CH.p,
+                         c->ccount.p, rc.p, pos.p);
+  }
+  check_launch("k_rec_pos");
+  c->head.alloc(c->nrec);
+  c->meta.alloc(c->nrec);
+  hipLaunchKernelGGL(k_rec_write, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->CH.p, c->ccount.p,
+                     leaf_start, rc.p, pos.p, c->CM.p, c->R.p, (uint32_t)c->nrec, c->head.p, c->meta.p);
+  check_launch("k_rec_write");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+void cloud_build_device(gwn_cloud* c, hipStream_t st, bool moments) {
   const int64_t n = c->n;
   if (n == 0) {
     // a single empty root cell
@@ -628,6 +748,8 @@ void cloud_build_device(gwn_cloud* c, hipStream_t st) {
     GWN_HIP(hipMemcpyAsync(c->EC.p, ec.data(), 4 * ec.size(), hipMemcpyHostToDevice, st));
     GWN_HIP(hipStreamSynchronize(st));
     c->nrec = 0;
+    c->root_leaf = true;
+    c->root_start = c->root_count = 0;
     return;
   }
   // root cube: bbox centre and longest side (octree.cpp:139-150), in float
@@ -738,104 +860,137 @@ void cloud_build_device(gwn_cloud* c, hipStream_t st) {
     GWN_CHECK(lvl_off.back() == S, GWN_ERR_INVALID, "octree split cells != scan total");
   }
 
-  // precompute
-  const int nt = c->nterms;
-  c->CM.alloc(3 * (size_t)ncells);
-  c->R.alloc(ncells);
-  c->EC.alloc((size_t)nt * ncells);
-  {
-    DevBuf<int32_t> small(ncells), large(ncells);
-    DevBuf<uint32_t> ns(2);
-    GWN_HIP(hipMemsetAsync(ns.p, 0, 8, st));
-    hipLaunchKernelGGL(k_cell_lists, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, small.p, ns.p,
-                       large.p, ns.p + 1);
-    check_launch("k_cell_lists");
-    uint32_t cnts[2];
-    GWN_HIP(hipMemcpyAsync(cnts, ns.p, 8, hipMemcpyDeviceToHost, st));
-    GWN_HIP(hipStreamSynchronize(st));
-    hipLaunchKernelGGL(k_cell_empty, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, nt, c->CM.p,
-                       c->R.p, c->EC.p);
-    check_launch("k_cell_empty");
-    // split-cell centres, level by level
-    for (size_t d = 0; d + 1 < lvl_off.size(); ++d) {
-      const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
-      if (e == b) continue;
-      const uint32_t m = e - b;
-      DevBuf<uint32_t> cnt(m), off(m), tot(1);
-      hipLaunchKernelGGL(k_split_counts, dim3(div_up(m, 256)), dim3(256), 0, st, all.p + b, m, cnt.p);
-      check_launch("k_split_counts");
-      DevBuf<uint8_t> sct(scan_temp_bytes(m));
-      exclusive_scan_u32(cnt.p, off.p, m, tot.p, sct.p, st);
-      uint32_t T = 0;
-      GWN_HIP(hipMemcpyAsync(&T, tot.p, 4, hipMemcpyDeviceToHost, st));
-      GWN_HIP(hipStreamSynchronize(st));
-      DevBuf<uint64_t> keys(T);
-      DevBuf<uint32_t> dummy(T);
-      hipLaunchKernelGGL(k_lvl_emit, dim3(m), dim3(64), 0, st, all.p + b, off.p, c->perm.p, keys.p);
-      check_launch("k_lvl_emit");
-      int obits = 1;
-      while ((1u << obits) < m) ++obits;
-      const size_t tb = radix_sort_temp_bytes(T, true);
-      DevBuf<uint8_t> tmp(tb);
-      radix_sort_u64(keys.p, dummy.p, T, 32 + obits, tmp.p, tb, st);
-      hipLaunchKernelGGL(k_lvl_fold, dim3(div_up(m, 64)), dim3(64), 0, st, all.p + b, m, off.p, keys.p, c->P.p,
-                         c->A.p, c->CM.p);
-      check_launch("k_lvl_fold");
-      GWN_HIP(hipStreamSynchronize(st));
-    }
-#define GWN_CELL_LAUNCH(NTV)                                                                                        \
-  do {                                                                                                              \
-    if (cnts[0])                                                                                                    \
-      hipLaunchKernelGGL(k_cell_small<NTV>, dim3(div_up(cnts[0], 128)), dim3(128), 0, st, small.p, cnts[0],        \
-                         c->CH.p, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
-                         c->EC.p);                                                                                  \
-    if (cnts[1])                                                                                                    \
-      hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)cnts[1] * 64, 256)), dim3(256), 0, st, large.p,    \
-                         cnts[1], c->CH.p, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
-                         c->EC.p);                                                                                  \
-  } while (0)
-    if (nt == 3)
-      GWN_CELL_LAUNCH(3);
-    else if (nt == 12)
-      GWN_CELL_LAUNCH(12);
-    else
-      GWN_CELL_LAUNCH(39);
-#undef GWN_CELL_LAUNCH
-    check_launch("k_cell_moments");
-  }
-
-  // rope records
-  if (S == 0) {
+  c->root_leaf = S == 0;
+  c->root_start = 0;
+  c->root_count = n;
+  if (!moments) {  // igl::octree alone
     c->nrec = 0;
     GWN_HIP(hipStreamSynchronize(st));
     return;
   }
-  DevBuf<uint32_t> rc(ncells), pos(ncells);
-  const int nlev = (int)lvl_off.size() - 1;
-  for (int d = nlev - 1; d >= 0; --d) {
-    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
-    if (e > b)
-      hipLaunchKernelGGL(k_rec_count, dim3(div_up(e - b, 256)), dim3(256), 0, st, all.p + b, e - b, c->CH.p,
-                         c->ccount.p, rc.p);
+  cell_moments(c, true, all.p, lvl_off, st);
+  rope_records(c, all.p, lvl_off, c->cstart.p, st);
+}
+
+// Precompute and rope over a caller's octree: CH (ncells x 8, -1 for
+// leaves), point lists as cell_start (ncells + 1) / point_list, optionally
+// CM, R, EC already computed (fast_winding_number.h:88-99 takes them as
+// inputs).  The cells reachable from cell 0 must form a tree in which a cell
+// either has 8 children or none (CH(c,0) == -1), as octree.cpp:7-169 builds
+// it and fast_winding_number.cpp:230-290 walks it.
+void cloud_build_from_octree_device(gwn_cloud* c, const int32_t* hCH, int64_t ncells, const int64_t* hstart,
+                                    const int32_t* hlist, const float* hCM, const float* hR, const float* hEC,
+                                    hipStream_t st) {
+  const int64_t n = c->n;
+  GWN_CHECK(ncells >= 1 && ncells < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad cell count");
+  GWN_CHECK(hCH && hstart && (hstart[ncells] == 0 || hlist), GWN_ERR_INVALID, "null octree input");
+  GWN_CHECK(hstart[0] == 0, GWN_ERR_INVALID, "cell_start[0] must be 0");
+  for (int64_t i = 0; i < ncells; ++i) {
+    GWN_CHECK(hstart[i + 1] >= hstart[i], GWN_ERR_INVALID, "cell_start must be non-decreasing");
+    GWN_CHECK(hstart[i + 1] - hstart[i] < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "cell list too long");
   }
-  check_launch("k_rec_count");
-  uint32_t root_rc = 0;
-  GWN_HIP(hipMemcpyAsync(&root_rc, rc.p, 4, hipMemcpyDeviceToHost, st));
-  GWN_HIP(hipStreamSynchronize(st));
-  c->nrec = root_rc - 1;  // the root itself has no record
-  for (int d = 0; d < nlev; ++d) {
-    const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
-    if (e > b)
-      hipLaunchKernelGGL(k_rec_pos, dim3(div_up(e - b, 256)), dim3(256), 0, st, all.p + b, e - b, c->CH.p,
-                         c->ccount.p, rc.p, pos.p);
+  const int64_t L = hstart[ncells];
+  for (int64_t k = 0; k < L; ++k)
+    GWN_CHECK(hlist[k] >= 0 && hlist[k] < n, GWN_ERR_INVALID, "point index out of range in point_indices");
+  // breadth-first walk from the root: levels of split cells, reachability
+  std::vector<uint8_t> seen((size_t)ncells, 0);
+  std::vector<SplitCell> lv;
+  std::vector<uint32_t> lvl_off{0};
+  std::vector<int32_t> frontier{0}, next, leaves;
+  seen[0] = 1;
+  for (int depth = 0; !frontier.empty(); ++depth) {
+    next.clear();
+    for (int32_t cc : frontier) {
+      const int32_t* ch = hCH + 8 * (int64_t)cc;
+      if (ch[0] == -1) {
+        leaves.push_back(cc);
+        continue;
+      }
+      lv.push_back(SplitCell{cc, depth, hstart[cc], hstart[cc + 1], 0u, 0u});
+      for (int o = 0; o < 8; ++o) {
+        GWN_CHECK(ch[o] >= 0 && ch[o] < ncells, GWN_ERR_INVALID, "CH: a split cell needs 8 valid children");
+        GWN_CHECK(!seen[ch[o]], GWN_ERR_INVALID, "CH does not describe a tree (a cell is reached twice)");
+        seen[ch[o]] = 1;
+        next.push_back(ch[o]);
+      }
+    }
+    if (lv.size() > lvl_off.back()) lvl_off.push_back((uint32_t)lv.size());
+    frontier.swap(next);
   }
-  check_launch("k_rec_pos");
-  c->head.alloc(c->nrec);
-  c->meta.alloc(c->nrec);
-  hipLaunchKernelGGL(k_rec_write, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->CH.p, c->ccount.p,
-                     c->cstart.p, rc.p, pos.p, c->CM.p, c->R.p, (uint32_t)c->nrec, c->head.p, c->meta.p);
-  check_launch("k_rec_write");
+  c->ncells = ncells;
+  c->user_octree = true;
+  c->root_leaf = hCH[0] == -1;
+  // point data: all points in index order (brute force, beta <= 0), then the
+  // lists of the reachable non-empty leaves back to back (rope leaf records)
+  std::vector<int64_t> lstart((size_t)ncells, 0);
+  std::vector<uint32_t> gather((size_t)n);
+  for (int64_t i = 0; i < n; ++i) gather[i] = (uint32_t)i;
+  for (int32_t lc : leaves) {
+    lstart[lc] = (int64_t)gather.size();
+    for (int64_t k = hstart[lc]; k < hstart[lc + 1]; ++k) gather.push_back((uint32_t)hlist[k]);
+  }
+  GWN_CHECK(gather.size() < (size_t)0x7FFFFFFF, GWN_ERR_INVALID, "leaf lists too long");
+  c->root_start = c->root_leaf ? lstart[0] : 0;
+  c->root_count = c->root_leaf ? hstart[1] - hstart[0] : 0;
+  const int64_t G = (int64_t)gather.size();
+  {
+    DevBuf<uint32_t> dg(G > 0 ? G : 1);
+    DevBuf<int32_t> pout(G > 0 ? G : 1);
+    c->Ps.alloc(3 * (size_t)(G > 0 ? G : 1));
+    c->NAs.alloc(3 * (size_t)(G > 0 ? G : 1));
+    if (G > 0) {
+      GWN_HIP(hipMemcpyAsync(dg.p, gather.data(), 4 * (size_t)G, hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_permute_points, dim3(div_up(G, 256)), dim3(256), 0, st, c->P.p, c->N.p, c->A.p, dg.p, G,
+                         c->Ps.p, c->NAs.p, pout.p);
+      check_launch("k_permute_points");
+    }
+    GWN_HIP(hipStreamSynchronize(st));
+  }
+  // octree arrays as given; CN and W are not part of the input
+  std::vector<int64_t> cs((size_t)ncells);
+  std::vector<int32_t> cn((size_t)ncells);
+  for (int64_t i = 0; i < ncells; ++i) {
+    cs[i] = hstart[i];
+    cn[i] = (int32_t)(hstart[i + 1] - hstart[i]);
+  }
+  c->CH.alloc(8 * (size_t)ncells);
+  c->cstart.alloc(ncells);
+  c->ccount.alloc(ncells);
+  c->perm.alloc(L > 0 ? L : 1);
+  GWN_HIP(hipMemcpyAsync(c->CH.p, hCH, 32 * (size_t)ncells, hipMemcpyHostToDevice, st));
+  GWN_HIP(hipMemcpyAsync(c->cstart.p, cs.data(), 8 * (size_t)ncells, hipMemcpyHostToDevice, st));
+  GWN_HIP(hipMemcpyAsync(c->ccount.p, cn.data(), 4 * (size_t)ncells, hipMemcpyHostToDevice, st));
+  if (L > 0) GWN_HIP(hipMemcpyAsync(c->perm.p, hlist, 4 * (size_t)L, hipMemcpyHostToDevice, st));
+  // query ordering box: the points' bounding box
+  std::vector<float> hp(3 * (size_t)n);
+  if (n > 0) GWN_HIP(hipMemcpyAsync(hp.data(), c->P.p, 4 * hp.size(), hipMemcpyDeviceToHost, st));
   GWN_HIP(hipStreamSynchronize(st));
+  for (int d = 0; d < 3; ++d) {
+    c->bbox[d] = n > 0 ? hp[d] : 0.0f;
+    c->bbox[3 + d] = n > 0 ? hp[d] : 0.0f;
+  }
+  for (int64_t i = 1; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      c->bbox[d] = std::min(c->bbox[d], hp[3 * i + d]);
+      c->bbox[3 + d] = std::max(c->bbox[3 + d], hp[3 * i + d]);
+    }
+  DevBuf<SplitCell> all(lv.size() > 0 ? lv.size() : 1);
+  if (!lv.empty())
+    GWN_HIP(hipMemcpyAsync(all.p, lv.data(), sizeof(SplitCell) * lv.size(), hipMemcpyHostToDevice, st));
+  if (hCM && hR && hEC) {
+    c->CM.alloc(3 * (size_t)ncells);
+    c->R.alloc(ncells);
+    c->EC.alloc((size_t)c->nterms * ncells);
+    GWN_HIP(hipMemcpyAsync(c->CM.p, hCM, 12 * (size_t)ncells, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->R.p, hR, 4 * (size_t)ncells, hipMemcpyHostToDevice, st));
+    GWN_HIP(hipMemcpyAsync(c->EC.p, hEC, 4 * (size_t)c->nterms * ncells, hipMemcpyHostToDevice, st));
+  } else {
+    GWN_CHECK(!hCM && !hR && !hEC, GWN_ERR_INVALID, "give all of CM, R, EC or none");
+    cell_moments(c, false, all.p, lvl_off, st);
+  }
+  DevBuf<int64_t> dls(ncells);
+  GWN_HIP(hipMemcpyAsync(dls.p, lstart.data(), 8 * (size_t)ncells, hipMemcpyHostToDevice, st));
+  rope_records(c, all.p, lvl_off, dls.p, st);
 }
 
 void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st) {
@@ -865,17 +1020,20 @@ void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double 
                      1024.0f / ey, 1024.0f / ez, keys, perm);
   check_launch("k_cloud_morton_q");
   radix_sort_u32(keys, perm, nq, 30, tmp, tb, st);
-  const bool root_leaf = c->nrec == 0;
+  const bool rl = c->root_leaf;
+  const float* Ps = c->Ps.p + (rl ? 3 * c->root_start : 0);
+  const float* NAs = c->NAs.p + (rl ? 3 * c->root_start : 0);
+  const int64_t npts = rl ? c->root_count : 0;
   ProfileScope prof("cloud_traverse", st);
   if (c->nterms == 3)
     hipLaunchKernelGGL(k_cloud_traverse<3>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   else if (c->nterms == 12)
     hipLaunchKernelGGL(k_cloud_traverse<12>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   else
     hipLaunchKernelGGL(k_cloud_traverse<39>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, c->Ps.p, c->NAs.p, c->n, root_leaf, Q, perm, nq, beta, WN);
+                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   check_launch("k_cloud_traverse");
 }
 

@@ -175,6 +175,10 @@ struct gwn_soup {
 struct gwn_cloud {
   int64_t n = 0, ncells = 0, nrec = 0;
   int order = 2, nterms = 39;
+  bool root_leaf = true;               // the root has no children: brute force over its points
+  int64_t root_start = 0, root_count = 0;  // the root leaf's points in Ps/NAs
+  bool user_octree = false;            // built over a caller's octree (no CN/W)
+  bool octree_only = false;            // built without normals/areas: no expansion, no queries
   float bbox[6] = {0, 0, 0, 0, 0, 0};
   gwn::DevBuf<float> P, N, A;          // n x 3, n x 3, n (original order)
   gwn::DevBuf<float> Ps, NAs;          // points in octree order: xyz and N*A

@@ -129,9 +129,35 @@ def C_byref(x):
 
 
 class PointCloudFWN:
-    """Device octree + expansion coefficients of the point-cloud path."""
+    """Device octree + expansion coefficients of the point-cloud path.
 
-    def __init__(self, P, N, A, order=2):
+    PointCloudFWN(P, N, A, order) builds the octree (igl::octree) and the
+    precompute; with N and A None it holds the octree alone.
+    PointCloudFWN.from_octree(...) works over a caller's octree."""
+
+    def __init__(self, P, N=None, A=None, order=2):
+        P = _rows3(P, "P", np.float32)
+        if order not in (0, 1, 2):
+            raise ValueError("expansion_order must be 0, 1 or 2")
+        h = nat.vp()
+        if N is None and A is None:
+            check(lib().gwn_cloud_build(P, None, None, P.shape[0], order, C_byref(h)))
+        else:
+            N = _rows3(N, "N", np.float32)
+            A = np.ascontiguousarray(np.asarray(A).reshape(-1), dtype=np.float32)
+            if not (P.shape[0] == N.shape[0] == A.shape[0]):
+                raise ValueError("P, N, A sizes differ")
+            check(lib().gwn_cloud_build(P, N.ctypes.data, A.ctypes.data, P.shape[0], order, C_byref(h)))
+        self._h = h.value
+        self.order = order
+        self.n = P.shape[0]
+        self.user_octree = False
+
+    @classmethod
+    def from_octree(cls, P, N, A, point_indices, CH, order=2, CM=None, R=None, EC=None):
+        """Precompute (CM, R, EC None) or evaluation state (CM, R, EC given)
+        over a caller's octree: point_indices (one index list per cell) and
+        CH (ncells x 8, -1 for leaves), as igl::octree returns them."""
         P = _rows3(P, "P", np.float32)
         N = _rows3(N, "N", np.float32)
         A = np.ascontiguousarray(np.asarray(A).reshape(-1), dtype=np.float32)
@@ -139,11 +165,37 @@ class PointCloudFWN:
             raise ValueError("P, N, A sizes differ")
         if order not in (0, 1, 2):
             raise ValueError("expansion_order must be 0, 1 or 2")
+        CH = np.ascontiguousarray(np.asarray(CH).reshape(-1, 8), dtype=np.int32)
+        m = CH.shape[0]
+        if len(point_indices) != m:
+            raise ValueError("point_indices and CH disagree on the number of cells")
+        lens = np.fromiter((len(x) for x in point_indices), dtype=np.int64, count=m)
+        start = np.zeros(m + 1, np.int64)
+        np.cumsum(lens, out=start[1:])
+        flat = np.ascontiguousarray(
+            np.concatenate([np.asarray(x, dtype=np.int64).reshape(-1) for x in point_indices])
+            if m and start[-1] else np.zeros(1, np.int64), dtype=np.int32)
+        given = [x is not None for x in (CM, R, EC)]
+        if any(given) and not all(given):
+            raise ValueError("give all of CM, R, EC or none")
+        ptrs = [None, None, None]
+        keep = []
+        if all(given):
+            nt = {0: 3, 1: 12, 2: 39}[order]
+            CMa = np.ascontiguousarray(np.asarray(CM).reshape(m, 3), dtype=np.float32)
+            Ra = np.ascontiguousarray(np.asarray(R).reshape(m), dtype=np.float32)
+            ECa = np.ascontiguousarray(np.asarray(EC).reshape(m, nt), dtype=np.float32)
+            keep = [CMa, Ra, ECa]
+            ptrs = [x.ctypes.data for x in keep]
+        self = cls.__new__(cls)
         h = nat.vp()
-        check(lib().gwn_cloud_build(P, N, A, P.shape[0], order, C_byref(h)))
+        check(lib().gwn_cloud_build_from_octree(P, N, A, P.shape[0], CH, m, start, flat, order, ptrs[0], ptrs[1],
+                                                ptrs[2], C_byref(h)))
         self._h = h.value
         self.order = order
         self.n = P.shape[0]
+        self.user_octree = True
+        return self
 
     def __del__(self):
         try:
@@ -190,7 +242,7 @@ class PointCloudFWN:
 
 def octree(P):
     """igl::octree(P, point_indices, CH, CN, W) computed on the GPU (FP32)."""
-    c = PointCloudFWN(P, np.zeros_like(np.asarray(P, dtype=np.float32)), np.zeros(np.asarray(P).shape[0]), 0)
+    c = PointCloudFWN(P, None, None, 0)
     CH, CN, W, cs, cc, pl = c.octree()
     point_indices = [np.sort(pl[cs[i]:cs[i] + cc[i]]).tolist() for i in range(CH.shape[0])]
     return point_indices, CH, CN, W
@@ -218,15 +270,14 @@ def fast_winding_number(*args, **kw):
         return PointCloudFWN(P, N, A, 2).query(Q, 2.0)
     if n == 6 and isinstance(args[3], (list, tuple)):
         P, N, A, point_indices, CH, order = args
-        c = PointCloudFWN(P, N, A, order)
-        return c.expansion()
+        return PointCloudFWN.from_octree(P, N, A, point_indices, CH, order).expansion()
     if n == 6:
         P, N, A, Q, order, beta = args
         return PointCloudFWN(P, N, A, order).query(Q, beta)
     if n == 10:
         P, N, A, point_indices, CH, CM, R, EC, Q, beta = args
         order = {3: 0, 12: 1, 39: 2}[np.asarray(EC).shape[1]]
-        return PointCloudFWN(P, N, A, order).query(Q, beta)
+        return PointCloudFWN.from_octree(P, N, A, point_indices, CH, order, CM, R, EC).query(Q, beta)
     raise TypeError("no fast_winding_number overload takes these arguments")
 
 

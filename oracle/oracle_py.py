@@ -217,6 +217,33 @@ class Octree:
         m = max(self.t.npoint_list, 1)
         self.point_list = np.ctypeslib.as_array(self.t.point_list, shape=(m,))[: self.t.npoint_list].copy()
 
+    @classmethod
+    def from_lists(cls, CH, point_indices, dtype=np.float64):
+        """An octree given by the caller (CH ncells x 8 and one index list per
+        cell), for precompute/eval over arbitrary trees and list orders."""
+        self = cls.__new__(cls)
+        self.dt = dtype
+        self.sfx = "f64" if dtype == np.float64 else "f32"
+        self.CH = np.ascontiguousarray(CH, dtype=np.int32).reshape(-1, 8)
+        m = self.CH.shape[0]
+        self.cell_count = np.array([len(x) for x in point_indices], np.int32)
+        self.cell_start = np.zeros(m, np.int64)
+        self.cell_start[1:] = np.cumsum(self.cell_count, dtype=np.int64)[:-1]
+        flat = [np.asarray(x, np.int64).reshape(-1) for x in point_indices]
+        self.point_list = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros(0), dtype=np.int32)
+        self._pl = self.point_list if self.point_list.size else np.zeros(1, np.int32)
+        t = OctreeT()
+        t.ncells = m
+        t.CH = self.CH.ctypes.data_as(C.POINTER(C.c_int32))
+        t.CN = None
+        t.W = None
+        t.cell_start = self.cell_start.ctypes.data_as(C.POINTER(C.c_int64))
+        t.cell_count = self.cell_count.ctypes.data_as(C.POINTER(C.c_int32))
+        t.point_list = self._pl.ctypes.data_as(C.POINTER(C.c_int32))
+        t.npoint_list = int(self.point_list.size)
+        self.t_borrowed = t  # arrays owned by numpy: never released by the oracle
+        return self
+
     def point_indices(self, c):
         s = self.cell_start[c]
         return self.point_list[s: s + self.cell_count[c]]
@@ -225,6 +252,33 @@ class Octree:
         if getattr(self, "t", None) is not None:
             lib().oracle_octree_release(C.byref(self.t))
             self.t = None
+
+
+def cloud_over(P, N, A, octree, order=2, dtype=np.float32, CM=None, R=None, EC=None):
+    """Precompute (or take CM, R, EC as given) over an Octree.from_lists tree;
+    returns (CM, R, EC, eval) where eval(Q, beta) runs the restated
+    evaluation over that tree."""
+    sfx = "f64" if dtype == np.float64 else "f32"
+    P, N = _c(P, dtype), _c(N, dtype)
+    A = _c(A, dtype).reshape(-1)
+    t = octree.t_borrowed
+    m = t.ncells
+    nt = {0: 3, 1: 12, 2: 39}[order]
+    if CM is None:
+        CM = np.empty((m, 3), dtype)
+        R = np.empty(m, dtype)
+        EC = np.empty((m, nt), dtype)
+        getattr(lib(), f"oracle_cloud_precompute_{sfx}")(P, N, A, C.byref(t), order, CM, R, EC)
+    else:
+        CM, R, EC = _c(CM, dtype), _c(R, dtype).reshape(-1), _c(EC, dtype)
+
+    def ev(Q, beta=2.0, _keep=octree):  # _keep: the tree's arrays must outlive t
+        Q = _c(Q, dtype).reshape(-1, 3)
+        W = np.empty(Q.shape[0], dtype)
+        getattr(lib(), f"oracle_cloud_eval_{sfx}")(P, N, A, C.byref(t), order, CM, R, EC, Q, Q.shape[0],
+                                                   float(beta), W, 0)
+        return W
+    return CM, R, EC, ev
 
 
 class Cloud:

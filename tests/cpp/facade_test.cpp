@@ -69,6 +69,40 @@ int main() {
   const double gold[4] = {0.38779369004261133, -0.00041235296362485, -0.00362978253577090, -0.00041235296362485};
   for (int i = 0; i < 4; ++i)
     if (std::fabs(WN(i, 0) - gold[i]) > 1e-6 * 0.4) bad |= 16;
+  // winding_number(V,F,p): single point (winding_number.h:49-60)
+  Mat<double> pin(1, 3);
+  pin(0, 0) = 0.2, pin(0, 1) = 0.2, pin(0, 2) = 0.2;
+  if (std::fabs(igl::gwn_detail::winding_number_point(V, F, pin) - 1.0) > 1e-15) bad |= 64;
+  // octree -> precompute -> eval over the caller's octree reproduces the
+  // one-shot overload: octree.h:36-41, fast_winding_number.h:44-53 / 88-99
+  {
+    Mat<double> Pc(9, 3), Nc(9, 3), Ac(9, 1), Qq(3, 3);
+    for (int i = 0; i < 8; ++i) {
+      Pc(i, 0) = (i & 1) ? 1 : -1, Pc(i, 1) = (i & 2) ? 1 : -1, Pc(i, 2) = (i & 4) ? 1 : -1;
+      for (int j = 0; j < 3; ++j) Nc(i, j) = Pc(i, j) / std::sqrt(3.0);
+      Ac(i, 0) = 0.5;
+    }
+    Pc(8, 0) = 0.9, Pc(8, 1) = 0.8, Pc(8, 2) = 0.85;
+    Nc(8, 0) = 1, Nc(8, 1) = 0, Nc(8, 2) = 0;
+    Ac(8, 0) = 0.25;
+    const double qq[3][3] = {{0, 0, 0}, {3, 0.5, -1}, {0.95, 0.9, 0.9}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Qq(i, j) = qq[i][j];
+    std::vector<std::vector<int>> PI;
+    Mat<int> CH;
+    Mat<double> CN, Wc;
+    igl::gwn_detail::octree(Pc, PI, CH, CN, Wc);
+    // root: all 9 points, split into 8 children; the octant of (1,1,1) holds 2
+    if (CH.rows() < 9 || CH(0, 0) != 1 || PI[0].size() != 9 || PI[8].size() != 2 || std::fabs(Wc(0, 0) - 2) > 0)
+      bad |= 128;
+    Mat<double> CM, R, EC, W1, W2;
+    igl::gwn_detail::cloud_precompute(Pc, Nc, Ac, PI, CH, 2, CM, R, EC);
+    if (CM.rows() != CH.rows() || EC.cols() != 39 || R.rows() != CH.rows()) bad |= 256;
+    igl::gwn_detail::cloud_eval(Pc, Nc, Ac, PI, CH, CM, R, EC, Qq, 2.0, W1);
+    igl::gwn_detail::cloud(Pc, Nc, Ac, Qq, 2, 2.0, W2);
+    for (int i = 0; i < 3; ++i)
+      if (std::fabs(W1(i, 0) - W2(i, 0)) > 1e-6) bad |= 512;
+  }
   // bad input is reported, not asserted
   Mat<int> Fbad(1, 3);
   Fbad(0, 0) = 0, Fbad(0, 1) = 1, Fbad(0, 2) = 9;

@@ -134,3 +134,118 @@ def test_cloud_workload_full_size():
     assert err.max() <= 1e-4
     r = np.linalg.norm(Q, axis=1)
     assert np.all(np.abs(W[r < 0.8] - 1) < 0.05) and np.all(np.abs(W[r > 1.2]) < 0.05)
+
+
+# ---- precompute / evaluation over a caller's octree (fast_winding_number.h:44-53, 88-99) ----
+
+def _lists(o):
+    return [o.point_indices(c).tolist() for c in range(o.CH.shape[0])]
+
+
+def test_from_octree_reference_lists():
+    """The reference's own octree and lists: CM, R bit-identical to the
+    oracle's precompute, EC within 2e-5 relative, W within 1e-4."""
+    P, N, A = _cloud(20000, seed=21)
+    o = O.Octree(P, np.float32)
+    lists = _lists(o)
+    c = gwn.PointCloudFWN.from_octree(P, N, A, lists, o.CH, 2)
+    CM, R, EC = c.expansion()
+    ot = O.Octree.from_lists(o.CH, lists, np.float32)
+    CMo, Ro, ECo, ev = O.cloud_over(P, N, A, ot, 2)
+    ne = o.cell_count > 0
+    assert np.array_equal(CM[ne], CMo[ne]) and np.array_equal(R[ne], Ro[ne])
+    scale = np.abs(ECo[ne]).max(axis=1, keepdims=True) + 1e-30
+    assert (np.abs(EC[ne] - ECo[ne]) / scale).max() < 2e-5
+    Q = np.random.default_rng(3).uniform(-1.3, 1.3, (4000, 3)).astype(np.float32)
+    W = c.query(Q, 2.0)
+    assert np.abs(W - ev(Q, 2.0)).max() <= 1e-4
+    # same tree and lists as the library's own octree: same answers
+    W_own = gwn.PointCloudFWN(P, N, A, 2).query(Q, 2.0)
+    assert np.abs(W - W_own).max() <= 1e-5
+
+
+def test_from_octree_list_order_is_summation_order():
+    """Centres are summed over each cell's list in list order (FP32, as
+    fast_winding_number.cpp:60-73): shuffled lists give the oracle's shuffled
+    centres bit for bit."""
+    P, N, A = _cloud(20000, seed=22)
+    o = O.Octree(P, np.float32)
+    rng = np.random.default_rng(5)
+    lists = [rng.permutation(o.point_indices(c)).tolist() for c in range(o.CH.shape[0])]
+    c = gwn.PointCloudFWN.from_octree(P, N, A, lists, o.CH, 1)
+    CM, R, EC = c.expansion()
+    CMo, Ro, ECo, ev = O.cloud_over(P, N, A, O.Octree.from_lists(o.CH, lists, np.float32), 1)
+    CMa, _, _, _ = O.cloud_over(P, N, A, O.Octree.from_lists(o.CH, _lists(o), np.float32), 1)
+    ne = o.cell_count > 0
+    assert np.array_equal(CM[ne], CMo[ne]) and np.array_equal(R[ne], Ro[ne])
+    assert not np.array_equal(CMo[ne], CMa[ne])  # the order is visible in the last bits
+    Q = np.random.default_rng(6).uniform(-1.3, 1.3, (2000, 3)).astype(np.float32)
+    assert np.abs(c.query(Q, 2.0) - ev(Q, 2.0)).max() <= 1e-4
+
+
+def test_from_octree_given_expansion_is_used():
+    """CM, R, EC passed to the evaluation overload are used as given (here
+    scaled so they differ from any precompute)."""
+    P, N, A = _cloud(5000, seed=23)
+    o = O.Octree(P, np.float32)
+    lists = _lists(o)
+    ot = O.Octree.from_lists(o.CH, lists, np.float32)
+    CMo, Ro, ECo, _ = O.cloud_over(P, N, A, ot, 2)
+    CMg, Rg, ECg = CMo.copy(), (Ro * 0.75).astype(np.float32), (ECo * 1.5).astype(np.float32)
+    *_, ev = O.cloud_over(P, N, A, ot, 2, CM=CMg, R=Rg, EC=ECg)
+    Q = np.random.default_rng(7).uniform(-1.5, 1.5, (3000, 3)).astype(np.float32)
+    W = gwn.fast_winding_number(P, N, A, lists, o.CH, CMg, Rg, ECg, Q, 2.0)
+    Wo = ev(Q, 2.0)
+    assert np.abs(W - Wo).max() <= 1.5e-4
+    assert np.abs(W - gwn.PointCloudFWN(P, N, A, 2).query(Q, 2.0)).max() > 1e-3  # the scaling shows
+    # precompute overload returns the same CM/R as the oracle's over these lists
+    CM, R, EC = gwn.fast_winding_number(P, N, A, lists, o.CH, 2)
+    ne = o.cell_count > 0
+    assert np.array_equal(CM[ne], CMo[ne]) and np.array_equal(R[ne], Ro[ne])
+
+
+def test_from_octree_root_leaf_and_custom_tree():
+    P, N, A = _cloud(300, seed=24)
+    Q = np.random.default_rng(8).uniform(-1.5, 1.5, (500, 3)).astype(np.float32)
+    # a single leaf holding every point: brute force over its list
+    CH = -np.ones((1, 8), np.int32)
+    lists = [list(range(300))]
+    W = gwn.PointCloudFWN.from_octree(P, N, A, lists, CH, 2).query(Q, 2.0)
+    *_, ev = O.cloud_over(P, N, A, O.Octree.from_lists(CH, lists, np.float32), 2)
+    assert np.abs(W - ev(Q, 2.0)).max() <= 1e-5
+    # a hand-made two-level tree that is no geometric octree (points dealt
+    # round-robin to the 8 children), with one empty child
+    CH = -np.ones((9, 8), np.int32)
+    CH[0] = np.arange(1, 9)
+    lists = [list(range(300))] + [list(range(k, 300, 7)) for k in range(7)] + [[]]
+    c = gwn.PointCloudFWN.from_octree(P, N, A, lists, CH, 2)
+    *_, ev = O.cloud_over(P, N, A, O.Octree.from_lists(CH, lists, np.float32), 2)
+    for beta in (0.5, 2.0, 8.0):
+        assert np.abs(c.query(Q, beta) - ev(Q, beta)).max() <= 1e-4
+
+
+def test_from_octree_rejects_bad_trees():
+    P, N, A = _cloud(50, seed=25)
+    lists = [list(range(50))] + [[]] * 8
+    CH = -np.ones((9, 8), np.int32)
+    CH[0] = np.arange(1, 9)
+    CH[1] = [0, 2, 3, 4, 5, 6, 7, 8]  # cycle back to the root
+    with pytest.raises(gwn.GWNError):
+        gwn.PointCloudFWN.from_octree(P, N, A, lists, CH, 2)
+    CH[1] = -1
+    CH[0, 3] = -1  # split cell with a missing child
+    with pytest.raises(gwn.GWNError):
+        gwn.PointCloudFWN.from_octree(P, N, A, lists, CH, 2)
+    CH[0, 3] = 4
+    CH[0] = np.arange(1, 9)
+    bad = [list(range(50)) + [50]] + [[]] * 8  # point index out of range
+    with pytest.raises(gwn.GWNError):
+        gwn.PointCloudFWN.from_octree(P, N, A, bad, CH, 2)
+    # an octree built without normals/areas holds no expansion
+    c = gwn.PointCloudFWN(P)
+    with pytest.raises(gwn.GWNError):
+        c.query(P[:3])
+    pi, CH2, CN2, W2 = gwn.octree(P)
+    o = O.Octree(P, np.float32)
+    assert np.array_equal(CH2, o.CH) and np.array_equal(CN2, o.CN) and np.array_equal(W2, o.W)
+    assert all(pi[k] == o.point_indices(k).tolist() for k in range(len(pi)))

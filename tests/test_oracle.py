@@ -194,3 +194,20 @@ def test_parallel_for_slicing_matches_serial():
     s = O.Soup(V, F)
     Q = np.random.default_rng(1).uniform(-1.5, 1.5, (3000, 3))
     assert np.array_equal(s.query(Q, 2.0, nthreads=1), s.query(Q, 2.0, nthreads=7))
+
+
+def test_oracle_octree_from_lists_matches_own_octree():
+    """Plumbing of caller-made octrees: the oracle's own octree passed back as
+    plain lists gives the same precompute and evaluation bit for bit."""
+    rng = np.random.default_rng(3)
+    P = rng.normal(size=(3000, 3)).astype(np.float32)
+    N = (P / np.linalg.norm(P, axis=1, keepdims=True)).astype(np.float32)
+    A = np.full(3000, 1e-3, np.float32)
+    c = O.Cloud(P, N, A, 2, np.float32)
+    o = c.oct
+    lists = [o.point_indices(k).tolist() for k in range(o.CH.shape[0])]
+    CM, R, EC, ev = O.cloud_over(P, N, A, O.Octree.from_lists(o.CH, lists, np.float32), 2)
+    ne = o.cell_count > 0
+    assert np.array_equal(CM[ne], c.CM[ne]) and np.array_equal(R[ne], c.R[ne]) and np.array_equal(EC[ne], c.EC[ne])
+    Q = rng.uniform(-2, 2, (200, 3)).astype(np.float32)
+    assert np.array_equal(ev(Q, 2.0), c.eval(Q, 2.0))
