@@ -36,8 +36,8 @@ FLOP_EXACT = 76   # exact triangle (6078-6109)
 # HBM bytes moved per k_traverse launch on the default workload (1M queries,
 # SAH4 tree), keyed by accuracy_scale: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
-# for a 256 MiB device copy (fetch x2.000, write x1.000).
-MEASURED_TRAFFIC = {5.0: int((155.0 + 33.3) * 2**20)}
+# for a 256 MiB device copy.  Empty until measured for the current kernel.
+MEASURED_TRAFFIC = {}
 
 
 def log(*a):
@@ -56,6 +56,7 @@ def main():
                     help="skip the accuracy sweep and time this accuracy_scale only (profiling runs)")
     ap.add_argument("--tree", choices=["sah4", "lbvh"], default="sah4")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 1.3M-triangle mixed vs FP64 comparison")
+    ap.add_argument("--no-rows", action="store_true", help="skip the cloud / exact / signed-distance legs")
     ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter calibration)")
@@ -263,6 +264,14 @@ def main():
         except Exception as e:
             log(f"[bench] mixed workload skipped: {e}")
 
+    others = {}
+    if not args.no_rows:
+        for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance)):
+            try:
+                others[name] = fn(args.no_cpu)
+            except Exception as e:
+                log(f"[bench] {name} skipped: {e}")
+
     serving = None
     if args.serving_duration > 0:
         try:
@@ -286,6 +295,7 @@ def main():
                  "kernel_ms": round(kd_ms, 4)},
         "build_s": round(build_s, 4), "sort_ms": round(sort_ms, 4),
         "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "mixed_vs_fp64": mixed_cfg4,
+        "other_rows": others,
     }
     print(json.dumps(out), flush=True)
 
@@ -355,6 +365,149 @@ def run_mixed_workload(beta_grid):
         log(f"[bench] cfg4 {name}: {out[name]}")
     out["mixed_over_fp64"] = round(out["mixed"]["queries_per_s"] / out["fp64"]["queries_per_s"], 3)
     del dQ, dW, dV, dF, dQs, dWx
+    return out
+
+
+def _kernel_ms(name):
+    import ctypes
+
+    from libigl_amd import _native as nat
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    nat.lib().gwn_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value / max(n.value, 1)
+
+
+def _timed_device(fn, steps, kernel):
+    """warm-up call, then `steps` calls between synchronizations; returns
+    (seconds per call, average kernel ms from HIP events)."""
+    import torch
+
+    from libigl_amd import _native as nat
+    fn()
+    torch.cuda.synchronize()
+    nat.lib().gwn_profile_reset()
+    nat.lib().gwn_profile_enable(1)
+    t = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    nat.lib().gwn_profile_enable(0)
+    return dt, _kernel_ms(kernel)
+
+
+def run_cloud(no_cpu):
+    """Point-cloud path: 2M oriented points on the noisy level-5 sphere, 160^3
+    grid queries, order 2, beta 2; W checked against the oracle on a sample."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import _native as nat
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+
+    P, N, A, Q = D.cloud_workload()
+    tb = time.perf_counter()
+    c = gwn.PointCloudFWN(P, N, A, 2)
+    build_s = time.perf_counter() - tb
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    nq = Q.shape[0]
+    dQ = torch.from_numpy(Q.astype(np.float32)).to(dev)
+    dW = torch.empty(nq, dtype=torch.float32, device=dev)
+    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_cloud_query_device(c._h, dQ.data_ptr(), nq, 2.0,
+                                                                               dW.data_ptr(), st)),
+                            5, "cloud_traverse")
+    W = dW.cpu().numpy()
+    out = {"workload": "2M points, 160^3 grid queries, order 2, beta 2", "cells": int(c.ncells()),
+           "build_s": round(build_s, 4), "queries_per_s": round(nq / dt, 1), "kernel_ms": round(kms, 4)}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        oc = O.Cloud(P.astype(np.float32), N.astype(np.float32), A.astype(np.float32), 2, np.float32)
+        sub = np.arange(0, nq, nq // 20_000)
+        threads = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        Wo = oc.eval(Q[sub].astype(np.float32), 2.0, threads)
+        cdt = time.perf_counter() - t
+        out["max_abs_dw_vs_oracle"] = float(np.abs(W[sub] - Wo).max())
+        out["cpu_baseline"] = {"value": round(sub.size / cdt, 1), "unit": "queries/s", "cores": threads,
+                               "kind": "port", "sample": f"{sub.size} grid queries, oracle restatement"}
+    log(f"[bench] cloud: {out}")
+    return out
+
+
+def run_exact(no_cpu):
+    """Exact FP64 winding number: level-5 noisy icosphere (20,480 triangles),
+    100k queries uniform in [-1.5,1.5]^3."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import _native as nat
+    from libigl_amd import datagen as D
+
+    V, F, Q = D.exact_workload()
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    dV = torch.from_numpy(np.ascontiguousarray(V)).to(dev)
+    dF = torch.from_numpy(F).to(dev)
+    dQ = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
+    dW = torch.empty(Q.shape[0], dtype=torch.float64, device=dev)
+    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_exact_winding_number_device(
+        dV.data_ptr(), V.shape[0], dF.data_ptr(), F.shape[0], dQ.data_ptr(), Q.shape[0], dW.data_ptr(), st)),
+        5, "exact")
+    W = dW.cpu().numpy()
+    pairs = Q.shape[0] * F.shape[0]
+    out = {"workload": "20,480 tris, 100k queries, FP64", "queries_per_s": round(Q.shape[0] / dt, 1),
+           "kernel_ms": round(kms, 4), "pairs_per_s": round(pairs / (kms * 1e-3), 1)}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        sub = np.arange(0, Q.shape[0], 50)
+        threads = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        Wo = O.winding_number_bf(V, F, Q[sub], threads)
+        cdt = time.perf_counter() - t
+        out["max_abs_dw_vs_oracle"] = float(np.abs(W[sub] - Wo).max())
+        out["cpu_baseline"] = {"value": round(sub.size / cdt, 1), "unit": "queries/s", "cores": threads,
+                               "kind": "port", "sample": f"{sub.size} queries, brute-force FP64 restatement"}
+    log(f"[bench] exact: {out}")
+    return out
+
+
+def run_signed_distance(no_cpu):
+    """igl::signed_distance with the fast-winding-number sign on the 81,920-
+    triangle workload mesh, 1M points (closest point + FWN sign)."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import _native as nat
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+
+    V, F, Q = D.soup_workload()
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    n = Q.shape[0]
+    dP = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
+    dS = torch.empty(n, dtype=torch.float64, device=dev)
+    dI = torch.empty(n, dtype=torch.int32, device=dev)
+    dC = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    lo, hi = gwn.DBL_MIN, gwn.DBL_MAX
+    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_signed_distance_device(
+        bvh.handle, dP.data_ptr(), n, gwn.SIGN_FAST_WINDING_NUMBER, lo, hi, dS.data_ptr(), dI.data_ptr(),
+        dC.data_ptr(), st)), 3, "closest_point")
+    S = dS.cpu().numpy()
+    out = {"workload": "81,920 tris, 1M points, FAST_WINDING_NUMBER sign", "points_per_s": round(n / dt, 1),
+           "closest_point_kernel_ms": round(kms, 4)}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        sub = np.arange(0, n, 2500)
+        So, _, _ = O.signed_distance(V, F, Q[sub], gwn.SIGN_FAST_WINDING_NUMBER)
+        out["max_rel_dS_vs_oracle"] = float(np.max(np.abs(S[sub] - So) / np.maximum(np.abs(So), 1e-12)))
+    log(f"[bench] signed distance: {out}")
     return out
 
 

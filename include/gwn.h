@@ -175,6 +175,32 @@ int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta,
 int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Signed distance (signed_distance.h:61-89, signed_distance.cpp:28-140):
+ * closest point on the soup (FP64, from the vertices as given) and a sign
+ * from the winding number.  Sign types carry the reference's enum values.
+ */
+typedef enum {
+  GWN_SIGN_PSEUDONORMAL = 0,        /* not supported here: GWN_ERR_INVALID */
+  GWN_SIGN_WINDING_NUMBER = 1,      /* s = 1 - 2 w, w the exact winding number */
+  GWN_SIGN_DEFAULT = 2,             /* as WINDING_NUMBER */
+  GWN_SIGN_UNSIGNED = 3,            /* s = 1 */
+  GWN_SIGN_FAST_WINDING_NUMBER = 4  /* s = 1 - 2 |w|, w = fast winding number at accuracy_scale 2 */
+} gwn_sign_type;
+
+/* igl::signed_distance(P,V,F,sign_type,lower_bound,upper_bound,S,I,C,N) over
+ * the soup's mesh (the FWN sign uses the soup's tree and expansion order; the
+ * reference builds an order-2 tree).  P np x 3; S np (s * distance), I np
+ * (closest face), C np x 3 (closest point).  A point whose squared distance
+ * lies outside [low^2, up^2) derived from the bounds as signed_distance.cpp:79-84
+ * does gets S = NaN, I = nf + 1, C = 0; the reference's unbounded overload
+ * (signed_distance.h:81-89) is lower = DBL_MIN, upper = DBL_MAX.  An empty
+ * mesh gives S = NaN, I = -1, C = NaN.  N (pseudonormals) is not produced. */
+int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                        double upper_bound, double* S, int32_t* I, double* C);
+int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                               double upper_bound, double* S, int32_t* I, double* C, void* stream);
+
+/* ------------------------------------------------------------------------
  * Serving: continuous batching of soup queries against resident meshes (no
  * counterpart in the reference, whose callers such as
  * signed_distance.cpp:50-121 query one point at a time on the host).

@@ -15,6 +15,13 @@
 //   igl::fast_winding_number(P,N,A,Q,order,beta,WN)   fast_winding_number.h:111-118
 //   igl::fast_winding_number(P,N,A,Q,WN)              fast_winding_number.h:126-131
 //   struct igl::FastWindingNumberBVH                  fast_winding_number.h:140-152
+//   igl::SignedDistanceType                           signed_distance.h:20-34
+//   igl::signed_distance(P,V,F,type,lower,upper,S,I,C,N)  signed_distance.h:61-71
+//   igl::signed_distance(P,V,F,type,S,I,C,N)          signed_distance.h:81-89
+//   igl::signed_distance_fast_winding_number(P,V,F,tree,fwn_bvh,S)
+//                                                     signed_distance.h (batch, .cpp:665-687)
+// (signed distance for the unsigned, winding-number and fast-winding-number
+// sign types; the pseudonormal type throws)
 // A program switches by including this header instead of the libigl ones and
 // linking libgwn.so.  Matrices of any storage order and scalar type are
 // accepted (converted to the row-major float/double buffers of the C ABI);
@@ -27,6 +34,7 @@
 #pragma once
 #include <algorithm>
 #include <cstdint>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <type_traits>
@@ -36,6 +44,15 @@
 #include "gwn.h"
 
 namespace igl {
+
+enum SignedDistanceType {
+  SIGNED_DISTANCE_TYPE_PSEUDONORMAL = GWN_SIGN_PSEUDONORMAL,
+  SIGNED_DISTANCE_TYPE_WINDING_NUMBER = GWN_SIGN_WINDING_NUMBER,
+  SIGNED_DISTANCE_TYPE_DEFAULT = GWN_SIGN_DEFAULT,
+  SIGNED_DISTANCE_TYPE_UNSIGNED = GWN_SIGN_UNSIGNED,
+  SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER = GWN_SIGN_FAST_WINDING_NUMBER,
+  NUM_SIGNED_DISTANCE_TYPE = 5
+};
 
 // igl::FastWindingNumberBVH: owns the GPU tree.  Like the reference's (which
 // holds raw pointers into itself) it is neither copyable nor movable.
@@ -272,6 +289,34 @@ void cloud_eval(const MP& P, const MN& N, const MA& A, const std::vector<std::ve
     WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
 }
 
+// signed distance over a built soup; S, I, C resized as the reference does
+// (N is only written by the pseudonormal type, which is not provided here)
+template <typename MP, typename MS, typename MI, typename MC>
+void signed_distance_bvh(gwn_soup* h, const MP& P, int type, double lower, double upper, MS& S, MI& I, MC& C) {
+  const auto p = rows3<double>(P, "P");
+  const int64_t n = P.rows();
+  std::vector<double> s((size_t)n), c(3 * (size_t)n);
+  std::vector<int32_t> i((size_t)n);
+  if (n > 0) check(gwn_signed_distance(h, p.data(), n, type, lower, upper, s.data(), i.data(), c.data()));
+  S.resize(n, 1);
+  I.resize(n, 1);
+  C.resize(n, 3);
+  for (int64_t k = 0; k < n; ++k) {
+    S(k, 0) = static_cast<typename std::decay<decltype(S(0, 0))>::type>(s[k]);
+    I(k, 0) = static_cast<typename std::decay<decltype(I(0, 0))>::type>(i[k]);
+    for (int j = 0; j < 3; ++j) C(k, j) = static_cast<typename std::decay<decltype(C(0, 0))>::type>(c[3 * k + j]);
+  }
+}
+
+template <typename MP, typename MV, typename MF, typename MS, typename MI, typename MC>
+void signed_distance(const MP& P, const MV& V, const MF& F, int type, double lower, double upper, MS& S, MI& I,
+                     MC& C) {
+  if (type == GWN_SIGN_PSEUDONORMAL) throw std::runtime_error("gwn: pseudonormal signed distance is not provided");
+  FastWindingNumberBVH bvh;  // signed_distance.cpp:65-67 builds an order-2 tree for the FWN sign
+  build(V, F, 2, bvh);
+  signed_distance_bvh(bvh.handle, P, type, lower, upper, S, I, C);
+}
+
 }  // namespace gwn_detail
 }  // namespace igl
 
@@ -321,6 +366,46 @@ void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::Matr
                          Eigen::PlainObjectBase<DerivedWN>& WN) {
   gwn_detail::cloud_eval(P.derived(), N.derived(), A.derived(), point_indices, CH.derived(), CM.derived(),
                          R.derived(), EC.derived(), Q.derived(), (double)beta, WN.derived());
+}
+
+template <typename DerivedP, typename DerivedV, typename DerivedF, typename DerivedS, typename DerivedI,
+          typename DerivedC, typename DerivedN>
+void signed_distance(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>& V,
+                     const Eigen::MatrixBase<DerivedF>& F, const SignedDistanceType sign_type,
+                     const typename DerivedV::Scalar lower_bound, const typename DerivedV::Scalar upper_bound,
+                     Eigen::PlainObjectBase<DerivedS>& S, Eigen::PlainObjectBase<DerivedI>& I,
+                     Eigen::PlainObjectBase<DerivedC>& C, Eigen::PlainObjectBase<DerivedN>& N) {
+  (void)N;
+  gwn_detail::signed_distance(P.derived(), V.derived(), F.derived(), (int)sign_type, (double)lower_bound,
+                              (double)upper_bound, S.derived(), I.derived(), C.derived());
+}
+
+template <typename DerivedP, typename DerivedV, typename DerivedF, typename DerivedS, typename DerivedI,
+          typename DerivedC, typename DerivedN>
+void signed_distance(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>& V,
+                     const Eigen::MatrixBase<DerivedF>& F, const SignedDistanceType sign_type,
+                     Eigen::PlainObjectBase<DerivedS>& S, Eigen::PlainObjectBase<DerivedI>& I,
+                     Eigen::PlainObjectBase<DerivedC>& C, Eigen::PlainObjectBase<DerivedN>& N) {
+  typedef typename DerivedV::Scalar Scalar;  // signed_distance.cpp:436-439
+  signed_distance(P, V, F, sign_type, std::numeric_limits<Scalar>::min(), std::numeric_limits<Scalar>::max(), S, I,
+                  C, N);
+}
+
+// signed_distance.cpp:665-687: the caller's tree is not needed (the closest
+// points use fwn_bvh's own tree); S = sqrt(d2) (1 - 2 |w|)
+template <typename DerivedP, typename DerivedV, typename DerivedF, typename Tree, typename DerivedS>
+void signed_distance_fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>& V,
+                                         const Eigen::MatrixBase<DerivedF>& F, const Tree& tree,
+                                         const FastWindingNumberBVH& fwn_bvh, Eigen::PlainObjectBase<DerivedS>& S) {
+  (void)V;
+  (void)F;
+  (void)tree;
+  if (!fwn_bvh.handle) throw std::runtime_error("FastWindingNumberBVH not built");
+  Eigen::Matrix<int, Eigen::Dynamic, 1> I;
+  Eigen::Matrix<double, Eigen::Dynamic, 3> C;
+  gwn_detail::signed_distance_bvh(fwn_bvh.handle, P.derived(), GWN_SIGN_FAST_WINDING_NUMBER,
+                                  std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), S.derived(),
+                                  I, C);
 }
 
 template <typename DerivedV, typename DerivedF>

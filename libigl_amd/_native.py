@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libgwn.so")
 
 GWN_OK, GWN_ERR_INVALID, GWN_ERR_HIP, GWN_ERR_NOMEM, GWN_ERR_NODEVICE = 0, 1, 2, 3, 4
 TREE_SAH4, TREE_LBVH = 0, 1
+SIGN_PSEUDONORMAL, SIGN_WINDING_NUMBER, SIGN_DEFAULT, SIGN_UNSIGNED, SIGN_FAST_WINDING_NUMBER = 0, 1, 2, 3, 4
 PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -67,6 +68,8 @@ SIGNATURES = {
     "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
     "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
     "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),
+    "gwn_signed_distance": (C.c_int, [vp, f64p, C.c_int64, C.c_int, C.c_double, C.c_double, f64p, i32p, f64p]),
+    "gwn_signed_distance_device": (C.c_int, [vp, vp, C.c_int64, C.c_int, C.c_double, C.c_double, vp, vp, vp, vp]),
     "gwn_server_create": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     "gwn_server_destroy": (None, [vp]),
     "gwn_server_add_mesh": (C.c_int, [vp, f64p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int,

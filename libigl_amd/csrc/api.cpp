@@ -67,8 +67,8 @@ static void check_faces(const int32_t* F, int64_t nf, int64_t nv) {
     GWN_CHECK(F[i] >= 0 && F[i] < nv, GWN_ERR_INVALID, "face index out of range at F[" + std::to_string(i) + "]");
 }
 
-static void soup_build_common(gwn_soup* s, std::vector<float>& U, const int32_t* F, int64_t nv, int64_t nf,
-                              int order, int tree_kind) {
+static void soup_build_common(gwn_soup* s, std::vector<float>& U, const double* V64, const int32_t* F, int64_t nv,
+                              int64_t nf, int order, int tree_kind) {
   GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
   GWN_CHECK(tree_kind == GWN_TREE_SAH4 || tree_kind == GWN_TREE_LBVH, GWN_ERR_INVALID, "unknown tree kind");
   GWN_CHECK(nv >= 0 && nf >= 0 && nf < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad sizes");
@@ -94,6 +94,12 @@ static void soup_build_common(gwn_soup* s, std::vector<float>& U, const int32_t*
     s->bbox[3 + a] = std::isfinite(hi[a]) ? hi[a] : 0.0f;
   }
   upload(s->U, U.data(), U.size(), st);
+  if (V64) {
+    upload(s->V64, V64, 3 * (size_t)nv, st);
+  } else {
+    std::vector<double> w(U.begin(), U.end());
+    upload(s->V64, w.data(), w.size(), st);
+  }
   upload(s->F, F, 3 * (size_t)nf, st);
   soup_setup(s, st);
   if (nf == 0) {
@@ -180,7 +186,7 @@ int gwn_soup_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, in
     for (size_t i = 0; i < U.size(); ++i) U[i] = (float)V[i];
     gwn_soup* s = new gwn_soup();
     try {
-      soup_build_common(s, U, F, nv, nf, order, tree_kind);
+      soup_build_common(s, U, V, F, nv, nf, order, tree_kind);
     } catch (...) {
       delete s;
       throw;
@@ -198,7 +204,7 @@ int gwn_soup_build_f32(const float* V, int64_t nv, const int32_t* F, int64_t nf,
     std::vector<float> U(V, V + 3 * (size_t)nv);
     gwn_soup* s = new gwn_soup();
     try {
-      soup_build_common(s, U, F, nv, nf, order, tree_kind);
+      soup_build_common(s, U, nullptr, F, nv, nf, order, tree_kind);
     } catch (...) {
       delete s;
       throw;
@@ -222,7 +228,7 @@ int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info) {
     info->depth = s->depth;
     info->device_bytes = (int64_t)(s->U.bytes() + s->F.bytes() + s->tri.bytes() + s->boxes.bytes() +
                                    s->nodes.bytes() + s->boxdata.bytes() + s->slots.bytes() + s->slots64.bytes() +
-                                   s->work.bytes());
+                                   s->V64.bytes() + s->dbox.bytes() + s->work.bytes());
     std::memcpy(info->bbox, s->bbox, sizeof(s->bbox));
   });
 }
@@ -310,6 +316,41 @@ int gwn_soup_copy_boxdata(const gwn_soup* s, float* out) {
     GWN_CHECK(s && out, GWN_ERR_INVALID, "null argument");
     if (s->nnodes)
       GWN_HIP(hipMemcpy(out, s->boxdata.p, sizeof(float) * 92 * (size_t)s->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+// ---------------- signed distance ----------------
+static void sd_check(gwn_soup* s, int64_t np, int sign_type, const double* P, double* S, int32_t* I, double* C) {
+  GWN_CHECK(s != nullptr, GWN_ERR_INVALID, "null handle");
+  GWN_CHECK(np >= 0 && np < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+  GWN_CHECK(np == 0 || (P && S && I && C), GWN_ERR_INVALID, "null input/output");
+  GWN_CHECK(sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT || sign_type == GWN_SIGN_UNSIGNED ||
+                sign_type == GWN_SIGN_FAST_WINDING_NUMBER,
+            GWN_ERR_INVALID, "sign type not supported (pseudonormal signs are not implemented)");
+}
+
+int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                               double upper_bound, double* S, int32_t* I, double* C, void* stream) {
+  return guard([&] {
+    sd_check(s, np, sign_type, P, S, I, C);
+    signed_distance_device(s, P, np, sign_type, lower_bound, upper_bound, S, I, C, (hipStream_t)stream);
+  });
+}
+
+int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                        double upper_bound, double* S, int32_t* I, double* C) {
+  return guard([&] {
+    sd_check(s, np, sign_type, P, S, I, C);
+    if (np == 0) return;
+    hipStream_t st = nullptr;
+    DevBuf<double> dP, dS(np), dC(3 * (size_t)np);
+    DevBuf<int32_t> dI(np);
+    upload(dP, P, 3 * (size_t)np, st);
+    signed_distance_device(s, dP.p, np, sign_type, lower_bound, upper_bound, dS.p, dI.p, dC.p, st);
+    GWN_HIP(hipMemcpyAsync(S, dS.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipMemcpyAsync(I, dI.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
   });
 }
 

@@ -509,6 +509,15 @@ __device__ __forceinline__ float expansion(float lx, float ly, float lz, const f
   return wn;
 }
 
+// Wave-coherent rope walk (the scheme of the soup traversal): each lane keeps
+// `skip`, the next record it needs; the wave walks one shared index and a lane
+// takes part in a record when the index reaches its skip.  The wave steps to
+// i+1 when some lane enters the cell (or at a leaf) and to the cell's skip
+// otherwise; every other lane needs a record at or beyond it, the subtrees of
+// the rope being nested intervals.  Each lane so visits the records of its own
+// walk in order (results independent of the other queries in its wave) while
+// every record, leaf point and coefficient a wave reads is the same for all
+// its lanes and is fetched once through the scalar cache.
 template <int NT>
 __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict__ head, const uint2* __restrict__ meta,
                                                         uint32_t nrec, const float* __restrict__ EC,
@@ -525,28 +534,31 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
     for (int64_t j = 0; j < npts; ++j)
       wn += dipole(Ps[3 * j] - qx, Ps[3 * j + 1] - qy, Ps[3 * j + 2] - qz, NAs[3 * j], NAs[3 * j + 1], NAs[3 * j + 2]);
   } else {
-    uint32_t i = 0;
+    uint32_t i = 0, skip = 0;
     while (i < nrec) {
       const uint2 m = meta[i];
-      if (m.x & REC_LEAF) {
-        const int64_t s = m.x & ~REC_LEAF;
-        for (uint32_t j = 0; j < m.y; ++j) {
-          const int64_t k = s + j;
-          wn += dipole(Ps[3 * k] - qx, Ps[3 * k + 1] - qy, Ps[3 * k + 2] - qz, NAs[3 * k], NAs[3 * k + 1],
-                       NAs[3 * k + 2]);
+      const bool leaf = (m.x & REC_LEAF) != 0;
+      if (skip == i) {
+        skip = i + 1;
+        if (leaf) {
+          const int64_t s = m.x & ~REC_LEAF;
+          for (uint32_t j = 0; j < m.y; ++j) {
+            const int64_t k = s + j;
+            wn += dipole(Ps[3 * k] - qx, Ps[3 * k + 1] - qy, Ps[3 * k + 2] - qz, NAs[3 * k], NAs[3 * k + 1],
+                         NAs[3 * k + 2]);
+          }
+        } else {
+          const float4 h = head[i];
+          const float lx = h.x - qx, ly = h.y - qy, lz = h.z - qz;
+          const float nrm = sqrtf(lx * lx + ly * ly + lz * lz);
+          if ((double)nrm > beta * (double)h.w) {
+            wn += expansion<NT>(lx, ly, lz, EC + (int64_t)NT * m.y);
+            skip = m.x;
+          }
         }
-        ++i;
-        continue;
       }
-      const float4 h = head[i];
-      const float lx = h.x - qx, ly = h.y - qy, lz = h.z - qz;
-      const float nrm = sqrtf(lx * lx + ly * ly + lz * lz);
-      if ((double)nrm > beta * (double)h.w) {
-        wn += expansion<NT>(lx, ly, lz, EC + (int64_t)NT * m.y);
-        i = m.x;
-      } else {
-        ++i;
-      }
+      const bool step = leaf || __ballot(skip == i + 1) != 0;
+      i = __builtin_amdgcn_readfirstlane(step ? i + 1 : m.x);
     }
   }
   WN[qi] = wn;

@@ -1,0 +1,260 @@
+// distance.hip -- closest points on a triangle soup and igl::signed_distance
+// (signed_distance.cpp:28-140, 665-712) for the unsigned, winding-number and
+// fast-winding-number sign types.
+//
+// The closest point uses the soup's own rope (the FWN tree) with a bounding
+// box per slot record (soup_ensure_dist), walked wave-coherently like the
+// winding-number traversal: each lane keeps the next record it needs and the
+// wave steps through one shared index, so each lane visits the records of its
+// own depth-first branch-and-bound walk in order and its result does not
+// depend on which other queries share its wave.  Distances are FP64 from the
+// FP64 vertices, as the reference computes them for double V.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "gwn_internal.h"
+
+namespace gwn {
+namespace {
+
+constexpr uint32_t DL_LEAF = 0x80000000u;  // same tag as the rope links
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Closest point of triangle (A, B, C) to p by Voronoi regions (Ericson,
+// Real-Time Collision Detection, 2005, sec. 5.1.5), with the region tests in
+// the order and form of point_simplex_squared_distance.cpp:43-114 so that
+// points on region boundaries resolve to the same feature.  Returns |p-c|^2.
+__device__ double tri_closest(const double* p, const double* A, const double* B, const double* C, double* c) {
+  double ab[3], ac[3], ap[3], bp[3], cp[3];
+  for (int k = 0; k < 3; ++k) {
+    ab[k] = B[k] - A[k];
+    ac[k] = C[k] - A[k];
+    ap[k] = p[k] - A[k];
+  }
+  const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  bool done = false;
+  if (d1 <= 0.0 && d2 <= 0.0) {
+    for (int k = 0; k < 3; ++k) c[k] = A[k];
+    done = true;
+  }
+  double d3 = 0, d4 = 0;
+  if (!done) {
+    for (int k = 0; k < 3; ++k) bp[k] = p[k] - B[k];
+    d3 = dot3(ab, bp);
+    d4 = dot3(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+      for (int k = 0; k < 3; ++k) c[k] = B[k];
+      done = true;
+    }
+  }
+  double vc = 0;
+  if (!done) {
+    vc = d1 * d4 - d3 * d2;
+    const bool ab_distinct = A[0] != B[0] || A[1] != B[1] || A[2] != B[2];
+    if (ab_distinct && vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+      const double v = d1 / (d1 - d3);
+      for (int k = 0; k < 3; ++k) c[k] = A[k] + v * ab[k];
+      done = true;
+    }
+  }
+  double d5 = 0, d6 = 0;
+  if (!done) {
+    for (int k = 0; k < 3; ++k) cp[k] = p[k] - C[k];
+    d5 = dot3(ab, cp);
+    d6 = dot3(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
+      for (int k = 0; k < 3; ++k) c[k] = C[k];
+      done = true;
+    }
+  }
+  double vb = 0;
+  if (!done) {
+    vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+      const double w = d2 / (d2 - d6);
+      for (int k = 0; k < 3; ++k) c[k] = A[k] + w * ac[k];
+      done = true;
+    }
+  }
+  if (!done) {
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+      const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      for (int k = 0; k < 3; ++k) c[k] = B[k] + w * (C[k] - B[k]);
+    } else {
+      const double denom = 1.0 / (va + vb + vc);
+      const double v = vb * denom, w = vc * denom;
+      for (int k = 0; k < 3; ++k) c[k] = A[k] + ab[k] * v + ac[k] * w;
+    }
+  }
+  const double e0 = p[0] - c[0], e1 = p[1] - c[1], e2 = p[2] - c[2];
+  return e0 * e0 + e1 * e1 + e2 * e2;
+}
+
+// squared distance from p to a box (0 inside), AlignedBox::squaredExteriorDistance
+__device__ __forceinline__ double box_d2(const double* p, const float4& lo, const float4& hi) {
+  const double l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+  double s = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double e = p[k] < l[k] ? l[k] - p[k] : (p[k] > h[k] ? p[k] - h[k] : 0.0);
+    s += e * e;
+  }
+  return s;
+}
+
+__global__ void k_narrow3(const double* __restrict__ P, int64_t n, float* __restrict__ Pf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * n) Pf[i] = (float)P[i];
+}
+
+// Nearest triangle: depth-first branch and bound over the rope, a subtree
+// entered only when its box is closer than the best distance so far
+// (AABB.cpp:1135-1215 looks into a child when its exterior distance is below
+// the current minimum).  `best` starts at up2, as in the reference.
+__global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox, uint32_t nslots,
+                                                 const double* __restrict__ V, const int32_t* __restrict__ F,
+                                                 const double* __restrict__ P, const uint32_t* __restrict__ perm,
+                                                 int64_t np, double up2, double* __restrict__ best_out,
+                                                 int32_t* __restrict__ face_out, double* __restrict__ c_out) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int64_t pi = perm ? (int64_t)perm[t] : t;
+  const double p[3] = {P[3 * pi], P[3 * pi + 1], P[3 * pi + 2]};
+  double best = up2, bc[3] = {0.0, 0.0, 0.0};
+  int32_t bf = -1;
+  uint32_t skip = 0, i = 0;
+  while (i < nslots) {
+    const float4 lo = dbox[2 * (size_t)i], hi = dbox[2 * (size_t)i + 1];
+    const uint32_t link = __float_as_uint(lo.w);
+    const bool leaf = (link & DL_LEAF) != 0;
+    if (skip == i) {
+      skip = i + 1;
+      if (!(box_d2(p, lo, hi) < best)) {
+        if (!leaf) skip = link;
+      } else if (leaf) {
+        const uint32_t f = link & ~DL_LEAF;
+        const int32_t* tf = F + 3 * (size_t)f;
+        double A[3], B[3], C[3], c[3];
+        for (int k = 0; k < 3; ++k) {
+          A[k] = V[3 * (int64_t)tf[0] + k];
+          B[k] = V[3 * (int64_t)tf[1] + k];
+          C[k] = V[3 * (int64_t)tf[2] + k];
+        }
+        const double d2 = tri_closest(p, A, B, C, c);
+        if (d2 < best) {  // AABB.cpp set_min: strictly smaller replaces
+          best = d2;
+          bf = (int32_t)f;
+          for (int k = 0; k < 3; ++k) bc[k] = c[k];
+        }
+      }
+    }
+    const bool step = leaf || __ballot(skip == i + 1) != 0;
+    i = __builtin_amdgcn_readfirstlane(step ? i + 1 : link);
+  }
+  best_out[pi] = best;
+  face_out[pi] = bf;
+  for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = bc[k];
+}
+
+// signed_distance.cpp:96-135: out of [low2, up2) -> NaN, F.rows()+1, 0;
+// otherwise S = s * sqrt(d2) with s from the sign type.
+__global__ void k_sd_combine(int64_t np, const double* __restrict__ best, const int32_t* __restrict__ face,
+                             const double* __restrict__ cl, const double* __restrict__ W, int sign_type,
+                             double low2, double up2, int64_t nf, double* __restrict__ S, int32_t* __restrict__ I,
+                             double* __restrict__ C) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double d2 = best[p];
+  if (d2 >= up2 || d2 < low2) {
+    S[p] = __longlong_as_double(0x7FF8000000000000ll);
+    I[p] = (int32_t)(nf + 1);
+    for (int k = 0; k < 3; ++k) C[3 * p + k] = 0.0;
+    return;
+  }
+  double s = 1.0;
+  if (sign_type == GWN_SIGN_FAST_WINDING_NUMBER) {
+    const double w = (double)(float)W[p];  // the reference's FWN value is a float
+    s = 1.0 - 2.0 * fabs(w);
+  } else if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT) {
+    s = 1.0 - 2.0 * W[p];
+  }
+  S[p] = s * sqrt(d2);
+  I[p] = face[p];
+  for (int k = 0; k < 3; ++k) C[3 * p + k] = cl[3 * p + k];
+}
+
+__global__ void k_sd_empty(int64_t np, double* __restrict__ S, int32_t* __restrict__ I, double* __restrict__ C) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double nan = __longlong_as_double(0x7FF8000000000000ll);
+  S[p] = nan;
+  I[p] = -1;
+  for (int k = 0; k < 3; ++k) C[3 * p + k] = nan;
+}
+
+}  // namespace
+
+void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
+                  hipStream_t st, DevBuf<double>& tri_scratch);
+void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
+                       hipStream_t st);
+
+// signed_distance.cpp:79-84: bounds on the unsigned squared distance.
+static void sqr_bounds(double lower, double upper, double& low2, double& up2) {
+  const auto mx = [](double a, double b) { return (a < b) ? b : a; };  // std::max
+  const double max_abs = mx(std::fabs(lower), std::fabs(upper));
+  up2 = std::pow(max_abs, 2.0);
+  low2 = std::pow(mx(max_abs - (upper - lower), 0.0), 2.0);
+}
+
+void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
+                            double* S, int32_t* I, double* C, hipStream_t st) {
+  if (np <= 0) return;
+  const unsigned g = div_up(np, 256);
+  if (s->nf == 0) {  // signed_distance.cpp:397-403
+    hipLaunchKernelGGL(k_sd_empty, dim3(g), dim3(256), 0, st, np, S, I, C);
+    check_launch("k_sd_empty");
+    return;
+  }
+  soup_ensure_dist(s, st);
+  double low2, up2;
+  sqr_bounds(lower, upper, low2, up2);
+  DevBuf<float> Pf(3 * (size_t)np);
+  DevBuf<uint32_t> keys(np), perm(np);
+  DevBuf<double> best(np), cl(3 * (size_t)np), W;
+  DevBuf<int32_t> face(np);
+  hipLaunchKernelGGL(k_narrow3, dim3(div_up(3 * np, 256)), dim3(256), 0, st, P, np, Pf.p);
+  check_launch("k_narrow3");
+  morton_keys(Pf.p, np, s->bbox, keys.p, perm.p, st);
+  {
+    const size_t tb = radix_sort_temp_bytes(np, false);
+    DevBuf<uint8_t> tmp(tb);
+    radix_sort_u32(keys.p, perm.p, np, 30, tmp.p, tb, st);
+  }
+  {
+    ProfileScope prof("closest_point", st);
+    hipLaunchKernelGGL(k_closest, dim3(g), dim3(256), 0, st, s->dbox.p, (uint32_t)s->nslots, s->V64.p, s->F.p, P,
+                       perm.p, np, up2, best.p, face.p, cl.p);
+    check_launch("k_closest");
+  }
+  if (sign_type == GWN_SIGN_FAST_WINDING_NUMBER) {
+    // signed_distance.cpp:125-130: fast_winding_number(fwn_bvh, 2, q.cast<float>())
+    W.alloc(np);
+    soup_query_device(s, Pf.p, np, 2.0f, GWN_PREC_FP32, W.p, st);
+  } else if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT) {
+    // signed_distance.cpp:121-124: the exact winding number
+    W.alloc(np);
+    DevBuf<double> scratch;
+    exact_device(s->V64.p, s->nv, s->F.p, s->nf, P, np, W.p, st, scratch);
+    GWN_HIP(hipStreamSynchronize(st));
+  }
+  hipLaunchKernelGGL(k_sd_combine, dim3(g), dim3(256), 0, st, np, best.p, face.p, cl.p, W.p, sign_type, low2, up2,
+                     s->nf, S, I, C);
+  check_launch("k_sd_combine");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace gwn

@@ -148,6 +148,10 @@ int64_t lbvh_build(const float* tri_boxes_dev, int64_t nf, const float* bbox_hos
 // soup queries (soup.hip)
 size_t soup_query_work_bytes(int64_t nq);
 void soup_ensure_fp64(gwn_soup* s, hipStream_t st);
+void soup_ensure_dist(gwn_soup* s, hipStream_t st);
+void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
+void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
+                            double* S, int32_t* I, double* C, hipStream_t st);
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
 
@@ -167,6 +171,8 @@ struct gwn_soup {
   gwn::DevBuf<float4> slots;    // nslots x 6 (rope layout, FP32 moments)
   int64_t nslots = 0;
   gwn::DevBuf<double> slots64;  // nslots x 24 (FP64 moments), built on first FP64 use
+  gwn::DevBuf<double> V64;      // nv x 3, the vertices as given (closest-point queries)
+  gwn::DevBuf<float4> dbox;     // nslots x 2 slot boxes for closest-point queries, built on first use
   gwn_mixed_params mixed{1e-3f, 0.05f};
   // per-handle scratch for queries (grow-only)
   gwn::DevBuf<uint8_t> work;

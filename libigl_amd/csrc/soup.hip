@@ -147,9 +147,18 @@ __global__ void k_positions(const int32_t* __restrict__ level_nodes, int64_t cou
 }
 
 // Slot record (6 x float4):
-//   0: P.x P.y P.z maxP2          1: N.x N.y N.z link
-//   2: Nd0 Nd1 Nd2 (Nxy+Nyx)      3: (Nyz+Nzy) (Nzx+Nxz) Nkd0 Nkd1
-//   4: Nkd2 SP C2xxy C2xxz        5: C2yyz C2yyx C2zzx C2zzy
+//   0: P.x P.y P.z maxP2          1: -N.x -N.y -N.z link
+//   2: A B C Dxy                  3: Dyz Dzx g300 g030
+//   4: g003 g210 g201 g120        5: g021 g102 g012 g111
+// The far-field terms of the reference (7199-7245), with x = q/|q|:
+//   -(x.N)/|q|^2 + (tr Nd - 3 x'Nd x)/|q|^3 + (1.5 (x.s) - 7.5 K(x))/|q|^4,
+// s and the cubic K as in the reference (temp0/temp1), are rewritten with
+// |x| = 1 as m^2 [ x.(-N) + m ( Qf(x) + m G(x) ) ], m = 1/|q|, where
+// Qf(x) = A x^2 + B y^2 + C z^2 + Dxy xy + Dyz yz + Dzx zx with
+// A = tr - 3 Ndx ... and Dxy = -3 (Nxy+Nyx) ..., and the cubic
+// G(x) = 1.5 (s.x) |x|^2 - 7.5 K(x) with coefficients g_ijk (of x^i y^j z^k).
+// The coefficients are formed in FP64 from the reference's FP32 BoxData and
+// rounded once.
 // link = skip index for an internal child, LINK_LEAF|triangle for an item.
 __global__ void k_write_slots(int64_t nn, const uint32_t* __restrict__ nodes, const float* __restrict__ boxdata,
                               const uint32_t* __restrict__ nslots_sub, const uint32_t* __restrict__ first_slot,
@@ -170,12 +179,22 @@ __global__ void k_write_slots(int64_t nn, const uint32_t* __restrict__ nodes, co
       link = LINK_LEAF | c;
     }
     float4* r = slots + 6 * (int64_t)pos;
+    const double nd0 = bd[BD_ND + s], nd1 = bd[BD_ND + 4 + s], nd2 = bd[BD_ND + 8 + s];
+    const double tr = nd0 + nd1 + nd2;
+    const double k0 = bd[BD_NKD + s], k1 = bd[BD_NKD + 4 + s], k2 = bd[BD_NKD + 8 + s];
+    const double cxxy = bd[BD_CXXY + s], cxxz = bd[BD_CXXZ + s], cyyz = bd[BD_CYYZ + s];
+    const double cyyx = bd[BD_CYYX + s], czzx = bd[BD_CZZX + s], czzy = bd[BD_CZZY + s];
+    const double s0 = 3.0 * k0 + (cyyx + czzx), s1 = 3.0 * k1 + (czzy + cxxy), s2 = 3.0 * k2 + (cxxz + cyyz);
     r[0] = make_float4(bd[BD_P + s], bd[BD_P + 4 + s], bd[BD_P + 8 + s], bd[BD_MAXP2 + s]);
-    r[1] = make_float4(bd[BD_N + s], bd[BD_N + 4 + s], bd[BD_N + 8 + s], __uint_as_float(link));
-    r[2] = make_float4(bd[BD_ND + s], bd[BD_ND + 4 + s], bd[BD_ND + 8 + s], bd[BD_NXY + s]);
-    r[3] = make_float4(bd[BD_NYZ + s], bd[BD_NZX + s], bd[BD_NKD + s], bd[BD_NKD + 4 + s]);
-    r[4] = make_float4(bd[BD_NKD + 8 + s], bd[BD_SP + s], bd[BD_CXXY + s], bd[BD_CXXZ + s]);
-    r[5] = make_float4(bd[BD_CYYZ + s], bd[BD_CYYX + s], bd[BD_CZZX + s], bd[BD_CZZY + s]);
+    r[1] = make_float4(-bd[BD_N + s], -bd[BD_N + 4 + s], -bd[BD_N + 8 + s], __uint_as_float(link));
+    r[2] = make_float4((float)(tr - 3.0 * nd0), (float)(tr - 3.0 * nd1), (float)(tr - 3.0 * nd2),
+                       (float)(-3.0 * (double)bd[BD_NXY + s]));
+    r[3] = make_float4((float)(-3.0 * (double)bd[BD_NYZ + s]), (float)(-3.0 * (double)bd[BD_NZX + s]),
+                       (float)(1.5 * s0 - 7.5 * k0), (float)(1.5 * s1 - 7.5 * k1));
+    r[4] = make_float4((float)(1.5 * s2 - 7.5 * k2), (float)(1.5 * s1 - 7.5 * cxxy), (float)(1.5 * s2 - 7.5 * cxxz),
+                       (float)(1.5 * s0 - 7.5 * cyyx));
+    r[5] = make_float4((float)(1.5 * s2 - 7.5 * cyyz), (float)(1.5 * s0 - 7.5 * czzx), (float)(1.5 * s1 - 7.5 * czzy),
+                       (float)(-7.5 * (double)bd[BD_SP + s]));
     pos = next;
   }
 }
@@ -217,6 +236,111 @@ __global__ void k_write_slots64(int64_t nn, const uint32_t* __restrict__ nodes, 
     r[22] = bd[BD_CZZX + s];
     r[23] = bd[BD_CZZY + s];
     pos += (c & NODE_INTERNAL) ? 1 + nslots_sub[c & ~NODE_INTERNAL] : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// closest-point support: a bounding box per slot record, from the FP64
+// vertices and rounded outward to float, so a box distance computed from it
+// never exceeds the distance to anything inside.
+__device__ __forceinline__ float round_down_f(double x) {
+  float f = (float)x;
+  return ((double)f > x) ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float round_up_f(double x) {
+  float f = (float)x;
+  return ((double)f < x) ? nextafterf(f, INFINITY) : f;
+}
+__device__ __forceinline__ void tri_box64(const double* __restrict__ V, const int32_t* __restrict__ F, uint32_t t,
+                                          float lo[3], float hi[3]) {
+  for (int a = 0; a < 3; ++a) {
+    double l = V[3 * (int64_t)F[3 * (int64_t)t] + a], h = l;
+    for (int k = 1; k < 3; ++k) {
+      const double v = V[3 * (int64_t)F[3 * (int64_t)t + k] + a];
+      l = v < l ? v : l;
+      h = v > h ? v : h;
+    }
+    lo[a] = round_down_f(l);
+    hi[a] = round_up_f(h);
+  }
+}
+
+__global__ void k_node_boxes(const int32_t* __restrict__ level_nodes, int64_t count, const uint32_t* __restrict__ nodes,
+                             const double* __restrict__ V, const int32_t* __restrict__ F, float* __restrict__ nbox) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t node = level_nodes[t];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * (int64_t)node + s];
+    if (c == NODE_EMPTY) break;
+    float cl[3], chh[3];
+    if (c & NODE_INTERNAL) {
+      const float* b = nbox + 6 * (int64_t)(c & ~NODE_INTERNAL);
+      for (int a = 0; a < 3; ++a) {
+        cl[a] = b[a];
+        chh[a] = b[3 + a];
+      }
+    } else {
+      tri_box64(V, F, c, cl, chh);
+    }
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], cl[a]);
+      hi[a] = fmaxf(hi[a], chh[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    nbox[6 * (int64_t)node + a] = lo[a];
+    nbox[6 * (int64_t)node + 3 + a] = hi[a];
+  }
+}
+
+// first slot of each child node from the parent's first slot and the links
+__global__ void k_positions_rope(const int32_t* __restrict__ level_nodes, int64_t count,
+                                 const uint32_t* __restrict__ nodes, const float4* __restrict__ slots,
+                                 uint32_t* __restrict__ first_slot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t node = level_nodes[t];
+  uint32_t pos = first_slot[node];
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * (int64_t)node + s];
+    if (c == NODE_EMPTY) break;
+    if (c & NODE_INTERNAL) {
+      first_slot[c & ~NODE_INTERNAL] = pos + 1;
+      pos = __float_as_uint(slots[6 * (int64_t)pos + 1].w);
+    } else {
+      pos += 1;
+    }
+  }
+}
+
+// slot box record (2 x float4): [lo.xyz, link] [hi.xyz, 0], in rope order
+__global__ void k_write_dbox(int64_t nn, const uint32_t* __restrict__ nodes, const float* __restrict__ nbox,
+                             const double* __restrict__ V, const int32_t* __restrict__ F,
+                             const uint32_t* __restrict__ first_slot, const float4* __restrict__ slots,
+                             float4* __restrict__ dbox) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nn) return;
+  uint32_t pos = first_slot[node];
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = nodes[4 * node + s];
+    if (c == NODE_EMPTY) break;
+    float lo[3], hi[3];
+    if (c & NODE_INTERNAL) {
+      const float* b = nbox + 6 * (int64_t)(c & ~NODE_INTERNAL);
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = b[a];
+        hi[a] = b[3 + a];
+      }
+    } else {
+      tri_box64(V, F, c, lo, hi);
+    }
+    const float link = slots[6 * (int64_t)pos + 1].w;
+    dbox[2 * (int64_t)pos] = make_float4(lo[0], lo[1], lo[2], link);
+    dbox[2 * (int64_t)pos + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    const uint32_t l = __float_as_uint(link);
+    pos = (l & LINK_LEAF) ? pos + 1 : l;
   }
 }
 
@@ -336,12 +460,56 @@ __device__ __forceinline__ T far_field(T x, T y, T z, T l2, const V4& b, const V
   return o;
 }
 
+// FP32 far field from the rewritten slot record (see k_write_slots), for
+// q = query - P and l2 = |q|^2.  With x = q/|q| (kept normalised so that the
+// cubic cannot overflow for distant queries) and m = 1/|q| the value is
+//   m^2 [ x.(-N) + m ( Qf(x) + m G(x) ) ].
+template <int ORDER>
+__device__ __forceinline__ float far_field_poly(float x, float y, float z, float l2, const float4& b,
+                                                const float4& c, const float4& d, const float4& e,
+                                                const float4& f) {
+#pragma clang fp contract(fast)
+  const float m = __builtin_amdgcn_rsqf(l2);
+  x *= m;
+  y *= m;
+  z *= m;
+  float t = x * b.x + y * b.y + z * b.z;  // x.(-N)
+  if (ORDER >= 1) {
+    const float z2 = z * z;
+    float qf = x * (c.x * x + c.w * y + d.y * z) + y * (c.y * y + d.x * z) + c.z * z2;
+    if (ORDER >= 2) {
+      const float gx = x * (d.z * x + e.y * y + e.z * z) + y * (e.w * y + f.w * z) + f.y * z2;
+      const float gy = y * (d.w * y + f.x * z) + f.z * z2;
+      const float g = x * gx + y * gy + e.x * (z2 * z);
+      qf += m * g;
+    }
+    t += m * qf;
+  }
+  return (m * m) * t;
+}
+
 struct D4 {
   double x, y, z, w;
 };
 
 // ---------------------------------------------------------------------------
 // traversal: FP32 (reference arithmetic) and mixed (FP64 near triangles + FP64 sum)
+//
+// Wave-coherent rope walk.  The slot records are in depth-first ("rope")
+// order: a far child adds its Taylor value and skips to `link` (the record
+// after its subtree), a near internal child is entered (next record), an item
+// is evaluated exactly when near.  Each lane keeps `skip`, the next record it
+// needs; the wavefront walks one shared index i and a lane takes part in a
+// record exactly when i == skip.  The wave steps to i+1 when some lane needs
+// it (a near child, or an item) and otherwise to link(i): every lane that is
+// not taking part needs a record at or beyond link(i), because the subtrees
+// of the rope are nested intervals.  So each lane visits exactly the records
+// of its own walk, in the same order, and sums the same terms in the same
+// order as an independent per-lane walk; but every record (and triangle) a
+// wave reads is the same for all its lanes, so it is fetched once per wave
+// through the scalar cache instead of 64 times through the vector memory path.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <int ORDER, bool MIXED>
 __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slots, uint32_t nslots,
                                                   const float* __restrict__ tri, const float* __restrict__ Q,
@@ -354,50 +522,62 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   float sum32 = 0.0f;
   double sum64 = 0.0;
+  uint32_t skip = 0;
   uint32_t i = 0;
   while (i < nslots) {
-    const float4 a = slots[6 * (size_t)i];
-    const float4 b = slots[6 * (size_t)i + 1];
+    const float4* r = slots + 6 * (size_t)i;
+    const float4 a = r[0], b = r[1];
+    float4 c, d, e, f;
+    if (ORDER >= 1) {
+      c = r[2];
+      d = r[3];
+    }
+    if (ORDER >= 2) {
+      e = r[4];
+      f = r[5];
+    }
+    // the whole record is requested together (one scalar-cache round trip)
+    // rather than where each part is first used
+    if (ORDER >= 2) asm volatile("" ::"s"(a.x), "s"(b.w), "s"(c.x), "s"(e.x), "s"(f.w));
+    else if (ORDER >= 1) asm volatile("" ::"s"(a.x), "s"(b.w), "s"(c.x), "s"(d.w));
+    else asm volatile("" ::"s"(a.x), "s"(b.w));
     const uint32_t link = __float_as_uint(b.w);
-    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
-    // |q-P|^2 rounded as the reference rounds it (no fused multiply-add), so
-    // every near/far decision is the reference's
-    const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
     const bool leaf = (link & LINK_LEAF) != 0;
-    if (!(l2 <= __fmul_rn(a.w, acc2))) {
-      float4 c, d, e, f;
-      if (ORDER >= 1) {
-        c = slots[6 * (size_t)i + 2];
-        d = slots[6 * (size_t)i + 3];
-      }
-      if (ORDER >= 2) {
-        e = slots[6 * (size_t)i + 4];
-        f = slots[6 * (size_t)i + 5];
-      }
-      const float o = far_field<ORDER, float, float4, true>(x, y, z, l2, b, c, d, e, f);
-      if (isfinite(o)) {
-        if (MIXED)
-          sum64 += (double)o;
-        else
-          sum32 += o;
-        i = leaf ? i + 1 : link;
-        continue;
-      }
-    }
-    if (leaf) {
-      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-      if (MIXED) {
-        const float r = sqrtf(a.w) + near_d;
-        if (l2 < r * r) {
-          sum64 += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
-        } else {
-          sum64 += (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+    if (skip == i) {
+      const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+      // |q-P|^2 rounded as the reference rounds it (no fused multiply-add),
+      // so every near/far decision is the reference's
+      const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+      bool exact = leaf;
+      skip = i + 1;
+      if (!(l2 <= __fmul_rn(a.w, acc2))) {
+        const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
+        if (isfinite(o)) {  // otherwise the reference descends / evaluates the item
+          if (MIXED)
+            sum64 += (double)o;
+          else
+            sum32 += o;
+          exact = false;
+          if (!leaf) skip = link;
         }
-      } else {
-        sum32 += tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+      }
+      if (exact) {
+        const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+        if (MIXED) {
+          const float rr = sqrtf(a.w) + near_d;
+          if (l2 < rr * rr) {
+            sum64 += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+          } else {
+            sum64 += (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
+                                                  qz);
+          }
+        } else {
+          sum32 += tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+        }
       }
     }
-    ++i;
+    const bool step = leaf || __ballot(skip == i + 1) != 0;
+    i = wave_uniform(step ? i + 1 : link);
   }
   W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
 }
@@ -408,33 +588,40 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
                                                     uint32_t nslots, const float* __restrict__ tri,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ perm,
                                                     int64_t nq, double acc2, double* __restrict__ W) {
+  // the same wave-coherent rope walk as k_traverse, over the FP64 records
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   double sum = 0.0;
+  uint32_t skip = 0;
   uint32_t i = 0;
   while (i < nslots) {
     const double* r = s64 + 24 * (size_t)i;
     const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
-    const double x = qx - r[0], y = qy - r[1], z = qz - r[2];
-    const double l2 = x * x + y * y + z * z;
     const bool leaf = (link & LINK_LEAF) != 0;
-    if (!(l2 <= r[3] * acc2)) {
-      const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
-          e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
-      const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
-      if (isfinite(o)) {
-        sum += o;
-        i = leaf ? i + 1 : link;
-        continue;
+    if (skip == i) {
+      const double x = qx - r[0], y = qy - r[1], z = qz - r[2];
+      const double l2 = x * x + y * y + z * z;
+      bool exact = leaf;
+      skip = i + 1;
+      if (!(l2 <= r[3] * acc2)) {
+        const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
+            e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
+        const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
+        if (isfinite(o)) {
+          sum += o;
+          exact = false;
+          if (!leaf) skip = link;
+        }
+      }
+      if (exact) {
+        const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+        sum += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
       }
     }
-    if (leaf) {
-      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-      sum += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
-    }
-    ++i;
+    const bool step = leaf || __ballot(skip == i + 1) != 0;
+    i = wave_uniform(step ? i + 1 : link);
   }
   W[qi] = sum / (4.0 * PI_D);
 }
@@ -575,6 +762,37 @@ void soup_ensure_fp64(gwn_soup* s, hipStream_t st) {
   s->slots64 = std::move(s64);
 }
 
+void soup_ensure_dist(gwn_soup* s, hipStream_t st) {
+  if (s->dbox.p || s->nnodes == 0) return;
+  const int64_t nn = s->nnodes;
+  Levels L;
+  compute_levels(s->nodes.p, nn, L, st);
+  DevBuf<float> nbox(6 * (size_t)nn);
+  for (int d = L.maxdepth; d >= 0; --d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_node_boxes, dim3(div_up(cnt, 128)), dim3(128), 0, st, L.order.p + L.offset[d], cnt,
+                       s->nodes.p, s->V64.p, s->F.p, nbox.p);
+    check_launch("k_node_boxes");
+  }
+  // first slot of every node, top-down, following the links of the rope
+  DevBuf<uint32_t> first(nn);
+  GWN_HIP(hipMemsetAsync(first.p, 0, sizeof(uint32_t), st));
+  for (int d = 0; d <= L.maxdepth; ++d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_positions_rope, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt,
+                       s->nodes.p, s->slots.p, first.p);
+    check_launch("k_positions_rope");
+  }
+  DevBuf<float4> db(2 * (size_t)s->nslots);
+  hipLaunchKernelGGL(k_write_dbox, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, nbox.p, s->V64.p,
+                     s->F.p, first.p, s->slots.p, db.p);
+  check_launch("k_write_dbox");
+  GWN_HIP(hipStreamSynchronize(st));
+  s->dbox = std::move(db);
+}
+
 void soup_setup(gwn_soup* s, hipStream_t st) {
   const int64_t nf = s->nf;
   s->boxes.alloc(6 * (size_t)nf);
@@ -595,6 +813,16 @@ size_t soup_query_work_bytes(int64_t nq) {
 
 void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
+
+// 30-bit Morton keys of the queries over a box (queries outside are clamped
+// to it), and the identity permutation to be sorted along with them
+void morton_keys(const float* Q, int64_t nq, const float* bb, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
+              ez = std::max(bb[5] - bb[2], 1e-30f);
+  hipLaunchKernelGGL(k_morton_q, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
+                     1024.0f / ey, 1024.0f / ez, keys, vals);
+  check_launch("k_morton_q");
+}
 
 void soup_query_device(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
                        double* W, hipStream_t st) {
@@ -630,12 +858,7 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
   uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
   uint32_t* ambn = (uint32_t*)take(4);
 
-  const float* bb = s->bbox;
-  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
-              ez = std::max(bb[5] - bb[2], 1e-30f);
-  hipLaunchKernelGGL(k_morton_q, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
-                     1024.0f / ey, 1024.0f / ez, keys, perm);
-  check_launch("k_morton_q");
+  morton_keys(Q, nq, s->bbox, keys, perm, st);
   {
     ProfileScope prof("radix_sort", st);
     radix_sort_u32(keys, perm, nq, 30, tmp, tmpb, st);

@@ -21,7 +21,10 @@ from __future__ import annotations
 import numpy as np
 
 from . import _native as nat
-from ._native import PREC_FP32, PREC_FP64, PREC_MIXED, TREE_LBVH, TREE_SAH4, GWNError, check, lib
+from ._native import (PREC_FP32, PREC_FP64, PREC_MIXED, SIGN_DEFAULT, SIGN_FAST_WINDING_NUMBER, SIGN_PSEUDONORMAL,
+                      SIGN_UNSIGNED, SIGN_WINDING_NUMBER, TREE_LBVH, TREE_SAH4, GWNError, check, lib)
+
+DBL_MIN, DBL_MAX = 2.2250738585072014e-308, 1.7976931348623157e308
 
 __all__ = ["FastWindingNumberBVH", "PointCloudFWN", "fast_winding_number", "winding_number", "octree",
            "PREC_FP32", "PREC_MIXED", "PREC_FP64", "TREE_SAH4", "TREE_LBVH", "GWNError"]
@@ -121,6 +124,16 @@ class FastWindingNumberBVH:
     def query_device(self, Q_ptr, nq, W_ptr, accuracy_scale=2.0, precision=PREC_FP32, stream=None):
         """Device pointers (e.g. torch tensor data_ptr()): Q float nq x 3, W double nq."""
         check(lib().gwn_soup_query_device(self.handle, Q_ptr, nq, float(accuracy_scale), precision, W_ptr, stream))
+
+    def signed_distance(self, P, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX):
+        """igl::signed_distance over this mesh: (S, I, C)."""
+        P = _rows3(P, "P", np.float64)
+        S = np.empty(P.shape[0])
+        I = np.empty(P.shape[0], np.int32)
+        C = np.empty((P.shape[0], 3))
+        check(lib().gwn_signed_distance(self.handle, P, P.shape[0], int(sign_type), float(lower_bound),
+                                        float(upper_bound), S, I, C))
+        return S, I, C
 
 
 def C_byref(x):
@@ -279,6 +292,14 @@ def fast_winding_number(*args, **kw):
         order = {3: 0, 12: 1, 39: 2}[np.asarray(EC).shape[1]]
         return PointCloudFWN.from_octree(P, N, A, point_indices, CH, order, CM, R, EC).query(Q, beta)
     raise TypeError("no fast_winding_number overload takes these arguments")
+
+
+def signed_distance(P, V, F, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX):
+    """igl::signed_distance(P,V,F,sign_type[,lower,upper],S,I,C,N) for the
+    unsigned, winding-number and fast-winding-number sign types (the FWN sign
+    uses an order-2 tree, as signed_distance.cpp:65-67 builds it).  Returns
+    (S, I, C)."""
+    return FastWindingNumberBVH().build(V, F, 2).signed_distance(P, sign_type, lower_bound, upper_bound)
 
 
 def sah4_tree(V, F):

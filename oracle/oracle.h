@@ -18,6 +18,11 @@
  *   oracle_soup_*
  *       bit-exact against the compiled reference UT_SolidAngle (oracle/_ref):
  *       node arrays, BoxData and per-query Omega; plus closed-mesh invariants.
+ *   oracle_point_tri_sqrdist / oracle_signed_distance
+ *       closed-form closest points in every Voronoi region of a right
+ *       triangle, a dense barycentric sampling bound, the single-tet golden
+ *       value of tests/include/igl/signed_distance.cpp:4-44 (sqrt(1/12)) and
+ *       sign checks on closed meshes.
  *   oracle_octree / oracle_cloud_*
  *       the reference golden values of tests/include/igl/fast_winding_number.cpp:46-52
  *       (one-point cloud, 1e-15), the beta<=0 brute-force branch
@@ -68,6 +73,17 @@ void oracle_soup_query(const oracle_soup* s, const double* Q, int64_t nq, float 
  * calls), accepted far-field child slots and exact triangles. */
 void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale,
                        int64_t* visits, int64_t* far_accepted, int64_t* exact_tris);
+
+/* ---- signed distance (signed_distance.cpp:28-140) ---- */
+/* closest point of triangle (a,b,c) to p, point_simplex_squared_distance.cpp:43-114;
+ * writes it to out (may be NULL) and returns the squared distance */
+double oracle_point_tri_sqrdist(const double* p, const double* a, const double* b, const double* c, double* out);
+/* igl::signed_distance(P,V,F,sign_type,lower,upper,S,I,C,N) for sign types
+ * 1/2 (winding number), 3 (unsigned), 4 (fast winding number), by brute force
+ * over all faces; returns 0, or -1 for an unsupported sign type */
+int oracle_signed_distance(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* P, int64_t np,
+                           int sign_type, double lower, double upper, double* S, int32_t* I, double* C,
+                           int nthreads);
 
 /* ---- point cloud: igl::octree + fast_winding_number(P,N,A,...) ---- */
 /* octree (octree.cpp:7-169).  Cells are numbered exactly as the reference

@@ -66,6 +66,11 @@ def lib():
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        L.oracle_point_tri_sqrdist.restype = C.c_double
+        L.oracle_point_tri_sqrdist.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_signed_distance.restype = C.c_int
+        L.oracle_signed_distance.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, C.c_int,
+                                             C.c_double, C.c_double, _f64p, _i32p, _f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -98,6 +103,30 @@ def ref():
 
 def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------- signed distance ----------------
+def point_tri_sqrdist(p, a, b, c):
+    """(squared distance, closest point) of triangle (a, b, c) to p."""
+    out = np.zeros(3)
+    d2 = lib().oracle_point_tri_sqrdist(_c(p, np.float64), _c(a, np.float64), _c(b, np.float64), _c(c, np.float64),
+                                        out)
+    return d2, out
+
+
+def signed_distance(V, F, P, sign_type, lower=2.2250738585072014e-308, upper=1.7976931348623157e308, nthreads=0):
+    """igl::signed_distance restated (brute force): returns S, I, C."""
+    V = _c(V, np.float64).reshape(-1, 3)
+    F = _c(F, np.int32).reshape(-1, 3)
+    P = _c(P, np.float64).reshape(-1, 3)
+    S = np.empty(P.shape[0])
+    I = np.empty(P.shape[0], np.int32)
+    C_ = np.empty((P.shape[0], 3))
+    rc = lib().oracle_signed_distance(V, V.shape[0], F, F.shape[0], P, P.shape[0], int(sign_type), float(lower),
+                                      float(upper), S, I, C_, nthreads)
+    if rc != 0:
+        raise ValueError("oracle_signed_distance failed")
+    return S, I, C_
 
 
 # ---------------- exact ----------------

@@ -1,0 +1,97 @@
+// Calls every Eigen-signature overload of include/gwn_igl.h (compiled against
+// the mock Eigen/Core in tests/cpp/mock_eigen) with the reference's own test
+// inputs: tests/include/igl/signed_distance.cpp:4-44 (single tet) and
+// tests/include/igl/fast_winding_number.cpp:17-53 (one-point cloud).
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "gwn_igl.h"
+
+int main() {
+  int bad = 0;
+  Eigen::MatrixXd V(4, 3);
+  const double v[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) V(i, j) = v[i][j];
+  Eigen::MatrixXi F(4, 3);
+  const int f[4][3] = {{0, 1, 3}, {0, 2, 1}, {0, 3, 2}, {1, 2, 3}};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) F(i, j) = f[i][j];
+  Eigen::MatrixXd P(1, 3);
+  P(0, 0) = 0.5, P(0, 1) = 0.5, P(0, 2) = 0.5;
+  const double Sexact = std::sqrt(1.0 / 12.0);
+  // igl::signed_distance, both overloads, every provided sign type
+  for (igl::SignedDistanceType t : {igl::SIGNED_DISTANCE_TYPE_WINDING_NUMBER, igl::SIGNED_DISTANCE_TYPE_DEFAULT,
+                                    igl::SIGNED_DISTANCE_TYPE_UNSIGNED,
+                                    igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER}) {
+    Eigen::VectorXd S;
+    Eigen::VectorXi I;
+    Eigen::MatrixXd C, N;
+    igl::signed_distance(P, V, F, t, S, I, C, N);
+    const double tol = t == igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER ? 1e-7 : 1e-15;
+    if (S.rows() != 1 || std::fabs(S(0, 0) - Sexact) > tol || I(0, 0) != 3 || C.cols() != 3) bad |= 1;
+    // bounds (0.05, 0.1): squared distances kept in [0.05^2, 0.1^2) (signed_distance.cpp:79-84);
+    // d^2 = 1/12 is outside: NaN, I = F.rows() + 1
+    igl::signed_distance(P, V, F, t, 0.05, 0.1, S, I, C, N);
+    if (!std::isnan(S(0, 0)) || I(0, 0) != 5) bad |= 2;
+  }
+  // signed_distance_fast_winding_number with a prebuilt bvh
+  {
+    igl::FastWindingNumberBVH bvh;
+    igl::fast_winding_number(V, F, 2, bvh);
+    Eigen::VectorXd S;
+    const int tree = 0;  // the caller's AABB is not used
+    igl::signed_distance_fast_winding_number(P, V, F, tree, bvh, S);
+    if (S.rows() != 1 || std::fabs(S(0, 0) - Sexact) > 1e-7) bad |= 4;
+    // the batch and single-point FWN overloads
+    Eigen::MatrixXd Q(2, 3);
+    Q(0, 0) = 0.1, Q(0, 1) = 0.1, Q(0, 2) = 0.1;
+    Q(1, 0) = 2.0, Q(1, 1) = 0.0, Q(1, 2) = 0.0;
+    Eigen::VectorXf W;
+    igl::fast_winding_number(bvh, 2.0f, Q, W);
+    if (W.rows() != 2 || std::fabs(W(0, 0) - 1.0f) > 1e-6f || std::fabs(W(1, 0)) > 1e-3f) bad |= 8;
+    Eigen::Matrix<double, 1, 3> p;
+    p(0, 0) = 0.1, p(0, 1) = 0.1, p(0, 2) = 0.1;
+    if (std::fabs(igl::fast_winding_number(bvh, 2.0f, p) - 1.0) > 1e-6) bad |= 16;
+  }
+  // exact winding number, batch and single point
+  {
+    Eigen::VectorXd W;
+    igl::winding_number(V, F, P, W);
+    Eigen::Matrix<double, 1, 3> p;
+    p(0, 0) = 0.1, p(0, 1) = 0.1, p(0, 2) = 0.1;
+    if (std::fabs(W(0, 0)) > 1e-15 || std::fabs(igl::winding_number(V, F, p) - 1.0) > 1e-15) bad |= 32;
+    Eigen::VectorXd W2;
+    igl::fast_winding_number(V, F, P, W2);
+    if (std::fabs(W2(0, 0)) > 1e-6) bad |= 64;
+  }
+  // point cloud: one-shot, default, octree + precompute + eval
+  {
+    Eigen::MatrixXd Pc(1, 3), Nc(1, 3), Ac(1, 1), Qc(4, 3);
+    Pc(0, 0) = 0.1, Pc(0, 1) = 0.2, Pc(0, 2) = 0.3;
+    const double nn = std::sqrt(16.0 + 25.0 + 36.0);
+    Nc(0, 0) = 4 / nn, Nc(0, 1) = 5 / nn, Nc(0, 2) = 6 / nn;
+    Ac(0, 0) = 0.7;
+    const double qc[4][3] = {{0, 0, 0}, {4, -3, 2}, {-1, 1, 3}, {0, 5, -2}};
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 3; ++j) Qc(i, j) = qc[i][j];
+    const double gold[4] = {0.38779369004261133, -0.00041235296362485, -0.00362978253577090, -0.00041235296362485};
+    Eigen::VectorXd W1, W2, W3;
+    igl::fast_winding_number(Pc, Nc, Ac, Qc, 2, 2.0, W1);
+    igl::fast_winding_number(Pc, Nc, Ac, Qc, W2);
+    std::vector<std::vector<int>> PI;
+    Eigen::MatrixXi CH;
+    Eigen::MatrixXd CN, Wc, CM, R, EC;
+    igl::octree(Pc, PI, CH, CN, Wc);
+    igl::fast_winding_number(Pc, Nc, Ac, PI, CH, 2, CM, R, EC);
+    igl::fast_winding_number(Pc, Nc, Ac, PI, CH, CM, R, EC, Qc, 2.0, W3);
+    for (int i = 0; i < 4; ++i)
+      if (std::fabs(W1(i, 0) - gold[i]) > 4e-7 || std::fabs(W2(i, 0) - gold[i]) > 4e-7 ||
+          std::fabs(W3(i, 0) - gold[i]) > 4e-7)
+        bad |= 128;
+    if (CH.rows() != 1 || PI.size() != 1 || EC.cols() != 39) bad |= 256;
+  }
+  std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
+  return bad;
+}

@@ -3,6 +3,7 @@
 // templates).  Built by tests/test_facade.py; run on the GPU.
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <vector>
 
 #include "gwn_igl.h"
@@ -102,6 +103,15 @@ int main() {
     igl::gwn_detail::cloud(Pc, Nc, Ac, Qq, 2, 2.0, W2);
     for (int i = 0; i < 3; ++i)
       if (std::fabs(W1(i, 0) - W2(i, 0)) > 1e-6) bad |= 512;
+  }
+  // signed_distance (signed_distance.cpp:4-44 single tet): S = sqrt(1/12)
+  {
+    Mat<double> Pt(1, 3), S, C;
+    Mat<int> I;
+    Pt(0, 0) = 0.5, Pt(0, 1) = 0.5, Pt(0, 2) = 0.5;
+    igl::gwn_detail::signed_distance(Pt, V, F, igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER,
+                                     std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), S, I, C);
+    if (S.rows() != 1 || std::fabs(S(0, 0) - std::sqrt(1.0 / 12.0)) > 1e-7 || I(0, 0) != 3) bad |= 1024;
   }
   // bad input is reported, not asserted
   Mat<int> Fbad(1, 3);

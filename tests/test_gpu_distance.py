@@ -1,0 +1,107 @@
+"""GPU igl::signed_distance (unsigned, winding-number and fast-winding-number
+sign types) through the C ABI, against the oracle's brute-force restatement
+(itself pinned by closed forms and the single-tet golden value).
+
+Tolerances: squared distances and closest points are FP64 on both sides and
+agree to 1e-12 (relative); the closest face agrees except where several faces
+are equally close (then the GPU's face must be at the same distance); the
+fast-winding-number sign differs only through the FWN value (FP32, order 2,
+accuracy_scale 2), within 1e-4 in w, so |dS| <= 2e-4 |S|."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from libigl_amd import datagen as D
+from libigl_amd import gwn
+
+pytestmark = pytest.mark.gpu
+
+SIGNS = (gwn.SIGN_UNSIGNED, gwn.SIGN_WINDING_NUMBER, gwn.SIGN_DEFAULT, gwn.SIGN_FAST_WINDING_NUMBER)
+
+
+def _check(V, F, P, S, I, C, So, Io, Co, sign):
+    nan = np.isnan(So)
+    assert np.array_equal(np.isnan(S), nan)
+    assert np.array_equal(I[nan], Io[nan]) and np.all(C[nan] == Co[nan])
+    m = ~nan
+    if sign == gwn.SIGN_FAST_WINDING_NUMBER:
+        assert np.all(np.abs(S[m] - So[m]) <= 2e-4 * np.abs(So[m]) + 1e-12)
+    else:
+        assert np.all(np.abs(S[m] - So[m]) <= 1e-12 * np.maximum(1.0, np.abs(So[m])))
+    same = I[m] == Io[m]
+    assert np.allclose(C[m][same], Co[m][same], rtol=0, atol=1e-12)
+    # ties: the GPU's face is as close as the oracle's
+    for p in np.nonzero(m)[0][~same]:
+        t = F[I[p]]
+        d2, _ = O.point_tri_sqrdist(P[p], V[t[0]], V[t[1]], V[t[2]])
+        to = F[Io[p]]
+        do, _ = O.point_tri_sqrdist(P[p], V[to[0]], V[to[1]], V[to[2]])
+        assert abs(d2 - do) <= 1e-12 * max(1.0, do), (p, d2, do)
+
+
+def test_single_tet_golden():
+    # tests/include/igl/signed_distance.cpp:4-44
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    F = np.array([[0, 1, 3], [0, 2, 1], [0, 3, 2], [1, 2, 3]], np.int32)
+    for sign, tol in ((gwn.SIGN_WINDING_NUMBER, 1e-15), (gwn.SIGN_DEFAULT, 1e-15), (gwn.SIGN_UNSIGNED, 1e-15),
+                      (gwn.SIGN_FAST_WINDING_NUMBER, 1e-7)):
+        S, I, C = gwn.signed_distance(np.array([[0.5, 0.5, 0.5]]), V, F, sign)
+        assert abs(S[0] - np.sqrt(1 / 12)) < tol, (sign, S)
+        assert I[0] == 3 and np.allclose(C[0], [1 / 3, 1 / 3, 1 / 3], atol=1e-15)
+    with pytest.raises(gwn.GWNError):
+        gwn.signed_distance(np.zeros((1, 3)), V, F, gwn.SIGN_PSEUDONORMAL)
+
+
+@pytest.mark.parametrize("sign", SIGNS)
+def test_signed_distance_vs_oracle(sign):
+    V, F = D.bumpy_sphere(4, 8, 5)  # 5,120 triangles
+    rng = np.random.default_rng(sign)
+    lo, hi = V.min(0) - 0.3, V.max(0) + 0.3
+    P = np.concatenate([D.uniform_queries(lo, hi, 3000, rng), D.near_surface_queries(V, F, 1000, 1e-3, rng),
+                        V[:50] + 0.0])  # points on vertices: ties between faces
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
+    S, I, C = bvh.signed_distance(P, sign)
+    So, Io, Co = O.signed_distance(V, F, P, sign)
+    _check(V, F, P, S, I, C, So, Io, Co, sign)
+
+
+@pytest.mark.parametrize("bounds", [(-0.05, 0.05), (0.02, 0.1), (-0.2, -0.01)])
+def test_signed_distance_bounds(bounds):
+    V, F = D.bumpy_sphere(3, 6, 9)
+    rng = np.random.default_rng(11)
+    P = D.uniform_queries(V.min(0) - 0.2, V.max(0) + 0.2, 2000, rng)
+    S, I, C = gwn.FastWindingNumberBVH().build(V, F, 2).signed_distance(P, gwn.SIGN_WINDING_NUMBER, *bounds)
+    So, Io, Co = O.signed_distance(V, F, P, gwn.SIGN_WINDING_NUMBER, *bounds)
+    assert np.isnan(So).any() and (~np.isnan(So)).any()
+    _check(V, F, P, S, I, C, So, Io, Co, gwn.SIGN_WINDING_NUMBER)
+
+
+def test_signed_distance_edge_cases():
+    V, F = D.icosphere(2)
+    # empty mesh: signed_distance.cpp:397-403
+    S, I, C = gwn.signed_distance(np.zeros((3, 3)), V, F[:0], gwn.SIGN_FAST_WINDING_NUMBER)
+    assert np.all(np.isnan(S)) and np.all(I == -1) and np.all(np.isnan(C))
+    # no points
+    S, I, C = gwn.signed_distance(np.zeros((0, 3)), V, F)
+    assert S.shape == (0,) and I.shape == (0,) and C.shape == (0, 3)
+    # a single triangle, points in every region
+    V1 = np.array([[0., 0, 0], [1, 0, 0], [0, 1, 0]])
+    F1 = np.array([[0, 1, 2]], np.int32)
+    P = np.array([[0.2, 0.3, 2.0], [-1, -1, 0.5], [2, -1, 0], [-0.5, 3, 1], [0.5, -1, 0], [-1, 0.5, -2],
+                  [0.8, 0.8, 0]])
+    S, I, C = gwn.signed_distance(P, V1, F1, gwn.SIGN_UNSIGNED)
+    So, Io, Co = O.signed_distance(V1, F1, P, gwn.SIGN_UNSIGNED)
+    assert np.allclose(S, So, rtol=1e-15, atol=0) and np.array_equal(I, Io) and np.allclose(C, Co, atol=1e-15)
+
+
+def test_signed_distance_large_batch():
+    """1M points on the 81,920-triangle workload mesh, checked on a sample."""
+    V, F, Q = D.soup_workload()
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
+    S, I, C = bvh.signed_distance(Q, gwn.SIGN_FAST_WINDING_NUMBER)
+    sub = np.arange(0, Q.shape[0], 997)
+    So, Io, Co = O.signed_distance(V, F, Q[sub], gwn.SIGN_FAST_WINDING_NUMBER)
+    _check(V, F, Q[sub], S[sub], I[sub], C[sub], So, Io, Co, gwn.SIGN_FAST_WINDING_NUMBER)
+    # a batch of one gives the same answer as the point in the large batch
+    S1, I1, C1 = bvh.signed_distance(Q[12345:12346], gwn.SIGN_FAST_WINDING_NUMBER)
+    assert S1[0] == S[12345] and I1[0] == I[12345] and np.array_equal(C1[0], C[12345])

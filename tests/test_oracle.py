@@ -211,3 +211,83 @@ def test_oracle_octree_from_lists_matches_own_octree():
     assert np.array_equal(CM[ne], c.CM[ne]) and np.array_equal(R[ne], c.R[ne]) and np.array_equal(EC[ne], c.EC[ne])
     Q = rng.uniform(-2, 2, (200, 3)).astype(np.float32)
     assert np.array_equal(ev(Q, 2.0), c.eval(Q, 2.0))
+
+
+# ---------------- signed distance ----------------
+
+def test_point_triangle_closed_forms():
+    """Every Voronoi region of the right triangle (0,0,0),(1,0,0),(0,1,0)
+    (point_simplex_squared_distance.cpp:43-114) against hand-derived answers."""
+    a, b, c = np.array([0., 0, 0]), np.array([1., 0, 0]), np.array([0., 1, 0])
+    cases = [  # point, closest point
+        ((0.2, 0.3, 2.0), (0.2, 0.3, 0.0)),    # face interior
+        ((-1.0, -1.0, 0.5), (0.0, 0.0, 0.0)),  # vertex a
+        ((2.0, -1.0, 0.0), (1.0, 0.0, 0.0)),   # vertex b
+        ((-0.5, 3.0, 1.0), (0.0, 1.0, 0.0)),   # vertex c
+        ((0.5, -1.0, 0.0), (0.5, 0.0, 0.0)),   # edge ab
+        ((-1.0, 0.5, -2.0), (0.0, 0.5, 0.0)),  # edge ac
+        ((0.8, 0.8, 0.0), (0.5, 0.5, 0.0)),    # edge bc
+        ((0.25, 0.25, 0.0), (0.25, 0.25, 0.0)),  # on the face
+    ]
+    for p, q in cases:
+        d2, cp = O.point_tri_sqrdist(p, a, b, c)
+        assert np.allclose(cp, q, atol=1e-15), (p, cp)
+        assert abs(d2 - np.sum((np.array(p) - q) ** 2)) < 1e-14
+
+
+def test_point_triangle_vs_dense_sampling():
+    """Random triangles: the closest point lies on the triangle, and no point
+    of a fine barycentric grid is closer (up to the grid's resolution)."""
+    rng = np.random.default_rng(3)
+    m = 80
+    u, v = np.meshgrid(np.linspace(0, 1, m), np.linspace(0, 1, m))
+    keep = (u + v) <= 1.0
+    u, v = u[keep], v[keep]
+    for _ in range(50):
+        a, b, c = rng.normal(size=(3, 3))
+        p = rng.normal(size=3) * 2
+        d2, cp = O.point_tri_sqrdist(p, a, b, c)
+        grid = a + np.outer(u, b - a) + np.outer(v, c - a)
+        g2 = np.min(np.sum((grid - p) ** 2, axis=1))
+        assert d2 <= g2 + 1e-12
+        h = max(np.linalg.norm(b - a), np.linalg.norm(c - a)) / (m - 1)
+        assert np.sqrt(g2) - np.sqrt(d2) <= 2 * h
+        # cp is on the triangle's plane and inside it (barycentric >= 0)
+        M = np.stack([b - a, c - a], 1)
+        lam, *_ = np.linalg.lstsq(M, cp - a, rcond=None)
+        assert np.allclose(M @ lam, cp - a, atol=1e-10) and lam.min() > -1e-10 and lam.sum() < 1 + 1e-10
+
+
+def test_signed_distance_single_tet_golden():
+    # tests/include/igl/signed_distance.cpp:4-44
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    F = np.array([[0, 1, 3], [0, 2, 1], [0, 3, 2], [1, 2, 3]], np.int32)
+    P = np.array([[0.5, 0.5, 0.5]])
+    for t, tol in ((1, 1e-15), (2, 1e-15), (3, 1e-15), (4, 1e-7)):
+        S, I, C = O.signed_distance(V, F, P, t)
+        assert abs(S[0] - np.sqrt(1 / 12)) < tol, (t, S)
+        assert I[0] == 3 and np.allclose(C[0], [1 / 3, 1 / 3, 1 / 3], atol=1e-15)
+
+
+def test_signed_distance_signs_bounds_and_empty():
+    V, F = D.icosphere(3)
+    rng = np.random.default_rng(4)
+    P = rng.uniform(-1.4, 1.4, (300, 3))
+    r = np.linalg.norm(P, axis=1)
+    ok = np.abs(r - 1) > 0.05
+    Su, _, _ = O.signed_distance(V, F, P, 3)
+    assert np.all(Su >= 0)
+    # exact winding number: s = 1 - 2w = -1 / +1 to rounding; the fast winding
+    # number (order 2, accuracy_scale 2) is within ~1e-3 of 0 or 1
+    for t, rtol in ((1, 1e-12), (4, 5e-3)):
+        S, _, _ = O.signed_distance(V, F, P, t)
+        assert np.all(np.sign(S[ok]) == np.where(r[ok] < 1, -1, 1))
+        assert np.allclose(np.abs(S[ok]), Su[ok], rtol=rtol)
+    # bounds: only points with distance in [0.1, 0.3) keep a value
+    S, I, C = O.signed_distance(V, F, P, 3, lower=-0.3, upper=0.3)
+    # signed_distance.cpp:79-84: up = 0.3^2, low = max(0.3 - 0.6, 0)^2 = 0
+    inside = Su < 0.3
+    assert np.all(np.isnan(S[~inside])) and np.all(I[~inside] == F.shape[0] + 1) and np.all(C[~inside] == 0)
+    assert np.allclose(S[inside], Su[inside])
+    S, I, C = O.signed_distance(V, F[:0], P[:5], 4)
+    assert np.all(np.isnan(S)) and np.all(I == -1) and np.all(np.isnan(C))
