@@ -152,8 +152,11 @@ void soup_ensure_dist(gwn_soup* s, hipStream_t st);
 void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
                             double* S, int32_t* I, double* C, hipStream_t st);
+// traversal kernel: WALK_WAVE (wave-coherent, batch queries) or WALK_LANE
+// (per-lane, used by the serving engine); bitwise the same results
+constexpr int WALK_WAVE = 0, WALK_LANE = 1;
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk);
 
 }  // namespace gwn
 

@@ -202,7 +202,7 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)cn));
     uint8_t* ws = base + a256(12 * (size_t)cn) + a256(8 * (size_t)cn);
     GWN_HIP(hipMemcpyAsync(dQ, hQ, 12 * (size_t)n, hipMemcpyHostToDevice, sl->stream));
-    soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn));
+    soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn), WALK_LANE);
     GWN_HIP(hipMemcpyAsync(sl->hW[b], dW, 8 * (size_t)n, hipMemcpyDeviceToHost, sl->stream));
     GWN_HIP(hipEventRecord(sl->ev[b], sl->stream));
   }

@@ -374,23 +374,21 @@ __device__ __forceinline__ T tri_solid_angle(T ax, T ay, T az, T bx, T by, T bz,
 // library); max error 1.2e-8 in exact arithmetic, ~1e-7 rad when evaluated in
 // FP32 (comparable to a correctly rounded atan2f).
 __device__ __forceinline__ float atan01(float a) {
-#pragma clang fp contract(fast)
   const float s = a * a;
   float p = 0.0028501534834504128f;
-  p = p * s - 0.016069747507572174f;
-  p = p * s + 0.04269346594810486f;
-  p = p * s - 0.07504474371671677f;
-  p = p * s + 0.10641028732061386f;
-  p = p * s - 0.14203672111034393f;
-  p = p * s + 0.1999262422323227f;
-  p = p * s - 0.33333075046539307f;
-  p = p * s + 1.0f;
+  p = fmaf(p, s, -0.016069747507572174f);
+  p = fmaf(p, s, 0.04269346594810486f);
+  p = fmaf(p, s, -0.07504474371671677f);
+  p = fmaf(p, s, 0.10641028732061386f);
+  p = fmaf(p, s, -0.14203672111034393f);
+  p = fmaf(p, s, 0.1999262422323227f);
+  p = fmaf(p, s, -0.33333075046539307f);
+  p = fmaf(p, s, 1.0f);
   return a * p;
 }
 
 // atan2 for y != 0 (the caller returns early on a zero numerator).
 __device__ __forceinline__ float atan2_fast(float y, float x) {
-#pragma clang fp contract(fast)
   const float ax = fabsf(x), ay = fabsf(y);
   const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
   float r = atan01(mn * __builtin_amdgcn_rcpf(mx));
@@ -403,13 +401,12 @@ __device__ __forceinline__ float atan2_fast(float y, float x) {
 // the normalisation; same zero-length and zero-numerator rules.
 __device__ __forceinline__ float tri_solid_angle_fast(float ax, float ay, float az, float bx, float by, float bz,
                                                       float cx, float cy, float cz, float qx, float qy, float qz) {
-#pragma clang fp contract(fast)
   float a0 = ax - qx, a1 = ay - qy, a2 = az - qz;
   float b0 = bx - qx, b1 = by - qy, b2 = bz - qz;
   float c0 = cx - qx, c1 = cy - qy, c2 = cz - qz;
-  const float la = a0 * a0 + a1 * a1 + a2 * a2;
-  const float lb = b0 * b0 + b1 * b1 + b2 * b2;
-  const float lc = c0 * c0 + c1 * c1 + c2 * c2;
+  const float la = fmaf(a2, a2, fmaf(a1, a1, a0 * a0));
+  const float lb = fmaf(b2, b2, fmaf(b1, b1, b0 * b0));
+  const float lc = fmaf(c2, c2, fmaf(c1, c1, c0 * c0));
   if (la == 0.0f || lb == 0.0f || lc == 0.0f) return 0.0f;
   if (fminf(la, fminf(lb, lc)) < 1e-30f)  // keep rsq away from denormals
     return tri_solid_angle<float>(ax, ay, az, bx, by, bz, cx, cy, cz, qx, qy, qz);
@@ -419,26 +416,25 @@ __device__ __forceinline__ float tri_solid_angle_fast(float ax, float ay, float 
   c0 *= rc; c1 *= rc; c2 *= rc;
   const float u0 = b0 - a0, u1 = b1 - a1, u2 = b2 - a2;
   const float w0 = c0 - a0, w1 = c1 - a1, w2 = c2 - a2;
-  const float num = a0 * (u1 * w2 - u2 * w1) + a1 * (u2 * w0 - u0 * w2) + a2 * (u0 * w1 - u1 * w0);
+  const float k0 = fmaf(u1, w2, -(u2 * w1)), k1 = fmaf(u2, w0, -(u0 * w2)), k2 = fmaf(u0, w1, -(u1 * w0));
+  const float num = fmaf(a2, k2, fmaf(a1, k1, a0 * k0));
   if (num == 0.0f) return 0.0f;
-  const float den = 1.0f + (a0 * b0 + a1 * b1 + a2 * b2) + (a0 * c0 + a1 * c1 + a2 * c2) + (b0 * c0 + b1 * c1 + b2 * c2);
+  const float ab = fmaf(a2, b2, fmaf(a1, b1, a0 * b0));
+  const float ac = fmaf(a2, c2, fmaf(a1, c1, a0 * c0));
+  const float bc = fmaf(b2, c2, fmaf(b1, c1, b0 * c0));
+  const float den = ((1.0f + ab) + ac) + bc;
   return 2.0f * atan2_fast(num, den);
 }
 
 // Far-field Taylor value of one child slot (SolidAngleFunctors::pre,
-// 7199-7245), for q = query - P already formed and l2 = |q|^2.
-template <int ORDER, typename T, typename V4, bool FAST = false>
+// 7199-7245) from the reference's moment layout (FP64 records), for
+// q = query - P already formed and l2 = |q|^2.
+template <int ORDER, typename T, typename V4>
 __device__ __forceinline__ T far_field(T x, T y, T z, T l2, const V4& b, const V4& c, const V4& d, const V4& e,
                                        const V4& f) {
 #pragma clang fp contract(fast)
-  T m1, m2;
-  if constexpr (FAST) {
-    m1 = __builtin_amdgcn_rsqf(l2);
-    m2 = m1 * m1;
-  } else {
-    m2 = T(1) / l2;
-    m1 = sqrt(m2);
-  }
+  const T m2 = T(1) / l2;
+  const T m1 = sqrt(m2);
   x *= m1;
   y *= m1;
   z *= m1;
@@ -468,22 +464,25 @@ template <int ORDER>
 __device__ __forceinline__ float far_field_poly(float x, float y, float z, float l2, const float4& b,
                                                 const float4& c, const float4& d, const float4& e,
                                                 const float4& f) {
-#pragma clang fp contract(fast)
+  // explicit multiply-adds (no compiler contraction): every kernel that
+  // inlines this computes each term bit for bit the same
   const float m = __builtin_amdgcn_rsqf(l2);
   x *= m;
   y *= m;
   z *= m;
-  float t = x * b.x + y * b.y + z * b.z;  // x.(-N)
+  float t = fmaf(z, b.z, fmaf(y, b.y, x * b.x));  // x.(-N)
   if (ORDER >= 1) {
     const float z2 = z * z;
-    float qf = x * (c.x * x + c.w * y + d.y * z) + y * (c.y * y + d.x * z) + c.z * z2;
+    const float q1 = fmaf(d.y, z, fmaf(c.w, y, c.x * x)), q2 = fmaf(d.x, z, c.y * y);
+    float qf = fmaf(c.z, z2, fmaf(y, q2, x * q1));
     if (ORDER >= 2) {
-      const float gx = x * (d.z * x + e.y * y + e.z * z) + y * (e.w * y + f.w * z) + f.y * z2;
-      const float gy = y * (d.w * y + f.x * z) + f.z * z2;
-      const float g = x * gx + y * gy + e.x * (z2 * z);
-      qf += m * g;
+      const float g1 = fmaf(e.z, z, fmaf(e.y, y, d.z * x)), g2 = fmaf(f.w, z, e.w * y);
+      const float gx = fmaf(f.y, z2, fmaf(y, g2, x * g1));
+      const float gy = fmaf(f.z, z2, y * fmaf(f.x, z, d.w * y));
+      const float g = fmaf(e.x, z2 * z, fmaf(y, gy, x * gx));
+      qf = fmaf(m, g, qf);
     }
-    t += m * qf;
+    t = fmaf(m, qf, t);
   }
   return (m * m) * t;
 }
@@ -515,7 +514,6 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
                                                   const float* __restrict__ tri, const float* __restrict__ Q,
                                                   const uint32_t* __restrict__ perm, int64_t nq, float acc2,
                                                   float near_d, double* __restrict__ W) {
-#pragma clang fp contract(fast)
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
@@ -554,9 +552,9 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
         const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
         if (isfinite(o)) {  // otherwise the reference descends / evaluates the item
           if (MIXED)
-            sum64 += (double)o;
+            sum64 = __dadd_rn(sum64, (double)o);
           else
-            sum32 += o;
+            sum32 = __fadd_rn(sum32, o);
           exact = false;
           if (!leaf) skip = link;
         }
@@ -566,13 +564,13 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
         if (MIXED) {
           const float rr = sqrtf(a.w) + near_d;
           if (l2 < rr * rr) {
-            sum64 += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+            sum64 = __dadd_rn(sum64, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
           } else {
-            sum64 += (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
-                                                  qz);
+            sum64 = __dadd_rn(sum64, (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
+                                                  qz));
           }
         } else {
-          sum32 += tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+          sum32 = __fadd_rn(sum32, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
         }
       }
     }
@@ -582,15 +580,82 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
   W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
 }
 
+// Per-lane rope walk: each lane follows its own skip-pointer path and gathers
+// its own records.  It visits the same records in the same order and sums the
+// same terms as k_traverse; the running sums use non-contracting adds in both
+// kernels, so no multiply is fused into them differently and the results are
+// bitwise the same.  Used by the
+// serving engine, whose many small, concurrent batches over different resident
+// trees run faster this way (fewer iterations per wave; no scalar-cache
+// round trip on each step's critical path).
+template <int ORDER, bool MIXED>
+__global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict__ slots, uint32_t nslots,
+                                                       const float* __restrict__ tri, const float* __restrict__ Q,
+                                                       const uint32_t* __restrict__ perm, int64_t nq, float acc2,
+                                                       float near_d, double* __restrict__ W) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = perm ? (int64_t)perm[t] : t;
+  const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+  float sum32 = 0.0f;
+  double sum64 = 0.0;
+  uint32_t i = 0;
+  while (i < nslots) {
+    const float4* r = slots + 6 * (size_t)i;
+    const float4 a = r[0], b = r[1];
+    const uint32_t link = __float_as_uint(b.w);
+    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+    const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+    const bool leaf = (link & LINK_LEAF) != 0;
+    if (!(l2 <= __fmul_rn(a.w, acc2))) {
+      float4 c, d, e, f;
+      if (ORDER >= 1) {
+        c = r[2];
+        d = r[3];
+      }
+      if (ORDER >= 2) {
+        e = r[4];
+        f = r[5];
+      }
+      const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
+      if (isfinite(o)) {
+        if (MIXED)
+          sum64 = __dadd_rn(sum64, (double)o);
+        else
+          sum32 = __fadd_rn(sum32, o);
+        i = leaf ? i + 1 : link;
+        continue;
+      }
+    }
+    if (leaf) {
+      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+      if (MIXED) {
+        const float rr = sqrtf(a.w) + near_d;
+        if (l2 < rr * rr) {
+          sum64 = __dadd_rn(sum64, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+        } else {
+          sum64 = __dadd_rn(sum64, (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+        }
+      } else {
+        sum32 = __fadd_rn(sum32, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+      }
+    }
+    ++i;
+  }
+  W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
+}
+
 // all-FP64 form: FP64 moments, far field, atan2 and sum
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
                                                     uint32_t nslots, const float* __restrict__ tri,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ perm,
-                                                    int64_t nq, double acc2, double* __restrict__ W) {
-  // the same wave-coherent rope walk as k_traverse, over the FP64 records
+                                                    int64_t nq, const uint32_t* __restrict__ nq_dev, double acc2,
+                                                    double* __restrict__ W) {
+  // the same wave-coherent rope walk as k_traverse, over the FP64 records;
+  // with nq_dev the query count is read on the device (launch sized for nq)
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
+  if (t >= (nq_dev ? (int64_t)*nq_dev : nq)) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   double sum = 0.0;
@@ -610,14 +675,14 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
             e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
         const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
         if (isfinite(o)) {
-          sum += o;
+          sum = __dadd_rn(sum, o);
           exact = false;
           if (!leaf) skip = link;
         }
       }
       if (exact) {
         const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-        sum += tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz);
+        sum = __dadd_rn(sum, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
       }
     }
     const bool step = leaf || __ballot(skip == i + 1) != 0;
@@ -812,7 +877,7 @@ size_t soup_query_work_bytes(int64_t nq) {
 }
 
 void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes);
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk);
 
 // 30-bit Morton keys of the queries over a box (queries outside are clamped
 // to it), and the identity permutation to be sorted along with them
@@ -830,14 +895,14 @@ void soup_query_device(const gwn_soup* sc, const float* Q, int64_t nq, float acc
   if (nq <= 0) return;
   const size_t need = soup_query_work_bytes(nq);
   s->work.reserve(need);
-  soup_query_device_ws(sc, Q, nq, accuracy_scale, precision, W, st, s->work.p, s->work.bytes());
+  soup_query_device_ws(sc, Q, nq, accuracy_scale, precision, W, st, s->work.p, s->work.bytes(), WALK_WAVE);
 }
 
 // Query with caller-provided device workspace (soup_query_work_bytes(nq)
 // bytes); calls with distinct workspaces may run concurrently on distinct
 // streams against the same tree once its FP64 moments exist.
 void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes) {
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk) {
   gwn_soup* s = const_cast<gwn_soup*>(sc);
   if (nq <= 0) return;
   if (s->nslots == 0) {
@@ -872,17 +937,24 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st);
     switch (s->order) {
-      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
-      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
-      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
     }
     check_launch("k_traverse64");
     return;
   }
   const bool mixed = precision == GWN_PREC_MIXED;
   const float nd = s->mixed.near_dist;
-#define GWN_LAUNCH_T(O, M) \
-  hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, acc2, nd, W)
+#define GWN_LAUNCH_T(O, M)                                                                                      \
+  do {                                                                                                          \
+    if (walk == WALK_LANE)                                                                                      \
+      hipLaunchKernelGGL((k_traverse_lane<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, \
+                         acc2, nd, W);                                                                          \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq,  \
+                         acc2, nd, W);                                                                          \
+  } while (0)
   {
   ProfileScope prof("soup_traverse", st);
   if (mixed) {
@@ -907,19 +979,15 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
     hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
                        ambn);
     check_launch("k_collect_ambiguous");
-    uint32_t na = 0;
-    GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
-    GWN_HIP(hipStreamSynchronize(st));
-    if (na > 0) {
-      const double a2 = (double)accuracy_scale * (double)accuracy_scale;
-      const unsigned ga = div_up(na, 256);
-      switch (s->order) {
-        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
-        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
-        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
-      }
-      check_launch("k_traverse64 (ambiguous)");
+    // the FP64 pass reads the number of collected queries on the device: no
+    // host round trip in the middle of a query (blocks past the count exit)
+    const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+    switch (s->order) {
+      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
+      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
     }
+    check_launch("k_traverse64 (ambiguous)");
   }
 }
 
