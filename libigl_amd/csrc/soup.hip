@@ -650,12 +650,10 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
                                                     uint32_t nslots, const float* __restrict__ tri,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ perm,
-                                                    int64_t nq, const uint32_t* __restrict__ nq_dev, double acc2,
-                                                    double* __restrict__ W) {
-  // the same wave-coherent rope walk as k_traverse, over the FP64 records;
-  // with nq_dev the query count is read on the device (launch sized for nq)
+                                                    int64_t nq, double acc2, double* __restrict__ W) {
+  // the same wave-coherent rope walk as k_traverse, over the FP64 records
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= (nq_dev ? (int64_t)*nq_dev : nq)) return;
+  if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   double sum = 0.0;
@@ -937,9 +935,9 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st);
     switch (s->order) {
-      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
     }
     check_launch("k_traverse64");
     return;
@@ -979,15 +977,22 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
     hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
                        ambn);
     check_launch("k_collect_ambiguous");
-    // the FP64 pass reads the number of collected queries on the device: no
-    // host round trip in the middle of a query (blocks past the count exit)
-    const double a2 = (double)accuracy_scale * (double)accuracy_scale;
-    switch (s->order) {
-      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
-      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
-      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, nq, ambn, a2, W); break;
+    // the count comes back to the host so that the FP64 pass is launched at its
+    // size (a launch sized for every query, reading the count on the device,
+    // measured slower under concurrent serving load)
+    uint32_t na = 0;
+    GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    if (na > 0) {
+      const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+      const unsigned ga = div_up(na, 256);
+      switch (s->order) {
+        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+      }
+      check_launch("k_traverse64 (ambiguous)");
     }
-    check_launch("k_traverse64 (ambiguous)");
   }
 }
 
