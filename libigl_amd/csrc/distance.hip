@@ -125,6 +125,48 @@ __global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox
   const double p[3] = {P[3 * pi], P[3 * pi + 1], P[3 * pi + 2]};
   double best = up2, bc[3] = {0.0, 0.0, 0.0};
   int32_t bf = -1;
+  // Initial bound: descend greedily into the child whose box is nearest (at
+  // most 4 siblings per level) down to one triangle.  The walk below then
+  // replaces it only by strictly closer triangles, so the answer is still the
+  // minimum; among equally close faces the greedy one is kept.
+  {
+    uint32_t lo = 0, hi = nslots;
+    for (int depth = 0; depth < 128 && lo < hi; ++depth) {
+      uint32_t pick = 0xFFFFFFFFu, plink = 0;
+      double pd = best;
+      for (uint32_t k = lo; k < hi;) {
+        const float4 blo = dbox[2 * (size_t)k], bhi = dbox[2 * (size_t)k + 1];
+        const uint32_t l = __float_as_uint(blo.w);
+        const double d = box_d2(p, blo, bhi);
+        if (d < pd) {
+          pd = d;
+          pick = k;
+          plink = l;
+        }
+        k = (l & DL_LEAF) ? k + 1 : l;
+      }
+      if (pick == 0xFFFFFFFFu) break;
+      if (plink & DL_LEAF) {
+        const uint32_t f = plink & ~DL_LEAF;
+        const int32_t* tf = F + 3 * (size_t)f;
+        double A[3], B[3], C[3], c[3];
+        for (int k = 0; k < 3; ++k) {
+          A[k] = V[3 * (int64_t)tf[0] + k];
+          B[k] = V[3 * (int64_t)tf[1] + k];
+          C[k] = V[3 * (int64_t)tf[2] + k];
+        }
+        const double d2 = tri_closest(p, A, B, C, c);
+        if (d2 < best) {
+          best = d2;
+          bf = (int32_t)f;
+          for (int k = 0; k < 3; ++k) bc[k] = c[k];
+        }
+        break;
+      }
+      lo = pick + 1;
+      hi = plink;
+    }
+  }
   uint32_t skip = 0, i = 0;
   while (i < nslots) {
     const float4 lo = dbox[2 * (size_t)i], hi = dbox[2 * (size_t)i + 1];
