@@ -201,6 +201,16 @@ int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sig
                                double upper_bound, double* S, int32_t* I, double* C, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Mesh files (read_triangle_mesh.h; readOFF.cpp, readOBJ.cpp,
+ * polygons_to_triangles.cpp): an .off or .obj file (chosen by extension) as a
+ * triangle mesh, polygons split into fans (p0, pc, pc+1).  *V (nv x 3 double)
+ * and *F (nf x 3 int32, 0-based, indices as in the file) are allocated here;
+ * release them with gwn_free.
+ */
+int gwn_read_triangle_mesh(const char* path, double** V, int64_t* nv, int32_t** F, int64_t* nf);
+void gwn_free(void* p);
+
+/* ------------------------------------------------------------------------
  * Serving: continuous batching of soup queries against resident meshes (no
  * counterpart in the reference, whose callers such as
  * signed_distance.cpp:50-121 query one point at a time on the host).
@@ -231,6 +241,8 @@ void gwn_server_destroy(gwn_server* s);
 /* builds the mesh's tree (and FP64 moments) and keeps it resident */
 int gwn_server_add_mesh(gwn_server* s, const double* V, int64_t nv, const int32_t* F, int64_t nf, int order,
                         int tree_kind, int32_t* mesh_id);
+/* the same for a mesh file (gwn_read_triangle_mesh) */
+int gwn_server_add_mesh_file(gwn_server* s, const char* path, int order, int tree_kind, int32_t* mesh_id);
 int gwn_server_submit(gwn_server* s, int32_t mesh_id, const double* Q, int64_t nq, float accuracy_scale,
                       int precision, double* W, int64_t* request_id);
 /* blocks until the request is done; returns its status */

@@ -15,6 +15,7 @@
 //   igl::fast_winding_number(P,N,A,Q,order,beta,WN)   fast_winding_number.h:111-118
 //   igl::fast_winding_number(P,N,A,Q,WN)              fast_winding_number.h:126-131
 //   struct igl::FastWindingNumberBVH                  fast_winding_number.h:140-152
+//   igl::read_triangle_mesh(str,V,F)                   read_triangle_mesh.h (.off / .obj)
 //   igl::SignedDistanceType                           signed_distance.h:20-34
 //   igl::signed_distance(P,V,F,type,lower,upper,S,I,C,N)  signed_distance.h:61-71
 //   igl::signed_distance(P,V,F,type,S,I,C,N)          signed_distance.h:81-89
@@ -317,6 +318,25 @@ void signed_distance(const MP& P, const MV& V, const MF& F, int type, double low
   signed_distance_bvh(bvh.handle, P, type, lower, upper, S, I, C);
 }
 
+// .off / .obj file; returns false (and leaves gwn_last_error) on failure, as
+// the reference returns false
+template <typename MV, typename MF>
+bool read_triangle_mesh(const std::string& path, MV& V, MF& F) {
+  double* v = nullptr;
+  int32_t* f = nullptr;
+  int64_t nv = 0, nf = 0;
+  if (gwn_read_triangle_mesh(path.c_str(), &v, &nv, &f, &nf) != GWN_OK) return false;
+  V.resize(nv, 3);
+  F.resize(nf, 3);
+  for (int64_t i = 0; i < nv; ++i)
+    for (int j = 0; j < 3; ++j) V(i, j) = static_cast<typename std::decay<decltype(V(0, 0))>::type>(v[3 * i + j]);
+  for (int64_t i = 0; i < nf; ++i)
+    for (int j = 0; j < 3; ++j) F(i, j) = static_cast<typename std::decay<decltype(F(0, 0))>::type>(f[3 * i + j]);
+  gwn_free(v);
+  gwn_free(f);
+  return true;
+}
+
 }  // namespace gwn_detail
 }  // namespace igl
 
@@ -366,6 +386,12 @@ void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::Matr
                          Eigen::PlainObjectBase<DerivedWN>& WN) {
   gwn_detail::cloud_eval(P.derived(), N.derived(), A.derived(), point_indices, CH.derived(), CM.derived(),
                          R.derived(), EC.derived(), Q.derived(), (double)beta, WN.derived());
+}
+
+template <typename DerivedV, typename DerivedF>
+bool read_triangle_mesh(const std::string str, Eigen::PlainObjectBase<DerivedV>& V,
+                        Eigen::PlainObjectBase<DerivedF>& F) {
+  return gwn_detail::read_triangle_mesh(str, V.derived(), F.derived());
 }
 
 template <typename DerivedP, typename DerivedV, typename DerivedF, typename DerivedS, typename DerivedI,

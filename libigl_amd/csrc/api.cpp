@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -318,6 +319,34 @@ int gwn_soup_copy_boxdata(const gwn_soup* s, float* out) {
       GWN_HIP(hipMemcpy(out, s->boxdata.p, sizeof(float) * 92 * (size_t)s->nnodes, hipMemcpyDeviceToHost));
   });
 }
+
+// ---------------- mesh files ----------------
+int gwn_read_triangle_mesh(const char* path, double** V, int64_t* nv, int32_t** F, int64_t* nf) {
+  return guard([&] {
+    GWN_CHECK(V && nv && F && nf, GWN_ERR_INVALID, "null output");
+    *V = nullptr;
+    *F = nullptr;
+    *nv = *nf = 0;
+    std::vector<double> v;
+    std::vector<int32_t> f;
+    read_triangle_mesh(path, v, f);
+    double* vo = (double*)std::malloc(sizeof(double) * std::max<size_t>(1, v.size()));
+    int32_t* fo = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(1, f.size()));
+    if (!vo || !fo) {
+      std::free(vo);
+      std::free(fo);
+      throw Error{GWN_ERR_NOMEM, "out of host memory"};
+    }
+    if (!v.empty()) std::memcpy(vo, v.data(), sizeof(double) * v.size());
+    if (!f.empty()) std::memcpy(fo, f.data(), sizeof(int32_t) * f.size());
+    *V = vo;
+    *F = fo;
+    *nv = (int64_t)(v.size() / 3);
+    *nf = (int64_t)(f.size() / 3);
+  });
+}
+
+void gwn_free(void* p) { std::free(p); }
 
 // ---------------- signed distance ----------------
 static void sd_check(gwn_soup* s, int64_t np, int sign_type, const double* P, double* S, int32_t* I, double* C) {

@@ -378,6 +378,19 @@ int gwn_server_add_mesh(gwn_server* s, const double* V, int64_t nv, const int32_
   return GWN_OK;
 }
 
+int gwn_server_add_mesh_file(gwn_server* s, const char* path, int order, int tree_kind, int32_t* mesh_id) {
+  std::vector<double> V;
+  std::vector<int32_t> F;
+  try {
+    read_triangle_mesh(path, V, F);
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  }
+  return gwn_server_add_mesh(s, V.data(), (int64_t)(V.size() / 3), F.data(), (int64_t)(F.size() / 3), order,
+                             tree_kind, mesh_id);
+}
+
 int gwn_server_submit(gwn_server* s, int32_t mesh_id, const double* Q, int64_t nq, float accuracy_scale,
                       int precision, double* W, int64_t* request_id) {
   if (!s || !request_id || nq < 0 || (nq > 0 && (!Q || !W)) || precision < GWN_PREC_FP32 ||

@@ -302,6 +302,23 @@ def signed_distance(P, V, F, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL
     return FastWindingNumberBVH().build(V, F, 2).signed_distance(P, sign_type, lower_bound, upper_bound)
 
 
+def read_triangle_mesh(path):
+    """igl::read_triangle_mesh for .off / .obj: (V float64 nv x 3, F int32 nf x 3),
+    polygons split into fans."""
+    import ctypes
+    pv, pf = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_int32)()
+    nv, nf = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().gwn_read_triangle_mesh(str(path).encode(), ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pf),
+                                       ctypes.byref(nf)))
+    try:
+        V = np.ctypeslib.as_array(pv, shape=(max(nv.value, 1) * 3,))[: 3 * nv.value].copy().reshape(-1, 3)
+        F = np.ctypeslib.as_array(pf, shape=(max(nf.value, 1) * 3,))[: 3 * nf.value].copy().reshape(-1, 3)
+    finally:
+        lib().gwn_free(ctypes.cast(pv, ctypes.c_void_p))
+        lib().gwn_free(ctypes.cast(pf, ctypes.c_void_p))
+    return V, F
+
+
 def sah4_tree(V, F):
     """Host-only reference-topology tree (4 uint32 per node, reference encoding)."""
     import ctypes

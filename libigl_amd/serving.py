@@ -49,6 +49,12 @@ class Server:
         """Device bytes a request of num_queries holds while it runs."""
         return int(lib().gwn_server_request_bytes(int(num_queries)))
 
+    def add_mesh_file(self, path, order=2, tree=nat.TREE_SAH4):
+        """Reads an .off / .obj file and keeps its tree resident."""
+        mid = C.c_int32()
+        check(lib().gwn_server_add_mesh_file(self._h, str(path).encode(), order, tree, C.byref(mid)))
+        return mid.value
+
     def submit(self, mesh, Q, accuracy_scale=2.0, precision=nat.PREC_FP32, out=None):
         """Queues a request; `out` (float64, nq) receives the result if given
         (a caller that reuses its result buffers avoids first-touch page

@@ -92,6 +92,20 @@ int main() {
         bad |= 128;
     if (CH.rows() != 1 || PI.size() != 1 || EC.cols() != 39) bad |= 256;
   }
+  // read_triangle_mesh: a quad written here reads as two triangles
+  {
+    const char* path = "/tmp/gwn_eigen_facade_quad.obj";
+    FILE* fp = std::fopen(path, "w");
+    if (fp) {
+      std::fputs("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n", fp);
+      std::fclose(fp);
+    }
+    Eigen::MatrixXd Vm;
+    Eigen::MatrixXi Fm;
+    if (!igl::read_triangle_mesh(path, Vm, Fm) || Vm.rows() != 4 || Fm.rows() != 2 || Fm(1, 2) != 3) bad |= 512;
+    std::remove(path);
+    if (igl::read_triangle_mesh("/tmp/does_not_exist.xyz", Vm, Fm)) bad |= 1024;
+  }
   std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
   return bad;
 }
