@@ -200,6 +200,12 @@ int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type,
 int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
                                double upper_bound, double* S, int32_t* I, double* C, void* stream);
 
+/* 2D curves: igl::winding_number(V,F,O,W) with F.cols() == 2
+ * (winding_number.cpp:22-33, 62-89; signed_angle.cpp:8-41), FP64 on the GPU.
+ * V nv x 2, E ne x 2 (edges), O no x 2, W no; host buffers. */
+int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, int64_t ne, const double* O,
+                                int64_t no, double* W);
+
 /* ------------------------------------------------------------------------
  * Mesh files (read_triangle_mesh.h; readOFF.cpp, readOBJ.cpp,
  * polygons_to_triangles.cpp): an .off or .obj file (chosen by extension) as a

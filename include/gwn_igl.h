@@ -1,7 +1,7 @@
 // gwn_igl.h -- libigl-signature facade over the gwn C ABI (header only).
 //
 // Replaces, with the same names, argument meaning and output resizing:
-//   igl::winding_number(V,F,O,W)                      winding_number.h:41-48
+//   igl::winding_number(V,F,O,W)                      winding_number.h:41-48 (triangles, or 2D edges)
 //   igl::winding_number(V,F,p)                        winding_number.h:49-60
 //   igl::octree(P,point_indices,CH,CN,W)              octree.h:36-41
 //   igl::fast_winding_number(P,N,A,point_indices,CH,expansion_order,CM,R,EC)
@@ -95,6 +95,23 @@ std::vector<int32_t> faces(const M& F) {
 
 template <typename MV, typename MF, typename MO, typename MW>
 void winding_number(const MV& V, const MF& F, const MO& O, MW& W) {
+  if (F.cols() == 2) {  // 2D curve of edges (winding_number.cpp:24-33)
+    if (V.cols() != 2 || (O.rows() > 0 && O.cols() != 2)) throw std::runtime_error("2D winding number needs 2D V and O");
+    std::vector<double> v(2 * (size_t)V.rows()), o(2 * (size_t)O.rows());
+    std::vector<int32_t> e(2 * (size_t)F.rows());
+    for (int64_t i = 0; i < (int64_t)V.rows(); ++i)
+      for (int j = 0; j < 2; ++j) v[2 * i + j] = static_cast<double>(V(i, j));
+    for (int64_t i = 0; i < (int64_t)F.rows(); ++i)
+      for (int j = 0; j < 2; ++j) e[2 * i + j] = static_cast<int32_t>(F(i, j));
+    for (int64_t i = 0; i < (int64_t)O.rows(); ++i)
+      for (int j = 0; j < 2; ++j) o[2 * i + j] = static_cast<double>(O(i, j));
+    std::vector<double> w((size_t)O.rows());
+    if (O.rows() > 0)
+      check(gwn_exact_winding_number_2d(v.data(), V.rows(), e.data(), F.rows(), o.data(), O.rows(), w.data()));
+    W.resize(O.rows(), 1);
+    for (int64_t i = 0; i < (int64_t)O.rows(); ++i) W(i, 0) = static_cast<typename std::decay<decltype(W(0, 0))>::type>(w[i]);
+    return;
+  }
   const auto v = rows3<double>(V, "V");
   const auto f = faces(F);
   const auto o = rows3<double>(O, "O");

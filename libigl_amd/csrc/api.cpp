@@ -28,6 +28,8 @@ void ensure_device() {
 }
 
 // defined in the .hip translation units
+void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no, double* W,
+                    hipStream_t st, DevBuf<double>& scratch);
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
                   hipStream_t st, DevBuf<double>& tri_scratch);
 void soup_setup(gwn_soup* s, hipStream_t st);
@@ -317,6 +319,31 @@ int gwn_soup_copy_boxdata(const gwn_soup* s, float* out) {
     GWN_CHECK(s && out, GWN_ERR_INVALID, "null argument");
     if (s->nnodes)
       GWN_HIP(hipMemcpy(out, s->boxdata.p, sizeof(float) * 92 * (size_t)s->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+// ---------------- 2D exact winding number ----------------
+int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, int64_t ne, const double* O,
+                                int64_t no, double* W) {
+  return guard([&] {
+    GWN_CHECK(nv >= 0 && ne >= 0 && no >= 0, GWN_ERR_INVALID, "negative size");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    if (no == 0) return;
+    if (ne > 0) {
+      GWN_CHECK(V && E, GWN_ERR_INVALID, "null curve");
+      for (int64_t i = 0; i < 2 * ne; ++i)
+        GWN_CHECK(E[i] >= 0 && E[i] < nv, GWN_ERR_INVALID, "edge index out of range");
+    }
+    ensure_device();
+    hipStream_t st = nullptr;
+    DevBuf<double> dV, dO, dW(no), scratch;
+    DevBuf<int32_t> dE;
+    upload(dV, V, 2 * (size_t)nv, st);
+    upload(dE, E, 2 * (size_t)ne, st);
+    upload(dO, O, 2 * (size_t)no, st);
+    exact2d_device(dV.p, dE.p, ne, dO.p, no, dW.p, st, scratch);
+    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
   });
 }
 

@@ -77,6 +77,66 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__
   }
 }
 
+// ---- 2D: winding number of a curve of edges (winding_number.cpp:62-89 with
+// F.cols() == 2): W(o) = sum over edges (a,b) of signed_angle(a, b, o)
+// (signed_angle.cpp:8-41): -atan2(cross, dot) of the unit vectors from a and b
+// to o, each term divided by 2 pi and summed in edge order, as the reference.
+__device__ __forceinline__ double signed_angle_2d(const double* e, double px, double py) {
+  double ax = px - e[0], ay = py - e[1], bx = px - e[2], by = py - e[3];
+  const double la = sqrt(ax * ax + ay * ay), lb = sqrt(bx * bx + by * by);
+  if (la != 0.0) {
+    ax /= la;
+    ay /= la;
+  }
+  if (lb != 0.0) {
+    bx /= lb;
+    by /= lb;
+  }
+  return -atan2(bx * ay - by * ax, bx * ax + by * ay) / (2.0 * EX_PI);
+}
+
+__global__ __launch_bounds__(EX_THREADS) void k_exact2d_gather(const double* __restrict__ V,
+                                                                 const int32_t* __restrict__ E, int64_t ne,
+                                                                 double* __restrict__ T) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ne) return;
+  for (int k = 0; k < 2; ++k)
+    for (int a = 0; a < 2; ++a) T[4 * f + 2 * k + a] = V[2 * (int64_t)E[2 * f + k] + a];
+}
+
+__global__ __launch_bounds__(EX_THREADS) void k_exact2d(const double* __restrict__ T, int64_t ne,
+                                                         const double* __restrict__ O, int64_t no,
+                                                         double* __restrict__ W) {
+  __shared__ double tile[EX_TILE * 4];
+  const int64_t q = (int64_t)blockIdx.x * EX_THREADS + threadIdx.x;
+  const bool ok = q < no;
+  const double px = ok ? O[2 * q] : 0.0, py = ok ? O[2 * q + 1] : 0.0;
+  double acc = 0.0;
+  for (int64_t f0 = 0; f0 < ne; f0 += EX_TILE) {
+    const int cnt = (int)((ne - f0) < EX_TILE ? (ne - f0) : EX_TILE);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt * 4; e += EX_THREADS) tile[e] = T[4 * f0 + e];
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) acc += signed_angle_2d(tile + 4 * j, px, py);
+  }
+  if (ok) W[q] = acc;
+}
+
+void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no, double* W,
+                    hipStream_t st, DevBuf<double>& scratch) {
+  if (no <= 0) return;
+  if (ne <= 0) {
+    GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * no, st));
+    return;
+  }
+  scratch.reserve(4 * (size_t)ne);
+  hipLaunchKernelGGL(k_exact2d_gather, dim3(div_up(ne, EX_THREADS)), dim3(EX_THREADS), 0, st, V, E, ne, scratch.p);
+  check_launch("k_exact2d_gather");
+  ProfileScope prof("exact2d", st);
+  hipLaunchKernelGGL(k_exact2d, dim3(div_up(no, EX_THREADS)), dim3(EX_THREADS), 0, st, scratch.p, ne, O, no, W);
+  check_launch("k_exact2d");
+}
+
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
                   hipStream_t st, DevBuf<double>& tri_scratch) {
   (void)nv;

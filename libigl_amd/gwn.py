@@ -40,7 +40,16 @@ def _rows3(X, name, dtype):
 
 
 def winding_number(V, F, O):
-    """Exact generalized winding number, FP64 (igl::winding_number(V,F,O,W))."""
+    """Exact generalized winding number, FP64 (igl::winding_number(V,F,O,W)):
+    triangles in 3D (F n x 3) or edges of a 2D curve (F n x 2, V and O 2D)."""
+    F = np.asarray(F)
+    if F.ndim == 2 and F.shape[1] == 2:
+        V = np.ascontiguousarray(np.asarray(V, dtype=np.float64).reshape(-1, 2))
+        E = np.ascontiguousarray(F, dtype=np.int32)
+        O = np.ascontiguousarray(np.asarray(O, dtype=np.float64).reshape(-1, 2))
+        W = np.empty(O.shape[0], np.float64)
+        check(lib().gwn_exact_winding_number_2d(V, V.shape[0], E, E.shape[0], O, O.shape[0], W))
+        return W
     V = _rows3(V, "V", np.float64)
     F = _rows3(F, "F", np.int32)
     O = _rows3(O, "O", np.float64)

@@ -85,6 +85,12 @@ int oracle_signed_distance(const double* V, int64_t nv, const int32_t* F, int64_
                            int sign_type, double lower, double upper, double* S, int32_t* I, double* C,
                            int nthreads);
 
+/* ---- 2D curves (winding_number.cpp:62-89 with F.cols() == 2) ----
+ * signed_angle.cpp:8-41; V nv x 2, E ne x 2, O no x 2 */
+double oracle_signed_angle_2d(const double* a, const double* b, const double* p);
+void oracle_winding_number_2d(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no,
+                              double* W);
+
 /* ---- point cloud: igl::octree + fast_winding_number(P,N,A,...) ---- */
 /* octree (octree.cpp:7-169).  Cells are numbered exactly as the reference
  * numbers them.  Outputs are malloc'ed and owned by the caller (free()).

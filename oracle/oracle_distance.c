@@ -11,6 +11,8 @@
  *                              query replaced by a brute-force minimum over
  *                              all faces (same strict "<" replacement rule as
  *                              AABB.cpp set_min, faces in index order)
+ *   oracle_winding_number_2d   winding_number.cpp:62-89 for 2D curves
+ *                              (signed_angle.cpp:8-41 summed over edges)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -186,4 +188,32 @@ int oracle_signed_distance(const double* V, int64_t nv, const int32_t* F, int64_
   free(d2);
   free(W);
   return 0;
+}
+
+/* ---- 2D curves: winding_number.cpp:62-89 (F.cols() == 2) summing
+ * signed_angle.cpp:8-41 over edges in order ---- */
+double oracle_signed_angle_2d(const double* a, const double* b, const double* p) {
+  double oa[2], ob[2], la = 0, lb = 0;
+  for (int i = 0; i < 2; ++i) {
+    oa[i] = p[i] - a[i];
+    ob[i] = p[i] - b[i];
+    la += oa[i] * oa[i];
+    lb += ob[i] * ob[i];
+  }
+  la = sqrt(la);
+  lb = sqrt(lb);
+  for (int i = 0; i < 2; ++i) {
+    if (la != 0) oa[i] /= la;
+    if (lb != 0) ob[i] /= lb;
+  }
+  return -atan2(ob[0] * oa[1] - ob[1] * oa[0], ob[0] * oa[0] + ob[1] * oa[1]) / (2. * 3.1415926535897932384626433832795);
+}
+
+void oracle_winding_number_2d(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no,
+                              double* W) {
+  for (int64_t o = 0; o < no; ++o) {
+    double w = 0;
+    for (int64_t e = 0; e < ne; ++e) w += oracle_signed_angle_2d(V + 2 * (int64_t)E[2 * e], V + 2 * (int64_t)E[2 * e + 1], O + 2 * o);
+    W[o] = w;
+  }
 }

@@ -66,6 +66,7 @@ def lib():
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        L.oracle_winding_number_2d.argtypes = [_f64p, _i32p, C.c_int64, _f64p, C.c_int64, _f64p]
         L.oracle_point_tri_sqrdist.restype = C.c_double
         L.oracle_point_tri_sqrdist.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
         L.oracle_signed_distance.restype = C.c_int
@@ -103,6 +104,16 @@ def ref():
 
 def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------- 2D curves ----------------
+def winding_number_2d(V, E, O):
+    V = _c(V, np.float64).reshape(-1, 2)
+    E = _c(E, np.int32).reshape(-1, 2)
+    O = _c(O, np.float64).reshape(-1, 2)
+    W = np.empty(O.shape[0])
+    lib().oracle_winding_number_2d(V, E, E.shape[0], O, O.shape[0], W)
+    return W
 
 
 # ---------------- signed distance ----------------

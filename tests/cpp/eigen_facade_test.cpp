@@ -106,6 +106,20 @@ int main() {
     std::remove(path);
     if (igl::read_triangle_mesh("/tmp/does_not_exist.xyz", Vm, Fm)) bad |= 1024;
   }
+  // 2D winding number of a counter-clockwise square (F.cols() == 2)
+  {
+    Eigen::MatrixXd V2(4, 2), O2(2, 2);
+    Eigen::MatrixXi E2(4, 2);
+    const double sq[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    for (int i = 0; i < 4; ++i) {
+      V2(i, 0) = sq[i][0], V2(i, 1) = sq[i][1];
+      E2(i, 0) = i, E2(i, 1) = (i + 1) % 4;
+    }
+    O2(0, 0) = 0.5, O2(0, 1) = 0.5, O2(1, 0) = 2.0, O2(1, 1) = 0.5;
+    Eigen::VectorXd W2;
+    igl::winding_number(V2, E2, O2, W2);
+    if (W2.rows() != 2 || std::fabs(W2(0, 0) - 1.0) > 1e-15 || std::fabs(W2(1, 0)) > 1e-15) bad |= 2048;
+  }
   std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
   return bad;
 }

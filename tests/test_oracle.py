@@ -291,3 +291,24 @@ def test_signed_distance_signs_bounds_and_empty():
     assert np.allclose(S[inside], Su[inside])
     S, I, C = O.signed_distance(V, F[:0], P[:5], 4)
     assert np.all(np.isnan(S)) and np.all(I == -1) and np.all(np.isnan(C))
+
+
+def test_winding_number_2d_closed_forms():
+    """2D curves (winding_number.cpp with F.cols() == 2): a counter-clockwise
+    square is 1 inside and 0 outside, clockwise is -1; a point on the curve's
+    vertex contributes 0 for that edge end; half of a closed curve seen from its
+    centre is 1/2; a circle polygon matches the angle it subtends."""
+    V = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    E = np.array([[0, 1], [1, 2], [2, 3], [3, 0]], np.int32)
+    Q = np.array([[0.5, 0.5], [0.1, 0.9], [2, 0.5], [-1, -1], [0.5, 3]])
+    W = O.winding_number_2d(V, E, Q)
+    assert np.allclose(W, [1, 1, 0, 0, 0], atol=1e-15)
+    assert np.allclose(O.winding_number_2d(V, E[:, ::-1], Q), [-1, -1, 0, 0, 0], atol=1e-15)
+    # half the square's boundary seen from its centre: the two edges cover 180 degrees
+    assert abs(O.winding_number_2d(V, E[:2], Q[:1])[0] - 0.5) < 1e-15
+    # an open arc of a circle from angle 0 to t seen from the centre: t / (2 pi)
+    t = 2.0
+    ang = np.linspace(0, t, 200)
+    C = np.stack([np.cos(ang), np.sin(ang)], 1)
+    Ea = np.stack([np.arange(199), np.arange(1, 200)], 1).astype(np.int32)
+    assert abs(O.winding_number_2d(C, Ea, np.zeros((1, 2)))[0] - t / (2 * np.pi)) < 1e-14
