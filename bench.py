@@ -220,6 +220,10 @@ def main():
                     "frac": round(achieved / 157.3, 4),
                     "traffic": (MEASURED_TRAFFIC.get(beta_acc)
                                 if (nq == 1_000_000 and args.tree == "sah4") else None),
+                    "hbm_GBps": (round(MEASURED_TRAFFIC[beta_acc] / (k_ms * 1e-3) / 1e9, 1)
+                                 if (beta_acc in MEASURED_TRAFFIC and nq == 1_000_000 and args.tree == "sah4")
+                                 else None),
+                    "hbm_peak_GBps": 8000.0,
                     "kernel": "k_traverse<2,false>", "kernel_ms": round(k_ms, 4),
                     "flop_per_query": round(flops_per_q, 1),
                     "note": "FP32 VALU; no MFMA on this path; packed FP32 disabled so 78.6 TFLOP/s is the "
@@ -550,6 +554,32 @@ def run_serving(duration, beta_acc):
             break
         stable = {"lambda": lam, "p99_ms": r["p99_ms"], "queries_per_s": r["queries_per_s"]}
     res["max_stable_lambda_beta_2"] = stable
+    cpu = None
+    try:  # the reference's CPU path on a sample of the same requests, same host
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["IGL_NUM_THREADS"] = str(threads)
+        if O.ref_available():
+            r2 = np.random.default_rng(9)
+            picks = r2.integers(0, len(meshes), size=8)
+            nq_tot, t_tot = 0, 0.0
+            for m in picks:
+                Vm, Fm = meshes[m]
+                ref = O.RefSoup(Vm, Fm)
+                nqs = int(np.exp(r2.uniform(np.log(1e3), np.log(1e6))))
+                Qm = batches[m][:nqs]
+                t = time.perf_counter()
+                ref.query(Qm, 2.0)
+                t_tot += time.perf_counter() - t
+                nq_tot += nqs
+            cpu = {"value": round(nq_tot / t_tot, 1), "unit": "queries/s", "cores": threads, "kind": "reference",
+                   "sample": f"8 requests of the trace's distribution ({nq_tot} queries), accuracy_scale 2, "
+                             "UT_SolidAngle + igl::parallel_for, trees prebuilt"}
+            log(f"[bench] serving CPU reference: {cpu}")
+    except Exception as e:
+        log(f"[bench] serving cpu baseline skipped: {e}")
+    res["cpu_baseline"] = cpu
     resident, held, budget = srv.memory()
     srv.close()
     res.update({"meshes": 64, "resident_triangles": int(ntris), "resident_bytes": int(resident), "slots": 8,
