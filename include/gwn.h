@@ -175,6 +175,50 @@ int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta,
 int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Point areas for a cloud without them: k nearest neighbours and a local
+ * Delaunay star (the pre-step of copyleft/cgal/fast_winding_number.cpp:32-43).
+ *
+ * gwn_knn replaces igl::knn(P,V,k,point_indices,CH,CN,W,I), knn.h:51-67
+ * (knn.cpp:24-127); igl::knn(P,k,point_indices,CH,CN,W,I) (knn.h:33-45) is
+ * V = P.  Row i of I lists the k candidates nearest to P.row(i), nearest
+ * first (FP64 distances; equal distances in ascending index order).  The
+ * candidates are cand[0 .. ncand) (indices into V): the first point of every
+ * non-empty leaf of the octree, the only points the reference's search
+ * reaches; cand = NULL makes every point of V a candidate.  When nv <= k the
+ * reference returns all of V sorted (knn.cpp:46-55) and so does this: I has
+ * gwn_knn_columns(nv, k) = min(nv, k) columns, row-major int32.
+ * GWN_ERR_INVALID when there are fewer than k candidates (where the
+ * reference's queue runs dry) or a coordinate is not finite. */
+int gwn_knn_columns(int64_t nv, int64_t k);
+int gwn_knn(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand, int64_t ncand,
+            int32_t* I);
+int gwn_knn_device(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
+                   int64_t ncand, int32_t* I, void* stream);
+/* gwn_point_areas replaces igl::copyleft::cgal::point_areas(P,I,N,A,T),
+ * point_areas.h:44-62 (point_areas.cpp:37-170): per point, the neighbours
+ * I.row(i) whose normal has a positive dot product with the first
+ * neighbour's, a least-squares plane through them (T = its unit normal, on
+ * the side of N.row(i)), and the area the reference sums over the Delaunay
+ * triangles of the projected neighbours at the first kept neighbour.  Fewer
+ * than 3 kept neighbours, or a non-finite sum, give A = 0 and (then) T = 0.
+ * The reference triangulates with CGAL; this builds the same star directly.
+ * N may be NULL (no normal filter, T unoriented); T may be NULL.  P, N n x 3,
+ * I n x k (indices into P), A n, T n x 3. */
+int gwn_point_areas(const double* P, int64_t n, const int32_t* I, int64_t k, const double* N, double* A,
+                    double* T);
+int gwn_point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k, const double* N, double* A,
+                           double* T, void* stream);
+/* gwn_cloud_build_estimated_areas is the build half of
+ * igl::copyleft::cgal::fast_winding_number(P,N,Q,expansion_order,beta,WN)
+ * (copyleft/cgal/fast_winding_number.h, fast_winding_number.cpp:16-43): the
+ * areas come from knn with k = 21 over the candidates of P's own octree (all
+ * points but exact copies of a lower-indexed one) and point areas, then the
+ * cloud is built as gwn_cloud_build builds it (FP32).  A (n doubles, may be
+ * NULL) receives the estimated areas.  Query it with gwn_cloud_query. */
+int gwn_cloud_build_estimated_areas(const double* P, const double* N, int64_t n, int order, double* A,
+                                    gwn_cloud** out);
+
+/* ------------------------------------------------------------------------
  * Signed distance (signed_distance.h:61-89, signed_distance.cpp:28-140):
  * closest point on the soup (FP64, from the vertices as given) and a sign
  * from the winding number.  Sign types carry the reference's enum values.

@@ -68,6 +68,12 @@ SIGNATURES = {
     "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
     "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
     "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),
+    "gwn_knn_columns": (C.c_int, [C.c_int64, C.c_int64]),
+    "gwn_knn": (C.c_int, [f64p, C.c_int64, f64p, C.c_int64, C.c_int64, vp, C.c_int64, i32p]),
+    "gwn_knn_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, C.c_int64, vp, C.c_int64, vp, vp]),
+    "gwn_point_areas": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, vp, f64p, vp]),
+    "gwn_point_areas_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, vp, vp, vp, vp]),
+    "gwn_cloud_build_estimated_areas": (C.c_int, [f64p, f64p, C.c_int64, C.c_int, vp, C.POINTER(vp)]),
     "gwn_signed_distance": (C.c_int, [vp, f64p, C.c_int64, C.c_int, C.c_double, C.c_double, f64p, i32p, f64p]),
     "gwn_signed_distance_device": (C.c_int, [vp, vp, C.c_int64, C.c_int, C.c_double, C.c_double, vp, vp, vp, vp]),
     "gwn_exact_winding_number_2d": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),

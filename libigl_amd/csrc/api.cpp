@@ -410,7 +410,132 @@ int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type,
   });
 }
 
+// ---------------- k nearest neighbours, point areas ----------------
+int gwn_knn_columns(int64_t nv, int64_t k) { return (k < 1 || nv < 0) ? 0 : knn_columns(nv, k); }
+
+int gwn_knn(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand, int64_t ncand,
+            int32_t* I) {
+  return guard([&] {
+    GWN_CHECK(np >= 0 && nv >= 0 && np < (int64_t)0x7FFFFFFF && nv < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID,
+              "bad point count");
+    GWN_CHECK(k >= 1 && k < (1 << 20), GWN_ERR_INVALID, "k must be in [1, 2^20)");
+    GWN_CHECK(!cand || (ncand >= 0 && ncand <= nv), GWN_ERR_INVALID, "bad candidate count");
+    if (np == 0 || nv == 0) return;
+    GWN_CHECK(P && V && I, GWN_ERR_INVALID, "null input");
+    if (cand)
+      for (int64_t j = 0; j < ncand; ++j)
+        GWN_CHECK(cand[j] >= 0 && cand[j] < nv, GWN_ERR_INVALID, "candidate index out of range");
+    ensure_device();
+    hipStream_t st = nullptr;
+    const int64_t kk = knn_columns(nv, k);
+    DevBuf<double> dP, dV;
+    DevBuf<int32_t> dc, dI((size_t)np * kk);
+    upload(dP, P, 3 * (size_t)np, st);
+    upload(dV, V, 3 * (size_t)nv, st);
+    if (cand) upload(dc, cand, (size_t)ncand, st);
+    knn_device(dP.p, np, dV.p, nv, k, cand ? dc.p : nullptr, ncand, dI.p, st);
+    GWN_HIP(hipMemcpyAsync(I, dI.p, sizeof(int32_t) * (size_t)np * kk, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_knn_device(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
+                   int64_t ncand, int32_t* I, void* stream) {
+  return guard([&] {
+    GWN_CHECK(np >= 0 && nv >= 0 && np < (int64_t)0x7FFFFFFF && nv < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID,
+              "bad point count");
+    GWN_CHECK(k >= 1 && k < (1 << 20), GWN_ERR_INVALID, "k must be in [1, 2^20)");
+    GWN_CHECK(!cand || (ncand >= 0 && ncand <= nv), GWN_ERR_INVALID, "bad candidate count");
+    if (np == 0 || nv == 0) return;
+    GWN_CHECK(P && V && I, GWN_ERR_INVALID, "null input");
+    ensure_device();
+    knn_device(P, np, V, nv, k, cand, ncand, I, static_cast<hipStream_t>(stream));
+  });
+}
+
+int gwn_point_areas(const double* P, int64_t n, const int32_t* I, int64_t k, const double* N, double* A,
+                    double* T) {
+  return guard([&] {
+    GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+    GWN_CHECK(k >= 1 && k < (1 << 20), GWN_ERR_INVALID, "I must have 1 to 2^20 columns");
+    if (n == 0) return;
+    GWN_CHECK(P && I && A, GWN_ERR_INVALID, "null input");
+    for (int64_t j = 0; j < n * k; ++j)
+      GWN_CHECK(I[j] >= 0 && I[j] < n, GWN_ERR_INVALID, "neighbour index out of range at I[" + std::to_string(j) + "]");
+    ensure_device();
+    hipStream_t st = nullptr;
+    DevBuf<double> dP, dN, dA(n), dT;
+    DevBuf<int32_t> dI;
+    upload(dP, P, 3 * (size_t)n, st);
+    upload(dI, I, (size_t)n * k, st);
+    if (N) upload(dN, N, 3 * (size_t)n, st);
+    if (T) dT.alloc(3 * (size_t)n);
+    point_areas_device(dP.p, n, dI.p, k, N ? dN.p : nullptr, dA.p, T ? dT.p : nullptr, st);
+    GWN_HIP(hipMemcpyAsync(A, dA.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (T) GWN_HIP(hipMemcpyAsync(T, dT.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k, const double* N, double* A,
+                           double* T, void* stream) {
+  return guard([&] {
+    GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+    GWN_CHECK(k >= 1 && k < (1 << 20), GWN_ERR_INVALID, "I must have 1 to 2^20 columns");
+    if (n == 0) return;
+    GWN_CHECK(P && I && A, GWN_ERR_INVALID, "null input");
+    ensure_device();
+    point_areas_device(P, n, I, k, N, A, T, static_cast<hipStream_t>(stream));
+  });
+}
+
 // ---------------- point cloud ----------------
+int gwn_cloud_build_estimated_areas(const double* P, const double* N, int64_t n, int order, double* A,
+                                    gwn_cloud** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
+    GWN_CHECK(n >= 0 && n < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
+    GWN_CHECK(n == 0 || (P && N), GWN_ERR_INVALID, "null input");
+    ensure_device();
+    hipStream_t st = nullptr;
+    gwn_cloud* c = new gwn_cloud();
+    try {
+      c->n = n;
+      c->order = order;
+      c->nterms = order == 0 ? 3 : (order == 1 ? 12 : 39);
+      c->P.alloc(3 * (size_t)(n > 0 ? n : 1));
+      c->N.alloc(3 * (size_t)(n > 0 ? n : 1));
+      c->A.alloc((size_t)(n > 0 ? n : 1));
+      if (n > 0) {
+        // copyleft/cgal/fast_winding_number.cpp:32-35: octree, knn with k = 21, point areas
+        constexpr int64_t kAreaNeighbours = 21;
+        DevBuf<double> dP, dN, dA(n);
+        upload(dP, P, 3 * (size_t)n, st);
+        upload(dN, N, 3 * (size_t)n, st);
+        DevBuf<int32_t> cand;
+        const int64_t nc = distinct_points_device(dP.p, n, cand, st);
+        const int64_t kk = knn_columns(n, kAreaNeighbours);
+        DevBuf<int32_t> dI((size_t)n * kk);
+        knn_device(dP.p, n, dP.p, n, kAreaNeighbours, cand.p, nc, dI.p, st);
+        point_areas_device(dP.p, n, dI.p, kk, dN.p, dA.p, nullptr, st);
+        to_float_device(dP.p, 3 * n, c->P.p, st);
+        to_float_device(dN.p, 3 * n, c->N.p, st);
+        to_float_device(dA.p, n, c->A.p, st);
+        if (A) GWN_HIP(hipMemcpyAsync(A, dA.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        GWN_HIP(hipStreamSynchronize(st));
+      }
+      cloud_build_device(c, st, true);
+      GWN_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
 int gwn_cloud_build(const float* P, const float* N, const float* A, int64_t n, int order, gwn_cloud** out) {
   return guard([&] {
     GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");

@@ -145,6 +145,14 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* 
 int64_t lbvh_build(const float* tri_boxes_dev, int64_t nf, const float* bbox_host, DevBuf<uint32_t>& nodes,
                    hipStream_t st);
 
+// Nodes of a 4-wide tree (reference encoding) grouped by depth: order[]
+// lists the nodes of depth d at offset[d] .. offset[d+1].
+struct Levels {
+  std::vector<uint32_t> offset;
+  DevBuf<int32_t> order;
+  int maxdepth = 0;
+};
+void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st);
 // soup queries (soup.hip)
 size_t soup_query_work_bytes(int64_t nq);
 void soup_ensure_fp64(gwn_soup* s, hipStream_t st);
@@ -154,6 +162,14 @@ void read_triangle_mesh(const char* path, std::vector<double>& V, std::vector<in
 void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
                             double* S, int32_t* I, double* C, hipStream_t st);
+// k nearest neighbours and point areas (knn.hip)
+int knn_columns(int64_t nv, int64_t k);
+void knn_device(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
+                int64_t ncand, int32_t* I, hipStream_t st);
+void point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k, const double* N, double* A,
+                        double* T, hipStream_t st);
+int64_t distinct_points_device(const double* V, int64_t n, DevBuf<int32_t>& cand, hipStream_t st);
+void to_float_device(const double* x, int64_t n, float* y, hipStream_t st);
 // traversal kernel: WALK_WAVE (wave-coherent, batch queries) or WALK_LANE
 // (per-lane, used by the serving engine); bitwise the same results
 constexpr int WALK_WAVE = 0, WALK_LANE = 1;

@@ -722,13 +722,7 @@ __global__ void k_morton_q(const float* __restrict__ Q, int64_t nq, float lx, fl
 
 // ---------------------------------------------------------------------------
 // host orchestration
-struct Levels {
-  std::vector<uint32_t> offset;  // per depth: start in order[]
-  DevBuf<int32_t> order;
-  int maxdepth = 0;
-};
-
-static void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st) {
+void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st) {
   DevBuf<int32_t> parent(nn), depth(nn);
   DevBuf<uint32_t> hist(4097);
   DevBuf<int32_t> maxd(1);

@@ -219,6 +219,28 @@ class PointCloudFWN:
         self.user_octree = True
         return self
 
+    @classmethod
+    def with_estimated_areas(cls, P, N, order=2):
+        """The cloud of igl::copyleft::cgal::fast_winding_number(P,N,Q,...)
+        (copyleft/cgal/fast_winding_number.cpp:16-43): areas from knn (k = 21)
+        and point_areas, computed on the GPU; `self.A` holds them."""
+        P = _rows3(P, "P", np.float64)
+        N = _rows3(N, "N", np.float64)
+        if P.shape[0] != N.shape[0]:
+            raise ValueError("P and N sizes differ")
+        if order not in (0, 1, 2):
+            raise ValueError("expansion_order must be 0, 1 or 2")
+        A = np.zeros(P.shape[0], np.float64)
+        self = cls.__new__(cls)
+        h = nat.vp()
+        check(lib().gwn_cloud_build_estimated_areas(P, N, P.shape[0], order, A.ctypes.data, C_byref(h)))
+        self._h = h.value
+        self.order = order
+        self.n = P.shape[0]
+        self.user_octree = False
+        self.A = A
+        return self
+
     def __del__(self):
         try:
             if getattr(self, "_h", None) is not None:
@@ -268,6 +290,72 @@ def octree(P):
     CH, CN, W, cs, cc, pl = c.octree()
     point_indices = [np.sort(pl[cs[i]:cs[i] + cc[i]]).tolist() for i in range(CH.shape[0])]
     return point_indices, CH, CN, W
+
+
+def octree_candidates(point_indices, CH):
+    """The points igl::knn can return over an octree (knn.cpp:101-123): the
+    first point of every non-empty leaf reachable from cell 0."""
+    CH = np.asarray(CH).reshape(-1, 8)
+    out = []
+    if CH.shape[0] == 0:
+        return np.zeros(0, np.int32)
+    stack = [0]
+    while stack:
+        c = stack.pop()
+        if CH[c, 0] == -1:
+            if len(point_indices[c]) > 0:
+                out.append(int(point_indices[c][0]))
+        else:
+            stack.extend(int(x) for x in CH[c])
+    return np.asarray(sorted(out), np.int32)
+
+
+def knn(P, V, k, point_indices=None, CH=None):
+    """igl::knn(P,V,k,point_indices,CH,CN,W,I) (knn.h:51-67): I, np x min(nv,k),
+    the k points of V nearest to each point of P, nearest first (equal
+    distances: lower index first).  With an octree (point_indices, CH, as
+    igl::octree returns them over V) the candidates are the points its search
+    can reach; without one, every point of V.  knn(P,k,...) is knn(P,P,k,...)."""
+    P = _rows3(P, "P", np.float64)
+    V = _rows3(V, "V", np.float64)
+    k = int(k)
+    if k < 1:
+        raise ValueError("k must be positive")
+    cols = lib().gwn_knn_columns(V.shape[0], k)
+    I = np.zeros((P.shape[0], cols), np.int32)
+    cand, nc = None, 0
+    if point_indices is not None:
+        c = octree_candidates(point_indices, CH)
+        cand, nc = c, c.shape[0]
+    keep = cand
+    check(lib().gwn_knn(P, P.shape[0], V, V.shape[0], k, None if cand is None else cand.ctypes.data, nc, I))
+    del keep
+    return I
+
+
+def point_areas(P, I, N=None):
+    """igl::copyleft::cgal::point_areas(P,I,N,A,T) (point_areas.h:44-62) on the
+    GPU: returns (A, T)."""
+    P = _rows3(P, "P", np.float64)
+    I = np.ascontiguousarray(np.asarray(I), dtype=np.int32)
+    if I.ndim != 2 or I.shape[0] != P.shape[0]:
+        raise ValueError("I must have one row per point")
+    Nn = None if N is None else _rows3(N, "N", np.float64)
+    if Nn is not None and Nn.shape[0] != P.shape[0]:
+        raise ValueError("P and N sizes differ")
+    A = np.zeros(P.shape[0], np.float64)
+    T = np.zeros((P.shape[0], 3), np.float64)
+    if P.shape[0] and I.shape[1]:
+        check(lib().gwn_point_areas(P, P.shape[0], I, I.shape[1], None if Nn is None else Nn.ctypes.data, A,
+                                    T.ctypes.data))
+    return A, T
+
+
+def fast_winding_number_estimated_areas(P, N, Q, expansion_order=2, beta=2.0):
+    """igl::copyleft::cgal::fast_winding_number(P,N,Q,expansion_order,beta,WN)
+    (copyleft/cgal/fast_winding_number.h): a cloud with normals but no areas;
+    the areas are estimated on the GPU (knn + point_areas)."""
+    return PointCloudFWN.with_estimated_areas(P, N, expansion_order).query(Q, beta)
 
 
 def fast_winding_number(*args, **kw):

@@ -23,6 +23,16 @@
  *       triangle, a dense barycentric sampling bound, the single-tet golden
  *       value of tests/include/igl/signed_distance.cpp:4-44 (sqrt(1/12)) and
  *       sign checks on closed meshes.
+ *   oracle_knn / oracle_knn_bf
+ *       the reference's golden vectors (tests/include/igl/knn.cpp: cube corners
+ *       with a tie, and nv <= k), each against the other (best-first octree
+ *       search vs brute force over the octree's candidates) on random and
+ *       clustered sets, and coincident points (only the first is reachable).
+ *   oracle_py.point_areas (Python: numpy SVD + scipy Delaunay)
+ *       closed forms: the Voronoi area of a lattice point ((sqrt 3 / 2) a^2 on
+ *       a triangular lattice, scaled by the stretch of a stretched one), the
+ *       plane normal of planar neighbourhoods, the normal filter, and the
+ *       degenerate cases (<= 2 neighbours, collinear neighbours: 0).
  *   oracle_octree / oracle_cloud_*
  *       the reference golden values of tests/include/igl/fast_winding_number.cpp:46-52
  *       (one-point cloud, 1e-15), the beta<=0 brute-force branch
@@ -131,6 +141,16 @@ void oracle_cloud_eval_f32(const float* P, const float* N, const float* A,
                            const oracle_octree_t* t, int order,
                            const float* CM, const float* R, const float* EC,
                            const float* Q, int64_t nq, double beta, float* WN, int nthreads);
+
+/* ---- k nearest neighbours (knn.cpp:24-127) ----
+ * oracle_knn: the reference's best-first search over an FP64 octree of V
+ * (oracle_octree_f64); I np x min(nv, k).  Returns -1 when fewer than k points
+ * are reachable.  oracle_knn_bf: brute force over cand[0 .. ncand) (NULL: all
+ * of V), ordered by (squared distance, index). */
+int oracle_knn(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const oracle_octree_t* t,
+               int32_t* I, int nthreads);
+int oracle_knn_bf(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
+                  int64_t ncand, int32_t* I, int nthreads);
 
 #ifdef __cplusplus
 }

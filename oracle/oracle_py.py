@@ -66,6 +66,9 @@ def lib():
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        L.oracle_knn.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int64, C.POINTER(OctreeT), _i32p, C.c_int]
+        L.oracle_knn_bf.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, _i32p,
+                                    C.c_int]
         L.oracle_winding_number_2d.argtypes = [_f64p, _i32p, C.c_int64, _f64p, C.c_int64, _f64p]
         L.oracle_point_tri_sqrdist.restype = C.c_double
         L.oracle_point_tri_sqrdist.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p]
@@ -347,3 +350,126 @@ class Cloud:
                                                         self.CM, self.R, self.EC, Q, Q.shape[0], float(beta), W,
                                                         nthreads)
         return W
+
+
+# ---------------- knn, point areas ----------------
+def knn(P, V, k, octree=None, nthreads=0):
+    """knn.cpp:24-127 restated: best-first search over `octree` (an Octree of V
+    in FP64; built here when None).  I is np x min(nv, k)."""
+    P = _c(P, np.float64).reshape(-1, 3)
+    V = _c(V, np.float64).reshape(-1, 3)
+    cols = min(V.shape[0], k)
+    I = np.zeros((P.shape[0], cols), np.int32)
+    if P.shape[0] == 0 or V.shape[0] == 0:
+        return I
+    if octree is None:
+        octree = Octree(V, np.float64)
+    rc = lib().oracle_knn(P, P.shape[0], V, V.shape[0], k, C.byref(octree.t), I, nthreads)
+    if rc != 0:
+        raise ValueError("oracle knn: fewer reachable points than k")
+    return I
+
+
+def knn_bf(P, V, k, cand=None, nthreads=0):
+    """Brute-force k nearest over cand (indices into V; None: all), ordered by
+    (squared distance, index)."""
+    P = _c(P, np.float64).reshape(-1, 3)
+    V = _c(V, np.float64).reshape(-1, 3)
+    cols = min(V.shape[0], k)
+    I = np.zeros((P.shape[0], cols), np.int32)
+    if P.shape[0] == 0 or V.shape[0] == 0:
+        return I
+    c = None if cand is None else _c(cand, np.int32)
+    rc = lib().oracle_knn_bf(P, P.shape[0], V, V.shape[0], k, None if c is None else c.ctypes.data,
+                             0 if c is None else c.shape[0], I, nthreads)
+    if rc != 0:
+        raise ValueError("oracle knn_bf: fewer candidates than k")
+    return I
+
+
+def octree_candidates(octree):
+    """The first point of every non-empty leaf reachable from cell 0
+    (knn.cpp:113-117): the points igl::knn can return."""
+    CH = octree.CH
+    out, stack = [], [0]
+    while stack:
+        c = stack.pop()
+        if CH[c, 0] == -1:
+            if octree.cell_count[c] > 0:
+                out.append(int(octree.point_list[octree.cell_start[c]]))
+        else:
+            stack.extend(int(x) for x in CH[c])
+    return np.asarray(sorted(out), np.int32)
+
+
+def _star_term(X, Y, Z):
+    """point_areas.cpp:121-155 for the triangle (X, Y, Z) at X."""
+    x = np.sqrt(np.sum((Y - Z) ** 2))
+    y = np.sqrt(np.sum((X - Z) ** 2))
+    z = np.sqrt(np.sum((Y - X) ** 2))
+    cosX = (z * z + y * y - x * x) / (2 * y * z)
+    cosY = (z * z + x * x - y * y) / (2 * x * z)
+    cosZ = (x * x + y * y - z * z) / (2 * y * x)
+    bary = np.array([x * cosX, y * cosY, z * cosZ])
+    bary /= bary.sum()
+    full = 0.25 * np.sqrt((x + (y + z)) * (z - (x - y)) * (z + (x - y)) * (x + (y - z)))
+    part = bary * full
+    if cosX < 0:
+        return 0.5 * full
+    if cosY < 0 or cosZ < 0:
+        return 0.25 * full
+    return (part[1] + part[2]) / 2
+
+
+def point_areas(P, I, N=None):
+    """point_areas.cpp:37-170 restated per point: the normal filter, the PCA
+    plane (numpy's SVD in place of Eigen's JacobiSVD), a Delaunay triangulation
+    of the projected neighbours (scipy / Qhull in place of CGAL; faces taken
+    counter-clockwise as CGAL gives them) and the area sum over the faces at
+    vertex 0.  Returns (A, T)."""
+    import warnings
+    from scipy.spatial import Delaunay, QhullError
+    with np.errstate(all="ignore"):  # degenerate faces give NaN sums, which count as 0
+        return _point_areas(P, I, N, Delaunay, QhullError, warnings)
+
+
+def _point_areas(P, I, N, Delaunay, QhullError, warnings):
+    P = np.asarray(P, np.float64).reshape(-1, 3)
+    I = np.asarray(I)
+    n = P.shape[0]
+    A = np.zeros(n)
+    T = np.zeros((n, 3))
+    for i in range(n):
+        nb = P[I[i]]
+        if N is not None and nb.shape[0] > 1:
+            nn = np.asarray(N, np.float64)[I[i]]
+            n0 = nn[0]
+            dot = n0[0] * nn[:, 0] + n0[1] * nn[:, 1] + n0[2] * nn[:, 2]
+            nb = nb[dot > 0]
+        if nb.shape[0] <= 2:
+            continue
+        X = nb - nb.mean(axis=0)
+        U, S, Vt = np.linalg.svd(X, full_matrices=False)
+        t = Vt[2].copy()
+        if N is not None and t @ np.asarray(N, np.float64)[i] < 0:
+            t = -t
+        T[i] = t
+        plane = (U * S)[:, :2]
+        try:
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                tri = Delaunay(plane)
+        except (QhullError, ValueError):
+            continue  # collinear: no faces
+        acc = 0.0
+        for f in tri.simplices:
+            a, b, c = plane[f[0]], plane[f[1]], plane[f[2]]
+            if (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]) < 0:
+                f = f[[0, 2, 1]]
+            w = np.nonzero(f == 0)[0]
+            if w.size == 0:
+                continue
+            X0 = int(w[0])
+            acc += _star_term(plane[f[X0]], plane[f[(X0 + 1) % 3]], plane[f[(X0 + 2) % 3]])
+        A[i] = acc if np.isfinite(acc) else 0.0
+    return A, T
