@@ -270,7 +270,8 @@ def main():
 
     others = {}
     if not args.no_rows:
-        for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance)):
+        for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance),
+                         ("point_areas", run_point_areas)):
             try:
                 others[name] = fn(args.no_cpu)
             except Exception as e:
@@ -512,6 +513,68 @@ def run_signed_distance(no_cpu):
         So, _, _ = O.signed_distance(V, F, Q[sub], gwn.SIGN_FAST_WINDING_NUMBER)
         out["max_rel_dS_vs_oracle"] = float(np.max(np.abs(S[sub] - So) / np.maximum(np.abs(So), 1e-12)))
     log(f"[bench] signed distance: {out}")
+    return out
+
+
+def run_point_areas(no_cpu):
+    """Areas for a cloud that has none (copyleft/cgal/fast_winding_number.cpp:
+    32-43) on the 2M oriented points of the point-cloud workload: knn with
+    k = 21 over all points, point areas, and the whole estimated-area cloud
+    build (coincident-point filter, knn, areas, octree, moments)."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import _native as nat
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+
+    P, N, _, _ = D.cloud_workload()
+    n, k = P.shape[0], 21
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    dP = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    dN = torch.from_numpy(np.ascontiguousarray(N)).to(dev)
+    dI = torch.empty((n, k), dtype=torch.int32, device=dev)
+    dA = torch.empty(n, dtype=torch.float64, device=dev)
+    dT = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    dt_knn, kms_knn = _timed_device(lambda: nat.check(nat.lib().gwn_knn_device(
+        dP.data_ptr(), n, dP.data_ptr(), n, k, None, 0, dI.data_ptr(), st)), 3, "knn")
+    dt_pa, kms_pa = _timed_device(lambda: nat.check(nat.lib().gwn_point_areas_device(
+        dP.data_ptr(), n, dI.data_ptr(), k, dN.data_ptr(), dA.data_ptr(), dT.data_ptr(), st)), 3, "point_areas")
+    tb = time.perf_counter()
+    c = gwn.PointCloudFWN.with_estimated_areas(P, N, 2)
+    build_s = time.perf_counter() - tb
+    I, A = dI.cpu().numpy(), dA.cpu().numpy()
+    out = {"workload": "2M oriented surface points (noisy level-5 sphere), k = 21",
+           "knn_points_per_s": round(n / dt_knn, 1), "knn_kernel_ms": round(kms_knn, 4),
+           "point_areas_points_per_s": round(n / dt_pa, 1), "point_areas_kernel_ms": round(kms_pa, 4),
+           "estimated_area_cloud_build_s": round(build_s, 4),
+           "area_sum_over_surface_area": None}
+    Vs, Fs = D.noisy_sphere(5, 0.01, 11)
+    out["area_sum_over_surface_area"] = round(float(A.sum() / D.face_normals(Vs, Fs)[1].sum()), 6)
+    assert np.array_equal(c.A, A)
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        threads = min(16, os.cpu_count() or 1)
+        tree = O.Octree(P, np.float64)
+        sub = np.arange(0, n, 50)
+        t = time.perf_counter()
+        Io = O.knn(P[sub], P, k, tree, threads)
+        cdt = time.perf_counter() - t
+        out["knn_matches_oracle"] = bool(np.array_equal(I[sub], Io))
+        out["cpu_baseline"] = {"value": round(sub.size / cdt, 1), "unit": "points/s (knn, k = 21)",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{sub.size} queries, best-first octree search restated (knn.cpp:72-127)"}
+        rows = sub[:400]
+        t = time.perf_counter()
+        Ao, _ = O.point_areas(P, I, N, rows=rows)
+        adt = time.perf_counter() - t
+        out["point_areas_max_rel_vs_oracle"] = float(np.max(np.abs(A[rows] - Ao[rows]) / np.maximum(Ao[rows], 1e-300)))
+        out["cpu_baseline_point_areas"] = {"value": round(rows.size / adt, 1), "unit": "points/s", "cores": 1,
+                                           "kind": "port",
+                                           "sample": f"{rows.size} points, numpy SVD + Qhull Delaunay restatement"}
+    log(f"[bench] point areas: {out}")
     return out
 
 

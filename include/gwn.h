@@ -75,7 +75,8 @@ const char* gwn_version(void);
 
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
- * "soup_traverse64", "exact", "cloud_traverse", "radix_sort") with HIP events
+ * "soup_traverse64", "exact", "cloud_traverse", "radix_sort", "closest_point",
+ * "knn", "point_areas", "cloud_brute", "exact2d") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises
  * on those events and returns the accumulated time and launch count. */
 int gwn_profile_enable(int on);

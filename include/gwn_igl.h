@@ -16,6 +16,11 @@
 //   igl::fast_winding_number(P,N,A,Q,WN)              fast_winding_number.h:126-131
 //   struct igl::FastWindingNumberBVH                  fast_winding_number.h:140-152
 //   igl::read_triangle_mesh(str,V,F)                   read_triangle_mesh.h (.off / .obj)
+//   igl::knn(P,k,point_indices,CH,CN,W,I)             knn.h:33-45
+//   igl::knn(P,V,k,point_indices,CH,CN,W,I)           knn.h:51-67
+//   igl::copyleft::cgal::point_areas(P,I,N,A[,T])     copyleft/cgal/point_areas.h:44-62
+//   igl::copyleft::cgal::fast_winding_number(P,N,Q[,expansion_order,beta],WN)
+//                                                     copyleft/cgal/fast_winding_number.h:35-53
 //   igl::SignedDistanceType                           signed_distance.h:20-34
 //   igl::signed_distance(P,V,F,type,lower,upper,S,I,C,N)  signed_distance.h:61-71
 //   igl::signed_distance(P,V,F,type,S,I,C,N)          signed_distance.h:81-89
@@ -335,6 +340,94 @@ void signed_distance(const MP& P, const MV& V, const MF& F, int type, double low
   signed_distance_bvh(bvh.handle, P, type, lower, upper, S, I, C);
 }
 
+// The points igl::knn can return over an octree (knn.cpp:101-123): the first
+// point of every non-empty leaf reachable from cell 0.
+template <typename Index, typename MCH>
+std::vector<int32_t> octree_candidates(const std::vector<std::vector<Index>>& point_indices, const MCH& CH) {
+  std::vector<int32_t> out;
+  if (CH.rows() == 0) return out;
+  if (CH.cols() != 8) throw std::runtime_error("CH should have 8 columns");
+  if ((int64_t)point_indices.size() != (int64_t)CH.rows())
+    throw std::runtime_error("point_indices and CH disagree on the number of cells");
+  std::vector<int64_t> stack(1, 0);
+  std::vector<char> seen((size_t)CH.rows(), 0);
+  while (!stack.empty()) {
+    const int64_t c = stack.back();
+    stack.pop_back();
+    if (c < 0 || c >= (int64_t)CH.rows() || seen[c]) throw std::runtime_error("CH is not a tree");
+    seen[c] = 1;
+    if (CH(c, 0) == -1) {
+      if (!point_indices[c].empty()) out.push_back(static_cast<int32_t>(point_indices[c][0]));
+    } else {
+      for (int j = 0; j < 8; ++j) stack.push_back(static_cast<int64_t>(CH(c, j)));
+    }
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+// knn.cpp:24-127: I (np x min(nv, k)) over the candidates of the caller's octree
+template <typename MP, typename MV, typename Index, typename MCH, typename MI>
+void knn(const MP& P, const MV& V, size_t k, const std::vector<std::vector<Index>>& point_indices, const MCH& CH,
+         MI& I) {
+  const auto p = rows3<double>(P, "P");
+  const auto v = rows3<double>(V, "V");
+  const int64_t np = P.rows(), nv = V.rows();
+  const int cols = gwn_knn_columns(nv, (int64_t)k);
+  std::vector<int32_t> out((size_t)np * cols);
+  if (np > 0 && nv > 0) {
+    const auto cand = octree_candidates(point_indices, CH);
+    check(gwn_knn(p.data(), np, v.data(), nv, (int64_t)k, cand.data(), (int64_t)cand.size(), out.data()));
+  }
+  I.resize(np, cols);
+  for (int64_t i = 0; i < np; ++i)
+    for (int j = 0; j < cols; ++j)
+      I(i, j) = static_cast<typename std::decay<decltype(I(0, 0))>::type>(out[(size_t)i * cols + j]);
+}
+
+// point_areas.cpp:37-170: A n x 1, T n x 3 (zero where no area is found)
+template <typename MP, typename MI, typename MN, typename MA, typename MT>
+void point_areas(const MP& P, const MI& I, const MN& N, MA& A, MT& T) {
+  const auto p = rows3<double>(P, "P");
+  const int64_t n = P.rows(), k = I.cols();
+  if (I.rows() != n) throw std::runtime_error("P and I must have the same number of rows");
+  std::vector<double> nn;
+  if (N.rows() > 0) {
+    if (N.rows() != n) throw std::runtime_error("P and N must have the same number of rows");
+    nn = rows3<double>(N, "N");
+  }
+  std::vector<int32_t> idx((size_t)n * k);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < k; ++j) idx[(size_t)i * k + j] = static_cast<int32_t>(I(i, j));
+  std::vector<double> a((size_t)n, 0.0), t(3 * (size_t)n, 0.0);
+  if (n > 0 && k > 0)
+    check(gwn_point_areas(p.data(), n, idx.data(), k, nn.empty() ? nullptr : nn.data(), a.data(), t.data()));
+  A.resize(n, 1);
+  T.resize(n, 3);
+  for (int64_t i = 0; i < n; ++i) {
+    A(i, 0) = static_cast<typename std::decay<decltype(A(0, 0))>::type>(a[i]);
+    for (int j = 0; j < 3; ++j) T(i, j) = static_cast<typename std::decay<decltype(T(0, 0))>::type>(t[3 * i + j]);
+  }
+}
+
+// copyleft/cgal/fast_winding_number.cpp:16-43: a cloud without areas
+template <typename MP, typename MN, typename MQ, typename MW>
+void cloud_estimated_areas(const MP& P, const MN& N, const MQ& Q, int order, double beta, MW& WN) {
+  const auto p = rows3<double>(P, "P");
+  const auto n = rows3<double>(N, "N");
+  if (N.rows() != P.rows()) throw std::runtime_error("P and N must have the same number of rows");
+  const auto q = rows3<float>(Q, "Q");
+  gwn_cloud* c = nullptr;
+  check(gwn_cloud_build_estimated_areas(p.data(), n.data(), P.rows(), order, nullptr, &c));
+  std::vector<float> w((size_t)Q.rows());
+  const int rc = Q.rows() > 0 ? gwn_cloud_query(c, q.data(), Q.rows(), beta, w.data()) : GWN_OK;
+  gwn_cloud_free(c);
+  check(rc);
+  WN.resize(Q.rows(), 1);
+  for (int64_t i = 0; i < (int64_t)Q.rows(); ++i)
+    WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
+}
+
 // .off / .obj file; returns false (and leaves gwn_last_error) on failure, as
 // the reference returns false
 template <typename MV, typename MF>
@@ -491,6 +584,61 @@ void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::Matr
                          Eigen::PlainObjectBase<DerivedWN>& WN) {
   gwn_detail::cloud(P.derived(), N.derived(), A.derived(), Q.derived(), 2, 2.0, WN.derived());
 }
+
+template <typename DerivedP, typename IndexType, typename DerivedCH, typename DerivedCN, typename DerivedW,
+          typename DerivedI>
+void knn(const Eigen::MatrixBase<DerivedP>& P, size_t k, const std::vector<std::vector<IndexType>>& point_indices,
+         const Eigen::MatrixBase<DerivedCH>& CH, const Eigen::MatrixBase<DerivedCN>& CN,
+         const Eigen::MatrixBase<DerivedW>& W, Eigen::PlainObjectBase<DerivedI>& I) {
+  (void)CN;
+  (void)W;
+  gwn_detail::knn(P.derived(), P.derived(), k, point_indices, CH.derived(), I.derived());
+}
+
+template <typename DerivedP, typename DerivedV, typename IndexType, typename DerivedCH, typename DerivedCN,
+          typename DerivedW, typename DerivedI>
+void knn(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>& V, size_t k,
+         const std::vector<std::vector<IndexType>>& point_indices, const Eigen::MatrixBase<DerivedCH>& CH,
+         const Eigen::MatrixBase<DerivedCN>& CN, const Eigen::MatrixBase<DerivedW>& W,
+         Eigen::PlainObjectBase<DerivedI>& I) {
+  (void)CN;  // the candidates and distances fix the answer; the cells' boxes only prune
+  (void)W;
+  gwn_detail::knn(P.derived(), V.derived(), k, point_indices, CH.derived(), I.derived());
+}
+
+namespace copyleft {
+namespace cgal {
+
+template <typename DerivedP, typename DerivedI, typename DerivedN, typename DerivedA, typename DerivedT>
+void point_areas(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedI>& I,
+                 const Eigen::MatrixBase<DerivedN>& N, Eigen::PlainObjectBase<DerivedA>& A,
+                 Eigen::PlainObjectBase<DerivedT>& T) {
+  gwn_detail::point_areas(P.derived(), I.derived(), N.derived(), A.derived(), T.derived());
+}
+
+template <typename DerivedP, typename DerivedI, typename DerivedN, typename DerivedA>
+void point_areas(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedI>& I,
+                 const Eigen::MatrixBase<DerivedN>& N, Eigen::PlainObjectBase<DerivedA>& A) {
+  Eigen::MatrixXd T;
+  gwn_detail::point_areas(P.derived(), I.derived(), N.derived(), A.derived(), T);
+}
+
+template <typename DerivedP, typename DerivedN, typename DerivedQ, typename BetaType, typename DerivedWN>
+void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedN>& N,
+                         const Eigen::MatrixBase<DerivedQ>& Q, const int expansion_order, const BetaType beta,
+                         Eigen::PlainObjectBase<DerivedWN>& WN) {
+  gwn_detail::cloud_estimated_areas(P.derived(), N.derived(), Q.derived(), expansion_order, (double)beta,
+                                    WN.derived());
+}
+
+template <typename DerivedP, typename DerivedN, typename DerivedQ, typename DerivedWN>
+void fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedN>& N,
+                         const Eigen::MatrixBase<DerivedQ>& Q, Eigen::PlainObjectBase<DerivedWN>& WN) {
+  gwn_detail::cloud_estimated_areas(P.derived(), N.derived(), Q.derived(), 2, 2.0, WN.derived());
+}
+
+}  // namespace cgal
+}  // namespace copyleft
 
 }  // namespace igl
 #endif

@@ -13,12 +13,14 @@
 //          of slot records: the slot's box (FP64 extents rounded outward to
 //          float, so a box distance never exceeds the distance to a point in
 //          it) and a skip link, or the bucket for a leaf slot
-//   query  queries in Morton order.  A lane seeds its list with the bucket a
-//          greedy nearest-box descent reaches plus k points either side of it
-//          along the curve, so its bound is finite from the start.  Then the
-//          wave walks the rope coherently (one shared record index; each lane
-//          follows its own depth-first branch and bound) and a lane enters a
-//          slot whose box is not farther than its current k-th distance.
+//   query  queries in Morton order.  A lane seeds its list with the k + 8
+//          candidates either side of its own place on the curve (a binary
+//          search of its code in the candidates' codes), so its bound is
+//          finite and close from the start.  Then the wave walks the rope
+//          coherently (one shared record index; each lane follows its own
+//          depth-first branch and bound) and a lane enters a slot whose box is
+//          not farther than its current k-th distance.  Candidates that beat
+//          the k-th are parked and inserted in rounds (see Parked).
 //          Candidates are ordered by (squared distance, index): equal
 //          distances go to the lower index, so the result depends neither on
 //          the visit order nor on the other queries of the wave.
@@ -329,12 +331,53 @@ __device__ __forceinline__ void kn_consider(List& L, const double* q, const doub
   if (L.better(d2, ci)) L.insert(d2, ci);
 }
 
-// Query t of the Morton-ordered batch.  KM > 0: list in registers; KM == 0:
-// list in device memory at md/mi (lane stride nq_chunk).
+// Candidates that beat a lane's k-th are parked here and inserted in rounds:
+// a wave then runs the sorted insert about once per round for all its lanes
+// instead of once for every candidate that any one lane takes.
+constexpr int KN_PARK = 4;
+struct Parked {
+  double d[KN_PARK];
+  int32_t id[KN_PARK];
+  int n = 0;
+  __device__ __forceinline__ void put(double c, int32_t ci) {
+#pragma unroll
+    for (int t = 0; t < KN_PARK; ++t)
+      if (t == n) {
+        d[t] = c;
+        id[t] = ci;
+      }
+    ++n;
+  }
+  template <typename List>
+  __device__ __forceinline__ void flush(List& L) {
+#pragma unroll
+    for (int t = 0; t < KN_PARK; ++t)
+      if (t < n && L.better(d[t], id[t])) L.insert(d[t], id[t]);
+    n = 0;
+  }
+};
+
+// First position in the sorted codes not below `code` (nc if none).
+__device__ __forceinline__ int64_t kn_lower_bound(const uint32_t* __restrict__ codes, int64_t nc, uint32_t code) {
+  int64_t lo = 0, hi = nc;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (codes[mid] < code)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Query t of the Morton-ordered batch (qcodes: its Morton code in the tree's
+// box).  KM > 0: list in registers; KM == 0: list in device memory at md/mi
+// (lane stride nq).
 template <int KM>
 __global__ __launch_bounds__(256) void k_knn(const float4* __restrict__ rec, uint32_t nrec,
-                                             const double4* __restrict__ pts, int64_t nc, const double* __restrict__ Q,
-                                             const uint32_t* __restrict__ perm, int64_t q0, int64_t nq, int k,
+                                             const double4* __restrict__ pts, const uint32_t* __restrict__ ccodes,
+                                             int64_t nc, const double* __restrict__ Q, const uint32_t* __restrict__ perm,
+                                             const uint32_t* __restrict__ qcodes, int64_t q0, int64_t nq, int k,
                                              double* __restrict__ md, int32_t* __restrict__ mi,
                                              int32_t* __restrict__ I) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
@@ -348,37 +391,14 @@ __global__ __launch_bounds__(256) void k_knn(const float4* __restrict__ rec, uin
     L.stride = nq;
   }
   L.init(k, t);
-  // seed: greedy nearest-box descent to a bucket, then that bucket and k
-  // candidates either side of it along the curve
-  int64_t b = 0;
-  {
-    uint32_t lo = 0, hi = nrec;
-    for (int depth = 0; depth < 256 && lo < hi; ++depth) {
-      uint32_t pick = 0xFFFFFFFFu, plink = 0;
-      double pd = INFINITY;
-      for (uint32_t s = lo; s < hi;) {
-        const float4 blo = rec[2 * (size_t)s], bhi = rec[2 * (size_t)s + 1];
-        const uint32_t l = __float_as_uint(blo.w);
-        const double d = kn_box_d2(q, blo, bhi);
-        if (pick == 0xFFFFFFFFu || d < pd) {
-          pd = d;
-          pick = s;
-          plink = l;
-        }
-        s = (l & KN_LEAF) ? s + 1 : l;
-      }
-      if (plink & KN_LEAF) {
-        b = plink & ~KN_LEAF;
-        break;
-      }
-      lo = pick + 1;
-      hi = plink;
-    }
-  }
-  const int64_t w0 = b * KNN_BUCKET - k > 0 ? b * KNN_BUCKET - k : 0;
-  const int64_t w1 = (b + 1) * KNN_BUCKET + k < nc ? (b + 1) * KNN_BUCKET + k : nc;
+  // seed: the candidates around the query's own place on the Morton curve
+  const int64_t c0 = kn_lower_bound(ccodes, nc, qcodes[q0 + t]);
+  const int64_t half = k + KNN_BUCKET / 2;
+  const int64_t w0 = c0 - half > 0 ? c0 - half : 0;
+  const int64_t w1 = c0 + half < nc ? c0 + half : nc;
   for (int64_t s = w0; s < w1; ++s) kn_consider(L, q, pts[s]);
   // depth-first branch and bound over the rope, wave-coherent
+  Parked park;
   uint32_t skip = 0, i = 0;
   while (i < nrec) {
     const float4 lo = rec[2 * (size_t)i], hi = rec[2 * (size_t)i + 1];
@@ -391,10 +411,18 @@ __global__ __launch_bounds__(256) void k_knn(const float4* __restrict__ rec, uin
       } else if (leaf) {
         const int64_t s0 = (int64_t)(link & ~KN_LEAF) * KNN_BUCKET;
         const int64_t s1 = s0 + KNN_BUCKET < nc ? s0 + KNN_BUCKET : nc;
-        for (int64_t s = s0; s < s1; ++s)
-          if (s < w0 || s >= w1) kn_consider(L, q, pts[s]);
+        for (int64_t s = s0; s < s1; ++s) {
+          if (s < w0 || s >= w1) {
+            const double4 p = pts[s];
+            const double d2 = kn_d2(q, p.x, p.y, p.z);
+            const int32_t ci = (int32_t)p.w;
+            if (L.better(d2, ci)) park.put(d2, ci);
+          }
+          if (__ballot(park.n == KN_PARK) != 0) park.flush(L);
+        }
       }
     }
+    if (__ballot(park.n > 0) != 0) park.flush(L);
     const bool step = leaf || __ballot(skip == i + 1) != 0;
     i = __builtin_amdgcn_readfirstlane(step ? i + 1 : link);
   }
@@ -713,7 +741,8 @@ struct KnnTree {
   int64_t nc = 0;
   uint32_t nrec = 0;
   float bbox[6] = {0, 0, 0, 0, 0, 0};
-  DevBuf<double4> pts;
+  DevBuf<double4> pts;    // candidates in curve order
+  DevBuf<uint32_t> codes; // their Morton codes, ascending
   DevBuf<float4> rec;
 };
 
@@ -749,12 +778,13 @@ static void knn_build(const double* V, int64_t nv, const int32_t* cand, int64_t 
   GWN_CHECK(nbad[1] == 0, GWN_ERR_INVALID, "knn: a candidate index lies outside V");
   GWN_CHECK(nbad[0] == 0, GWN_ERR_INVALID, "knn: points must be finite");
   knn_bbox(Cf.p, nc, T.bbox, st);
-  DevBuf<uint32_t> keys(nc), order(nc);
-  morton_keys(Cf.p, nc, T.bbox, keys.p, order.p, st);
+  DevBuf<uint32_t> order(nc);
+  T.codes.alloc((size_t)nc);
+  morton_keys(Cf.p, nc, T.bbox, T.codes.p, order.p, st);
   {
     const size_t tb = radix_sort_temp_bytes(nc, false);
     DevBuf<uint8_t> tmp(tb);
-    radix_sort_u32(keys.p, order.p, nc, 30, tmp.p, tb, st);
+    radix_sort_u32(T.codes.p, order.p, nc, 30, tmp.p, tb, st);
   }
   T.pts.alloc((size_t)nc);
   hipLaunchKernelGGL(k_kn_points, dim3(div_up(nc, 256)), dim3(256), 0, st, V, cand, order.p, nc, T.pts.p);
@@ -813,9 +843,9 @@ static void knn_query(const KnnTree& T, const double* Q, int64_t nq, int k, int3
     radix_sort_u32(keys.p, perm.p, nq, 30, tmp.p, tb, st);
   }
   ProfileScope prof("knn", st);
-#define GWN_KNN(KMV)                                                                                            \
-  hipLaunchKernelGGL(k_knn<KMV>, dim3(div_up(nq, 256)), dim3(256), 0, st, T.rec.p, T.nrec, T.pts.p, T.nc, Q,    \
-                     perm.p, (int64_t)0, nq, k, (double*)nullptr, (int32_t*)nullptr, I)
+#define GWN_KNN(KMV)                                                                                           \
+  hipLaunchKernelGGL(k_knn<KMV>, dim3(div_up(nq, 256)), dim3(256), 0, st, T.rec.p, T.nrec, T.pts.p, T.codes.p, \
+                     T.nc, Q, perm.p, keys.p, (int64_t)0, nq, k, (double*)nullptr, (int32_t*)nullptr, I)
   if (k <= 8)
     GWN_KNN(8);
   else if (k <= 16)
@@ -835,8 +865,8 @@ static void knn_query(const KnnTree& T, const double* Q, int64_t nq, int k, int3
     DevBuf<int32_t> mi((size_t)chunk * k);
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
       const int64_t m = std::min(chunk, nq - q0);
-      hipLaunchKernelGGL(k_knn<0>, dim3(div_up(m, 256)), dim3(256), 0, st, T.rec.p, T.nrec, T.pts.p, T.nc, Q,
-                         perm.p, q0, m, k, md.p, mi.p, I);
+      hipLaunchKernelGGL(k_knn<0>, dim3(div_up(m, 256)), dim3(256), 0, st, T.rec.p, T.nrec, T.pts.p, T.codes.p,
+                         T.nc, Q, perm.p, keys.p, q0, m, k, md.p, mi.p, I);
     }
   }
 #undef GWN_KNN

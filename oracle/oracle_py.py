@@ -421,25 +421,26 @@ def _star_term(X, Y, Z):
     return (part[1] + part[2]) / 2
 
 
-def point_areas(P, I, N=None):
+def point_areas(P, I, N=None, rows=None):
     """point_areas.cpp:37-170 restated per point: the normal filter, the PCA
     plane (numpy's SVD in place of Eigen's JacobiSVD), a Delaunay triangulation
     of the projected neighbours (scipy / Qhull in place of CGAL; faces taken
     counter-clockwise as CGAL gives them) and the area sum over the faces at
-    vertex 0.  Returns (A, T)."""
+    vertex 0.  Returns (A, T); with `rows`, only those points' entries are
+    computed (the others stay 0)."""
     import warnings
     from scipy.spatial import Delaunay, QhullError
     with np.errstate(all="ignore"):  # degenerate faces give NaN sums, which count as 0
-        return _point_areas(P, I, N, Delaunay, QhullError, warnings)
+        return _point_areas(P, I, N, rows, Delaunay, QhullError, warnings)
 
 
-def _point_areas(P, I, N, Delaunay, QhullError, warnings):
+def _point_areas(P, I, N, rows, Delaunay, QhullError, warnings):
     P = np.asarray(P, np.float64).reshape(-1, 3)
     I = np.asarray(I)
     n = P.shape[0]
     A = np.zeros(n)
     T = np.zeros((n, 3))
-    for i in range(n):
+    for i in (range(n) if rows is None else rows):
         nb = P[I[i]]
         if N is not None and nb.shape[0] > 1:
             nn = np.asarray(N, np.float64)[I[i]]

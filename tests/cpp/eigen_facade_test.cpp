@@ -120,6 +120,74 @@ int main() {
     igl::winding_number(V2, E2, O2, W2);
     if (W2.rows() != 2 || std::fabs(W2(0, 0) - 1.0) > 1e-15 || std::fabs(W2(1, 0)) > 1e-15) bad |= 2048;
   }
+  // knn over an octree: tests/include/igl/knn.cpp:24-49 (cube corners, with a
+  // tie resolved to the lower index)
+  {
+    Eigen::MatrixXd Vk(8, 3), Pk(3, 3);
+    const double vk[8][3] = {{0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}};
+    const double pk[3][3] = {{0, 0, 0.6}, {0.3, 0.1, 0.2}, {0.7, 0.6, 0}};
+    const int ik[3][3] = {{0, 3, 2}, {3, 4, 0}, {5, 4, 1}};
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 3; ++j) Vk(i, j) = vk[i][j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Pk(i, j) = pk[i][j];
+    std::vector<std::vector<int>> PI;
+    Eigen::MatrixXi CH, I;
+    Eigen::MatrixXd CN, Wk;
+    igl::octree(Vk, PI, CH, CN, Wk);
+    igl::knn(Pk, Vk, 3, PI, CH, CN, Wk, I);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        if (I.rows() != 3 || I.cols() != 3 || I(i, j) != ik[i][j]) bad |= 4096;
+    igl::knn(Vk, 2, PI, CH, CN, Wk, I);  // V = P: each point is its own nearest
+    for (int i = 0; i < 8; ++i)
+      if (I(i, 0) != i) bad |= 4096;
+  }
+  // point areas on a flat triangular lattice: the centre's Voronoi area is
+  // (sqrt 3 / 2) a^2; then the cloud FWN with estimated areas
+  {
+    const double a = 0.5;
+    std::vector<double> pts;
+    for (int i = -4; i <= 4; ++i)
+      for (int j = -4; j <= 4; ++j) {
+        pts.push_back(a * (i + 0.5 * j));
+        pts.push_back(a * j * std::sqrt(3.0) / 2);
+        pts.push_back(0.25);
+      }
+    const int n = (int)pts.size() / 3;
+    Eigen::MatrixXd Pl(n, 3), Nl(n, 3), Al, Tl;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < 3; ++j) {
+        Pl(i, j) = pts[3 * i + j];
+        Nl(i, j) = j == 2 ? 1.0 : 0.0;
+      }
+    std::vector<std::vector<int>> PI;
+    Eigen::MatrixXi CH, I;
+    Eigen::MatrixXd CN, Wl;
+    igl::octree(Pl, PI, CH, CN, Wl);
+    igl::knn(Pl, 21, PI, CH, CN, Wl, I);
+    igl::copyleft::cgal::point_areas(Pl, I, Nl, Al, Tl);
+    const int c = n / 2;  // (0, 0)
+    if (std::fabs(Al(c, 0) - std::sqrt(3.0) / 2 * a * a) > 1e-12 || std::fabs(Tl(c, 2) - 1.0) > 1e-12) bad |= 8192;
+    Eigen::MatrixXd A1;
+    igl::copyleft::cgal::point_areas(Pl, I, Nl, A1);
+    if (A1(c, 0) != Al(c, 0)) bad |= 8192;
+    // a closed cloud: 12 icosahedron vertices with outward normals
+    const double t = (1 + std::sqrt(5.0)) / 2;
+    const double ico[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                               {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    Eigen::MatrixXd Pi(12, 3), Ni(12, 3), Qi(2, 3);
+    for (int i = 0; i < 12; ++i) {
+      const double r = std::sqrt(1 + t * t);
+      for (int j = 0; j < 3; ++j) Pi(i, j) = ico[i][j] / r, Ni(i, j) = ico[i][j] / r;
+    }
+    Qi(0, 0) = Qi(0, 1) = Qi(0, 2) = 0.0;
+    Qi(1, 0) = 5.0, Qi(1, 1) = Qi(1, 2) = 0.0;
+    Eigen::VectorXd W1, W2;
+    igl::copyleft::cgal::fast_winding_number(Pi, Ni, Qi, 2, 2.0, W1);
+    igl::copyleft::cgal::fast_winding_number(Pi, Ni, Qi, W2);
+    if (W1.rows() != 2 || !(W1(0, 0) > 0.5) || !(std::fabs(W1(1, 0)) < 0.1) || W2(0, 0) != W1(0, 0)) bad |= 16384;
+  }
   std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
   return bad;
 }
