@@ -200,7 +200,9 @@ __global__ void k_permute_points(const float* __restrict__ P, const float* __res
 // Centre of mass of split cells, summed as the reference sums it: FP32,
 // sequentially, over the cell's points in ascending index order
 // (fast_winding_number.cpp:60-73).  Per level, the (cell, point) pairs are
-// sorted by (cell ordinal, point index) and one thread folds each cell.
+// sorted by (cell ordinal, point index), the terms are laid out in that
+// order, and each cell's four sums are folded in it: a wavefront per cell on
+// levels of few (large) cells, a thread per cell on levels of many.
 __global__ void k_lvl_emit(const SplitCell* __restrict__ lvl, const uint32_t* __restrict__ off,
                            const int32_t* __restrict__ perm, uint64_t* __restrict__ keys) {
   const uint32_t j = blockIdx.x;
@@ -211,26 +213,90 @@ __global__ void k_lvl_emit(const SplitCell* __restrict__ lvl, const uint32_t* __
     keys[o + t] = ((uint64_t)j << 32) | (uint32_t)perm[sc.start + t];
 }
 
+// The terms the reference adds for a cell, in key order: a, a*x, a*y, a*z
+// (FP32 products, fast_winding_number.cpp:64-66).
+__global__ void k_lvl_terms(int64_t T, const uint64_t* __restrict__ keys, const float* __restrict__ P,
+                            const float* __restrict__ A, float4* __restrict__ terms) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int64_t i = (int64_t)(keys[t] & 0xFFFFFFFFull);
+  const float a = A[i];
+  terms[t] = make_float4(a, a * P[3 * i], a * P[3 * i + 1], a * P[3 * i + 2]);
+}
+
+// One thread per cell (levels of many small cells).
 __global__ void k_lvl_fold(const SplitCell* __restrict__ lvl, uint32_t ncell, const uint32_t* __restrict__ off,
-                           const uint64_t* __restrict__ keys, const float* __restrict__ P,
-                           const float* __restrict__ A, float* __restrict__ CM) {
+                           const float4* __restrict__ terms, float* __restrict__ CM) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ncell) return;
   const SplitCell sc = lvl[j];
   const int64_t cnt = sc.end - sc.start;
-  const uint32_t o = off[j];
+  const float4* v = terms + off[j];
   float at = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
   for (int64_t t = 0; t < cnt; ++t) {
-    const int64_t i = (int64_t)(keys[o + t] & 0xFFFFFFFFull);
-    const float a = A[i];
-    at += a;
-    m0 += a * P[3 * i];
-    m1 += a * P[3 * i + 1];
-    m2 += a * P[3 * i + 2];
+    const float4 q = v[t];
+    at += q.x;
+    m0 += q.y;
+    m1 += q.z;
+    m2 += q.w;
   }
   CM[3 * (int64_t)sc.cell] = m0 / at;
   CM[3 * (int64_t)sc.cell + 1] = m1 / at;
   CM[3 * (int64_t)sc.cell + 2] = m2 / at;
+}
+
+// One wavefront per cell (levels of few, large cells).  The lanes stage 512
+// terms at a time in LDS, the next 512 already in flight, and lanes 0-3 each
+// fold one of the four running sums in order: the same sequential FP32 sums,
+// one add per term and lane instead of a dependent load chain per term.
+constexpr int FOLD_CHUNK = 512;
+__global__ __launch_bounds__(64) void k_lvl_fold_wave(const SplitCell* __restrict__ lvl,
+                                                      const uint32_t* __restrict__ off,
+                                                      const float4* __restrict__ terms, float* __restrict__ CM) {
+  __shared__ __align__(16) float buf[4][FOLD_CHUNK];
+  const SplitCell sc = lvl[blockIdx.x];
+  const int64_t cnt = sc.end - sc.start;
+  const float4* v = terms + off[blockIdx.x];
+  const int lane = threadIdx.x;
+  constexpr int PER = FOLD_CHUNK / 64;
+  float4 r[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u)
+    if (lane + 64 * u < cnt) r[u] = v[lane + 64 * u];
+  float s = 0.0f;
+  for (int64_t base = 0; base < cnt; base += FOLD_CHUNK) {
+    const int n = (int)(cnt - base < FOLD_CHUNK ? cnt - base : FOLD_CHUNK);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = lane + 64 * u;
+      if (e < n) {
+        buf[0][e] = r[u].x;
+        buf[1][e] = r[u].y;
+        buf[2][e] = r[u].z;
+        buf[3][e] = r[u].w;
+      }
+    }
+    __syncthreads();
+    const int64_t nb = base + FOLD_CHUNK;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (nb + lane + 64 * u < cnt) r[u] = v[nb + lane + 64 * u];
+    if (lane < 4) {
+      const float* b = buf[lane];
+      int e = 0;
+      for (; e + 4 <= n; e += 4) {
+        const float4 q = *reinterpret_cast<const float4*>(b + e);
+        s += q.x;
+        s += q.y;
+        s += q.z;
+        s += q.w;
+      }
+      for (; e < n; ++e) s += b[e];
+    }
+    __syncthreads();
+  }
+  const float at = __shfl(s, 0);
+  if (lane >= 1 && lane <= 3) CM[3 * (int64_t)sc.cell + lane - 1] = s / at;
 }
 
 struct CellSums {
@@ -308,63 +374,104 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// one wavefront per large cell
+// Centre of a large cell that is not a split cell of this library's octree
+// (an unsplittable leaf, or any cell of a caller's octree): the sequential
+// FP32 sums over its list in list order, folded as k_lvl_fold_wave folds them
+// (one wavefront per cell; split cells already have theirs).
+__global__ __launch_bounds__(64) void k_cm_fold_wave(const int32_t* __restrict__ cells,
+                                                     const int32_t* __restrict__ CH, bool fold_cm,
+                                                     const int64_t* __restrict__ cstart,
+                                                     const int32_t* __restrict__ ccount, const float* __restrict__ P,
+                                                     const float* __restrict__ A, const int32_t* __restrict__ perm,
+                                                     float* __restrict__ CM) {
+  __shared__ __align__(16) float buf[4][FOLD_CHUNK];
+  const int32_t c = cells[blockIdx.x];
+  if (fold_cm && CH[8 * (int64_t)c] >= 0) return;  // uniform over the block
+  const int64_t s0 = cstart[c];
+  const int64_t cnt = ccount[c];
+  const int lane = threadIdx.x;
+  constexpr int PER = FOLD_CHUNK / 64;
+  float4 r[PER];
+  auto gather = [&](int64_t base) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (base + lane + 64 * u < cnt) {
+        const int64_t p = perm[s0 + base + lane + 64 * u];
+        const float a = A[p];
+        r[u] = make_float4(a, a * P[3 * p], a * P[3 * p + 1], a * P[3 * p + 2]);
+      }
+  };
+  gather(0);
+  float s = 0.0f;
+  for (int64_t base = 0; base < cnt; base += FOLD_CHUNK) {
+    const int n = (int)(cnt - base < FOLD_CHUNK ? cnt - base : FOLD_CHUNK);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = lane + 64 * u;
+      if (e < n) {
+        buf[0][e] = r[u].x;
+        buf[1][e] = r[u].y;
+        buf[2][e] = r[u].z;
+        buf[3][e] = r[u].w;
+      }
+    }
+    __syncthreads();
+    gather(base + FOLD_CHUNK);
+    if (lane < 4) {
+      const float* bb = buf[lane];
+      int e = 0;
+      for (; e + 4 <= n; e += 4) {
+        const float4 q = *reinterpret_cast<const float4*>(bb + e);
+        s += q.x;
+        s += q.y;
+        s += q.z;
+        s += q.w;
+      }
+      for (; e < n; ++e) s += bb[e];
+    }
+    __syncthreads();
+  }
+  const float at = __shfl(s, 0);
+  if (lane >= 1 && lane <= 3) CM[3 * (int64_t)c + lane - 1] = s / at;
+}
+
+// Moments of a large cell, one wavefront per segment of at most
+// MOMENT_SEGMENT of its points (the root alone spreads over many CUs): FP64
+// partial sums of A n, A q n^T, 1/2 q_k A q n^T and the largest |q| (q = p -
+// CM), written to part[item * NT .. ].
+constexpr int MOMENT_SEGMENT = 4096;
 template <int NT>
-__global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ cells, uint32_t count,
-                                                    const int32_t* __restrict__ CH, bool fold_cm,
+__global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ item_cell,
+                                                    const int32_t* __restrict__ item_j0, uint32_t nitems,
                                                     const int64_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ ccount, const float* __restrict__ P,
                                                     const float* __restrict__ N, const float* __restrict__ A,
-                                                    const int32_t* __restrict__ perm, float* __restrict__ CM,
-                                                    float* __restrict__ R, float* __restrict__ EC) {
+                                                    const int32_t* __restrict__ perm, const float* __restrict__ CM,
+                                                    double* __restrict__ part) {
   const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (w >= count) return;
-  const int32_t c = cells[w];
+  if (w >= nitems) return;
+  const int32_t c = item_cell[w];
   const int64_t s = cstart[c];
-  const int32_t cnt = ccount[c];
-  // with fold_cm, split cells take their centre from k_lvl_fold; every other
-  // cell (an unsplittable leaf of more than 32 points with equal 42-level
-  // keys, or any cell of a caller's octree) sums it here, serially, in the
-  // order of its point list
-  float lc[3] = {0.0f, 0.0f, 0.0f};
-  const bool split = fold_cm && CH[8 * (int64_t)c] >= 0;
-  if (!split && lane == 0) {
-    float at = 0.0f;
-    for (int32_t j = 0; j < cnt; ++j) {
-      const int64_t p = perm[s + j];
-      const float af = A[p];
-      at += af;
-      for (int d = 0; d < 3; ++d) lc[d] += af * P[3 * p + d];
-    }
-    for (int d = 0; d < 3; ++d) lc[d] = lc[d] / at;
-  }
-  for (int d = 0; d < 3; ++d) lc[d] = __shfl(lc[d], 0, 64);
+  const int32_t j0 = item_j0[w];
+  const int32_t j1 = j0 + MOMENT_SEGMENT < ccount[c] ? j0 + MOMENT_SEGMENT : ccount[c];
+  const float c3[3] = {CM[3 * (int64_t)c], CM[3 * (int64_t)c + 1], CM[3 * (int64_t)c + 2]};
   double an0 = 0, an1 = 0, an2 = 0;
-  for (int32_t j = lane; j < cnt; j += 64) {
-    const int64_t p = perm[s + j];
-    const double a = A[p];
-    an0 += a * (double)N[3 * p];
-    an1 += a * (double)N[3 * p + 1];
-    an2 += a * (double)N[3 * p + 2];
-  }
-  an0 = wave_sum(an0);
-  an1 = wave_sum(an1);
-  an2 = wave_sum(an2);
-  const float c3[3] = {split ? CM[3 * (int64_t)c] : lc[0], split ? CM[3 * (int64_t)c + 1] : lc[1],
-                       split ? CM[3 * (int64_t)c + 2] : lc[2]};
   double e[NT > 3 ? NT - 3 : 1];
   for (int k = 0; k < NT - 3; ++k) e[k] = 0;
   float rmax = 0.0f;
-  for (int32_t j = lane; j < cnt; j += 64) {
+  for (int32_t j = j0 + lane; j < j1; j += 64) {
     const int64_t p = perm[s + j];
     const double a = A[p];
+    const double n3[3] = {N[3 * p], N[3 * p + 1], N[3 * p + 2]};
+    an0 += a * n3[0];
+    an1 += a * n3[1];
+    an2 += a * n3[2];
     float q[3];
     for (int d = 0; d < 3; ++d) q[d] = P[3 * p + d] - c3[d];
     const float nrm = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
     rmax = nrm > rmax ? nrm : rmax;
     if (NT >= 12) {
-      const double n3[3] = {N[3 * p], N[3 * p + 1], N[3 * p + 2]};
       for (int v = 0; v < 3; ++v)
         for (int u = 0; u < 3; ++u) {
           const double t = a * (double)q[u] * n3[v];
@@ -374,18 +481,44 @@ __global__ __launch_bounds__(256) void k_cell_large(const int32_t* __restrict__ 
         }
     }
   }
+  an0 = wave_sum(an0);
+  an1 = wave_sum(an1);
+  an2 = wave_sum(an2);
   rmax = wave_max(rmax);
   for (int k = 0; k < NT - 3; ++k) e[k] = wave_sum(e[k]);
   if (lane == 0) {
-    if (!split)
-      for (int d = 0; d < 3; ++d) CM[3 * (int64_t)c + d] = c3[d];
-    R[c] = rmax;
-    float* ec = EC + (int64_t)NT * c;
-    ec[0] = (float)an0;
-    ec[1] = (float)an1;
-    ec[2] = (float)an2;
-    for (int k = 0; k < NT - 3; ++k) ec[3 + k] = (float)e[k];
+    double* o = part + (int64_t)(NT + 1) * w;
+    o[0] = an0;
+    o[1] = an1;
+    o[2] = an2;
+    for (int k = 0; k < NT - 3; ++k) o[3 + k] = e[k];
+    o[NT] = rmax;
   }
+}
+
+// A large cell's segments summed in segment order; EC and R in FP32.
+template <int NT>
+__global__ void k_cell_combine(const int32_t* __restrict__ cells, uint32_t count, const uint32_t* __restrict__ item_off,
+                               const double* __restrict__ part, float* __restrict__ R, float* __restrict__ EC) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t c = cells[t];
+  double sum[NT];
+  for (int k = 0; k < NT; ++k) sum[k] = 0.0;
+  double rmax = 0.0;
+  for (uint32_t it = item_off[t]; it < item_off[t + 1]; ++it) {
+    const double* o = part + (int64_t)(NT + 1) * it;
+    for (int k = 0; k < NT; ++k) sum[k] += o[k];
+    rmax = o[NT] > rmax ? o[NT] : rmax;
+  }
+  R[c] = (float)rmax;
+  for (int k = 0; k < NT; ++k) EC[(int64_t)NT * c + k] = (float)sum[k];
+}
+
+__global__ void k_large_counts(const int32_t* __restrict__ cells, uint32_t count, const int32_t* __restrict__ ccount,
+                               int32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) out[t] = ccount[cells[t]];
 }
 
 __global__ void k_cell_empty(int64_t ncells, const int32_t* __restrict__ ccount, int nt, float* __restrict__ CM,
@@ -397,17 +530,23 @@ __global__ void k_cell_empty(int64_t ncells, const int32_t* __restrict__ ccount,
   for (int k = 0; k < nt; ++k) EC[nt * c + k] = 0.0f;
 }
 
-// classify non-empty cells into small / large lists
-__global__ void k_cell_lists(int64_t ncells, const int32_t* __restrict__ ccount, int32_t* __restrict__ small,
-                             uint32_t* __restrict__ nsmall, int32_t* __restrict__ large, uint32_t* __restrict__ nlarge) {
+// classify non-empty cells into small / large lists: flags, scans, scatter
+__global__ void k_cell_flags(int64_t ncells, const int32_t* __restrict__ ccount, uint32_t* __restrict__ fs,
+                             uint32_t* __restrict__ fl) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const int32_t k = ccount[c];
-  if (k <= 0) return;
-  if (k <= 32)
-    small[atomicAdd(nsmall, 1u)] = (int32_t)c;
-  else
-    large[atomicAdd(nlarge, 1u)] = (int32_t)c;
+  fs[c] = (k > 0 && k <= 32) ? 1u : 0u;
+  fl[c] = k > 32 ? 1u : 0u;
+}
+
+__global__ void k_cell_lists(int64_t ncells, const uint32_t* __restrict__ fs, const uint32_t* __restrict__ ps,
+                             const uint32_t* __restrict__ fl, const uint32_t* __restrict__ pl,
+                             int32_t* __restrict__ small, int32_t* __restrict__ large) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  if (fs[c]) small[ps[c]] = (int32_t)c;
+  if (fl[c]) large[pl[c]] = (int32_t)c;
 }
 
 // ---------------- rope over non-empty non-root cells ----------------
@@ -624,10 +763,17 @@ static void cell_moments(gwn_cloud* c, bool fold_cm, const SplitCell* all, const
   {
     DevBuf<int32_t> small(ncells), large(ncells);
     DevBuf<uint32_t> ns(2);
-    GWN_HIP(hipMemsetAsync(ns.p, 0, 8, st));
-    hipLaunchKernelGGL(k_cell_lists, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, small.p, ns.p,
-                       large.p, ns.p + 1);
-    check_launch("k_cell_lists");
+    {
+      DevBuf<uint32_t> fs(ncells), fl(ncells), ps(ncells), pl(ncells);
+      DevBuf<uint8_t> sc(scan_temp_bytes(ncells));
+      hipLaunchKernelGGL(k_cell_flags, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, fs.p, fl.p);
+      check_launch("k_cell_flags");
+      exclusive_scan_u32(fs.p, ps.p, ncells, ns.p, sc.p, st);
+      exclusive_scan_u32(fl.p, pl.p, ncells, ns.p + 1, sc.p, st);
+      hipLaunchKernelGGL(k_cell_lists, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, fs.p, ps.p, fl.p, pl.p,
+                         small.p, large.p);
+      check_launch("k_cell_lists");
+    }
     uint32_t cnts[2];
     GWN_HIP(hipMemcpyAsync(cnts, ns.p, 8, hipMemcpyDeviceToHost, st));
     GWN_HIP(hipStreamSynchronize(st));
@@ -656,21 +802,70 @@ static void cell_moments(gwn_cloud* c, bool fold_cm, const SplitCell* all, const
       const size_t tb = radix_sort_temp_bytes(T, true);
       DevBuf<uint8_t> tmp(tb);
       radix_sort_u64(keys.p, dummy.p, T, 32 + obits, tmp.p, tb, st);
-      hipLaunchKernelGGL(k_lvl_fold, dim3(div_up(m, 64)), dim3(64), 0, st, all + b, m, off.p, keys.p, c->P.p,
-                         c->A.p, c->CM.p);
-      check_launch("k_lvl_fold");
+      DevBuf<float4> terms(T);
+      hipLaunchKernelGGL(k_lvl_terms, dim3(div_up(T, 256)), dim3(256), 0, st, (int64_t)T, keys.p, c->P.p, c->A.p,
+                         terms.p);
+      check_launch("k_lvl_terms");
+      if (m <= 65536) {
+        hipLaunchKernelGGL(k_lvl_fold_wave, dim3(m), dim3(64), 0, st, all + b, off.p, terms.p, c->CM.p);
+        check_launch("k_lvl_fold_wave");
+      } else {
+        hipLaunchKernelGGL(k_lvl_fold, dim3(div_up(m, 64)), dim3(64), 0, st, all + b, m, off.p, terms.p, c->CM.p);
+        check_launch("k_lvl_fold");
+      }
       GWN_HIP(hipStreamSynchronize(st));
+    }
+    // large cells: centres of the ones not folded above, then their moments
+    // in segments of MOMENT_SEGMENT points
+    std::vector<int32_t> icell, ij0;
+    std::vector<uint32_t> ioff(1, 0);
+    if (cnts[1]) {
+      hipLaunchKernelGGL(k_cm_fold_wave, dim3(cnts[1]), dim3(64), 0, st, large.p, c->CH.p, fold_cm, c->cstart.p,
+                         c->ccount.p, c->P.p, c->A.p, c->perm.p, c->CM.p);
+      check_launch("k_cm_fold_wave");
+      DevBuf<int32_t> lc(cnts[1]);
+      hipLaunchKernelGGL(k_large_counts, dim3(div_up(cnts[1], 256)), dim3(256), 0, st, large.p, cnts[1],
+                         c->ccount.p, lc.p);
+      check_launch("k_large_counts");
+      std::vector<int32_t> hl(cnts[1]), hc(cnts[1]);
+      GWN_HIP(hipMemcpyAsync(hl.data(), large.p, 4 * (size_t)cnts[1], hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipMemcpyAsync(hc.data(), lc.p, 4 * (size_t)cnts[1], hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipStreamSynchronize(st));
+      ioff.reserve(cnts[1] + 1);
+      for (uint32_t t = 0; t < cnts[1]; ++t) {
+        for (int32_t j0 = 0; j0 < hc[t]; j0 += MOMENT_SEGMENT) {
+          icell.push_back(hl[t]);
+          ij0.push_back(j0);
+        }
+        ioff.push_back((uint32_t)icell.size());
+      }
+    }
+    const uint32_t nitems = (uint32_t)icell.size();
+    DevBuf<int32_t> dcell, dj0;
+    DevBuf<uint32_t> doff;
+    DevBuf<double> part;
+    if (nitems) {
+      dcell.alloc(nitems);
+      dj0.alloc(nitems);
+      doff.alloc(ioff.size());
+      part.alloc((size_t)(nt + 1) * nitems);
+      GWN_HIP(hipMemcpyAsync(dcell.p, icell.data(), 4 * (size_t)nitems, hipMemcpyHostToDevice, st));
+      GWN_HIP(hipMemcpyAsync(dj0.p, ij0.data(), 4 * (size_t)nitems, hipMemcpyHostToDevice, st));
+      GWN_HIP(hipMemcpyAsync(doff.p, ioff.data(), 4 * ioff.size(), hipMemcpyHostToDevice, st));
     }
 #define GWN_CELL_LAUNCH(NTV)                                                                                        \
   do {                                                                                                              \
     if (cnts[0])                                                                                                    \
       hipLaunchKernelGGL(k_cell_small<NTV>, dim3(div_up(cnts[0], 128)), dim3(128), 0, st, small.p, cnts[0],        \
-                         c->CH.p, fold_cm, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
-                         c->EC.p);                                                                                  \
-    if (cnts[1])                                                                                                    \
-      hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)cnts[1] * 64, 256)), dim3(256), 0, st, large.p,    \
-                         cnts[1], c->CH.p, fold_cm, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p, c->R.p,     \
-                         c->EC.p);                                                                                  \
+                         c->CH.p, fold_cm, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p,   \
+                         c->R.p, c->EC.p);                                                                          \
+    if (nitems) {                                                                                                   \
+      hipLaunchKernelGGL(k_cell_large<NTV>, dim3(div_up((int64_t)nitems * 64, 256)), dim3(256), 0, st, dcell.p,    \
+                         dj0.p, nitems, c->cstart.p, c->ccount.p, c->P.p, c->N.p, c->A.p, c->perm.p, c->CM.p,      \
+                         part.p);                                                                                   \
+      hipLaunchKernelGGL(k_cell_combine<NTV>, dim3(div_up(cnts[1], 256)), dim3(256), 0, st, large.p, cnts[1],      \
+                         doff.p, part.p, c->R.p, c->EC.p);                                                          \
+    }                                                                                                               \
   } while (0)
     if (nt == 3)
       GWN_CELL_LAUNCH(3);
