@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--tree", choices=["sah4", "lbvh"], default="sah4")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 1.3M-triangle mixed vs FP64 comparison")
     ap.add_argument("--no-rows", action="store_true", help="skip the cloud / exact / signed-distance legs")
+    ap.add_argument("--rows", default="cloud,exact,signed_distance,point_areas",
+                    help="comma-separated rows reported beside the headline")
     ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter calibration)")
@@ -270,8 +272,11 @@ def main():
 
     others = {}
     if not args.no_rows:
+        wanted = set(args.rows.split(","))
         for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance),
                          ("point_areas", run_point_areas)):
+            if name not in wanted:
+                continue
             try:
                 others[name] = fn(args.no_cpu)
             except Exception as e:

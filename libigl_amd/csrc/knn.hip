@@ -674,17 +674,19 @@ __device__ void point_area(const double* __restrict__ P, int64_t n, const int32_
   Aout[i] = isfinite(acc) ? acc : 0.0;
 }
 
-// points i0 .. i0 + m of the n; the memory form keeps lane t's plane points
-// at mx/my[j * m + t]
+// points order[i0 .. i0 + m) of the n (Morton order, so a wavefront's
+// neighbourhoods overlap in cache); the memory form keeps lane t's plane
+// points at mx/my[j * m + t]
 template <int KM>
-__global__ __launch_bounds__(256) void k_point_areas(const double* __restrict__ P, int64_t n, int64_t i0, int64_t m,
+__global__ __launch_bounds__(256) void k_point_areas(const double* __restrict__ P, int64_t n,
+                                                     const uint32_t* __restrict__ order, int64_t i0, int64_t m,
                                                      const int32_t* __restrict__ I, int k,
                                                      const double* __restrict__ N, double* __restrict__ mx,
                                                      double* __restrict__ my, double* __restrict__ A,
                                                      double* __restrict__ T, uint32_t* __restrict__ bad) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m) return;
-  const int64_t i = i0 + t;
+  const int64_t i = order[i0 + t];
   if constexpr (KM > 0) {
     RegPlane<KM> u;
     point_area<KM>(P, n, I + i * (int64_t)k, k, N, i, u, A, T, bad);
@@ -902,12 +904,27 @@ void point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k,
                         double* T, hipStream_t st) {
   GWN_CHECK(k >= 1 && k < (1 << 20), GWN_ERR_INVALID, "point_areas: I must have 1 to 2^20 columns");
   if (n == 0) return;
-  DevBuf<uint32_t> bad(1);
-  GWN_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+  DevBuf<uint32_t> bad(1), keys(n), order(n);
+  {
+    DevBuf<float> Pf(3 * (size_t)n);
+    GWN_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+    hipLaunchKernelGGL(k_kn_narrow, dim3(div_up(3 * n, 256)), dim3(256), 0, st, P, n, Pf.p, bad.p);
+    check_launch("k_kn_narrow");
+    float bb[6];
+    knn_bbox(Pf.p, n, bb, st);
+    morton_keys(Pf.p, n, bb, keys.p, order.p, st);
+    const size_t tb = radix_sort_temp_bytes(n, false);
+    DevBuf<uint8_t> tmp(tb);
+    radix_sort_u32(keys.p, order.p, n, 30, tmp.p, tb, st);
+    uint32_t nbad = 0;
+    GWN_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    GWN_CHECK(nbad == 0, GWN_ERR_INVALID, "point_areas: points must be finite");
+  }
   ProfileScope prof("point_areas", st);
-#define GWN_PA(KMV)                                                                                                \
-  hipLaunchKernelGGL(k_point_areas<KMV>, dim3(div_up(n, 256)), dim3(256), 0, st, P, n, (int64_t)0, n, I, (int)k, N, \
-                     (double*)nullptr, (double*)nullptr, A, T, bad.p)
+#define GWN_PA(KMV)                                                                                               \
+  hipLaunchKernelGGL(k_point_areas<KMV>, dim3(div_up(n, 256)), dim3(256), 0, st, P, n, order.p, (int64_t)0, n, I, \
+                     (int)k, N, (double*)nullptr, (double*)nullptr, A, T, bad.p)
   if (k <= 8)
     GWN_PA(8);
   else if (k <= 16)
@@ -921,8 +938,8 @@ void point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k,
     DevBuf<double> mx((size_t)chunk * k), my((size_t)chunk * k);
     for (int64_t i0 = 0; i0 < n; i0 += chunk) {
       const int64_t m = std::min(chunk, n - i0);
-      hipLaunchKernelGGL(k_point_areas<0>, dim3(div_up(m, 256)), dim3(256), 0, st, P, n, i0, m, I, (int)k, N, mx.p,
-                         my.p, A, T, bad.p);
+      hipLaunchKernelGGL(k_point_areas<0>, dim3(div_up(m, 256)), dim3(256), 0, st, P, n, order.p, i0, m, I, (int)k,
+                         N, mx.p, my.p, A, T, bad.p);
     }
   }
 #undef GWN_PA
