@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "winding queries/s at max |dw|<=1e-4 vs FP64 oracle"
 ACC_TARGET = 1e-4
-BETA_GRID = [2.0, 3.0, 3.5, 4.0, 4.5, 5.0, 5.5, 6.0, 7.0, 8.0]
+BETA_GRID = [2.0, 3.0, 3.5, 4.0, 4.25, 4.5, 4.6, 4.7, 4.8, 4.9, 5.0, 5.25, 5.5, 6.0, 7.0, 8.0]
 # Counted FLOPs per traversal event (add/mul/div/sqrt = 1, atan2 = 1):
 FLOP_TEST = 10    # |q-P|^2 <= maxP2 * beta^2 per child slot (7179-7191)
 FLOP_FAR = 83     # accepted order-2 far-field slot (7202-7241)
@@ -98,7 +98,7 @@ def main():
     torch.cuda.synchronize()
     build_s = time.perf_counter() - tb
     info = bvh.info()
-    log(f"[bench] SAH4 tree: {info.num_nodes} nodes, {info.num_slots} slots, depth {info.depth}, build {build_s:.3f}s")
+    log(f"[bench] {args.tree.upper()} tree: {info.num_nodes} nodes, {info.num_slots} slots, depth {info.depth}, build {build_s:.3f}s")
 
     nq = Q.shape[0]
     Qf = Q.astype(np.float32)

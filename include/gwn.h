@@ -272,7 +272,7 @@ void gwn_free(void* p);
  * when they do not fit).  gwn_server_submit returns at once; a request is
  * admitted, first come first served, when a slot is idle and its device
  * footprint (gwn_server_request_bytes: queries, results and sort workspace of
- * one chunk of at most 262,144 queries, doubled for larger requests, which run
+ * one chunk of at most 1,048,576 queries, doubled for larger requests, which run
  * as a pipeline of such chunks) fits in the remaining budget.
  * Admitted requests run concurrently on different streams, so their kernels
  * overlap on the device.  A request's result is written to the caller's W

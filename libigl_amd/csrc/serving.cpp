@@ -33,8 +33,13 @@ namespace {
 // slot (pinned host and device) let the host narrow chunk c+1 and copy out
 // chunk c-1 while the device works on chunk c; per-query results do not
 // depend on which other queries share a launch, so chunking leaves every
-// result bit-identical to one gwn_soup_query over the whole batch.
-constexpr int64_t kChunk = 1 << 18;
+// result bit-identical to one gwn_soup_query over the whole batch.  Chunks
+// are large: a launch over more queries packs them closer along the Morton
+// curve (more coherent wavefronts) and pays the per-chunk sort and count once
+// (measured on 64 meshes of 1k-500k triangles at 2000 requests/s: p99 4.4 ms
+// against 6.8 ms with chunks of 2^18; at scale 5, 2.2e8 against 1.7e8
+// queries/s sustained).
+constexpr int64_t kChunk = 1 << 20;
 
 size_t a256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -202,12 +207,12 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)cn));
     uint8_t* ws = base + a256(12 * (size_t)cn) + a256(8 * (size_t)cn);
     GWN_HIP(hipMemcpyAsync(dQ, hQ, 12 * (size_t)n, hipMemcpyHostToDevice, sl->stream));
-    // The per-lane walk gives the lower latency on this mix at accuracy_scale
-    // 2; at higher scales a query visits several times more records and the
-    // wave-coherent walk's throughput wins on large chunks (measured on 64
-    // meshes of 1k-500k triangles: at scale 5 and 1000 requests/s, p99 42 ms
-    // against 247 ms).  Both walks give bitwise the same result.
-    const int walk = (r->beta >= 4.0f && n >= 65536) ? WALK_WAVE : WALK_LANE;
+    // Small chunks take the per-lane walk (their queries are too sparse for
+    // wavefront coherence); large ones the wave-coherent walk, whose
+    // throughput is the higher one there (measured on the mix above: at scale
+    // 5 and 1000 requests/s, p99 15 ms against 42 ms with the per-lane walk
+    // everywhere).  Both walks give bitwise the same result.
+    const int walk = n >= 65536 ? WALK_WAVE : WALK_LANE;
     soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn), walk);
     GWN_HIP(hipMemcpyAsync(sl->hW[b], dW, 8 * (size_t)n, hipMemcpyDeviceToHost, sl->stream));
     GWN_HIP(hipEventRecord(sl->ev[b], sl->stream));
