@@ -52,6 +52,7 @@ typedef enum {
 
 typedef struct gwn_soup gwn_soup;   /* igl::FastWindingNumberBVH (fast_winding_number.h:140-152) */
 typedef struct gwn_cloud gwn_cloud; /* octree + CM/R/EC of the point-cloud path */
+typedef struct gwn_antipodal gwn_antipodal; /* igl::WindingNumberAntipodalScene */
 
 typedef struct {
   int64_t num_vertices, num_triangles;
@@ -174,6 +175,29 @@ int gwn_cloud_copy_expansion(const gwn_cloud* c, float* CM, float* R, float* EC)
  * branch (fast_winding_number.cpp:300-310).  Q nq x 3 float, WN float. */
 int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN);
 int gwn_cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Antipodal method (WindingNumberAntipodalScene.h, winding_number_antipodal.h:
+ * 17-53): W(p) = sum over the weighted open-boundary edges of the mesh of
+ * half the solid angle of (-x0, a - p, b - p) / (2 pi), plus the signed
+ * number of triangles the ray p + t x0 (t >= 0) crosses (+1 through a
+ * triangle's front, (v1-v0)x(v2-v0) . x0 > 0).  The reference counts the
+ * crossings with a caller-supplied intersector (Embree,
+ * embree/EmbreeIntersector.cpp:390-443); here they are counted on the GPU
+ * over the mesh's own tree, in FP64, with a watertight ray-triangle test.
+ * gwn_antipodal_build is the scene constructor (x0 NULL: the reference's
+ * default normalize(1, sqrt 2, sqrt 3)); gwn_antipodal_query its batch
+ * winding_number; gwn_winding_number_antipodal the one-shot
+ * igl::winding_number_antipodal(V,F,intersector,O,W).  V nv x 3, F nf x 3,
+ * O no x 3, W no; FP64. */
+int gwn_antipodal_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* x0,
+                        gwn_antipodal** out);
+void gwn_antipodal_free(gwn_antipodal* a);
+int gwn_antipodal_info(const gwn_antipodal* a, int64_t* num_boundary_segments, int64_t* num_faces, double* x0);
+int gwn_antipodal_query(const gwn_antipodal* a, const double* O, int64_t no, double* W);
+int gwn_antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no, double* W, void* stream);
+int gwn_winding_number_antipodal(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                 int64_t no, double* W);
 
 /* ------------------------------------------------------------------------
  * Point areas for a cloud without them: k nearest neighbours and a local

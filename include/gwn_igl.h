@@ -16,6 +16,8 @@
 //   igl::fast_winding_number(P,N,A,Q,WN)              fast_winding_number.h:126-131
 //   struct igl::FastWindingNumberBVH                  fast_winding_number.h:140-152
 //   igl::read_triangle_mesh(str,V,F)                   read_triangle_mesh.h (.off / .obj)
+//   igl::winding_number_antipodal(V,F,intersector,O,W) winding_number_antipodal.h:43-53
+//   igl::WindingNumberAntipodalScene<Scalar>          WindingNumberAntipodalScene.h:48-256
 //   igl::knn(P,k,point_indices,CH,CN,W,I)             knn.h:33-45
 //   igl::knn(P,V,k,point_indices,CH,CN,W,I)           knn.h:51-67
 //   igl::copyleft::cgal::point_areas(P,I,N,A[,T])     copyleft/cgal/point_areas.h:44-62
@@ -39,6 +41,7 @@
 // are declared when <Eigen/Core> is available.
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <limits>
 #include <numeric>
@@ -428,6 +431,26 @@ void cloud_estimated_areas(const MP& P, const MN& N, const MQ& Q, int order, dou
     WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
 }
 
+// WindingNumberAntipodalScene.h:69-78: the scene (x0 NULL: the default direction)
+template <typename MV, typename MF>
+gwn_antipodal* antipodal_build(const MV& V, const MF& F, const double* x0) {
+  const auto v = rows3<double>(V, "V");
+  const auto f = faces(F);
+  gwn_antipodal* a = nullptr;
+  check(gwn_antipodal_build(v.data(), V.rows(), f.data(), F.rows(), x0, &a));
+  return a;
+}
+
+template <typename MO, typename MW>
+void antipodal_query(const gwn_antipodal* a, const MO& O, MW& W) {
+  const auto o = rows3<double>(O, "O");
+  std::vector<double> w((size_t)O.rows());
+  if (O.rows() > 0) check(gwn_antipodal_query(a, o.data(), O.rows(), w.data()));
+  W.resize(O.rows(), 1);
+  for (int64_t i = 0; i < (int64_t)O.rows(); ++i)
+    W(i, 0) = static_cast<typename std::decay<decltype(W(0, 0))>::type>(w[i]);
+}
+
 // .off / .obj file; returns false (and leaves gwn_last_error) on failure, as
 // the reference returns false
 template <typename MV, typename MF>
@@ -604,6 +627,86 @@ void knn(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>
   (void)CN;  // the candidates and distances fix the answer; the cells' boxes only prune
   (void)W;
   gwn_detail::knn(P.derived(), V.derived(), k, point_indices, CH.derived(), I.derived());
+}
+
+// igl::WindingNumberAntipodalScene (WindingNumberAntipodalScene.h:48-256).
+// The intersector arguments of winding_number are accepted and not used: the
+// scene counts the ray crossings on the GPU over the mesh's own tree.  Like
+// the reference's scene it keeps no reference to V and F.
+template <typename Scalar>
+class WindingNumberAntipodalScene {
+ public:
+  using Point = Eigen::Matrix<Scalar, 1, 3>;
+  using Direction = Eigen::Matrix<Scalar, 1, 3>;
+
+  template <typename DerivedV, typename DerivedF>
+  WindingNumberAntipodalScene(const Eigen::MatrixBase<DerivedV>& V, const Eigen::MatrixBase<DerivedF>& F,
+                              const Direction& x0 = default_x0())
+      : m_x0(x0) {
+    const double d[3] = {(double)x0(0, 0), (double)x0(0, 1), (double)x0(0, 2)};
+    m_scene = gwn_detail::antipodal_build(V.derived(), F.derived(), d);
+    int64_t nb = 0, nf = 0;
+    gwn_detail::check(gwn_antipodal_info(m_scene, &nb, &nf, nullptr));
+    m_nb = (size_t)nb;
+    m_nf = (size_t)nf;
+  }
+  WindingNumberAntipodalScene(const WindingNumberAntipodalScene&) = delete;
+  WindingNumberAntipodalScene& operator=(const WindingNumberAntipodalScene&) = delete;
+  ~WindingNumberAntipodalScene() {
+    if (m_scene) gwn_antipodal_free(m_scene);
+  }
+
+  template <typename Intersector, typename Derivedp>
+  Scalar winding_number(const Intersector& intersector, const Eigen::MatrixBase<Derivedp>& p) const {
+    (void)intersector;
+    Eigen::Matrix<double, Eigen::Dynamic, Eigen::Dynamic> O(1, 3);
+    for (int j = 0; j < 3; ++j) O(0, j) = static_cast<double>(p.derived()(0, j));
+    Eigen::Matrix<double, Eigen::Dynamic, 1> W;
+    gwn_detail::antipodal_query(m_scene, O, W);
+    return static_cast<Scalar>(W(0, 0));
+  }
+
+  template <typename Intersector, typename DerivedO, typename DerivedW>
+  void winding_number(const Intersector& intersector, const Eigen::MatrixBase<DerivedO>& O,
+                      Eigen::PlainObjectBase<DerivedW>& W) const {
+    (void)intersector;
+    gwn_detail::antipodal_query(m_scene, O.derived(), W.derived());
+  }
+
+  const Direction& x0() const { return m_x0; }
+  size_t num_boundary_segments() const { return m_nb; }
+  size_t num_faces() const { return m_nf; }
+
+  // normalize(1, sqrt 2, sqrt 3) (WindingNumberAntipodalScene.h:167-173)
+  static Direction default_x0() {
+    const double n = std::sqrt((1.0 + 2.0) + 3.0);
+    Direction d;
+    d(0, 0) = static_cast<Scalar>(1.0 / n);
+    d(0, 1) = static_cast<Scalar>(std::sqrt(2.0) / n);
+    d(0, 2) = static_cast<Scalar>(std::sqrt(3.0) / n);
+    return d;
+  }
+
+ private:
+  gwn_antipodal* m_scene = nullptr;
+  Direction m_x0;
+  size_t m_nb = 0, m_nf = 0;
+};
+
+// winding_number_antipodal.h:43-53
+template <typename Intersector, typename DerivedV, typename DerivedF, typename DerivedO, typename DerivedW>
+void winding_number_antipodal(const Eigen::MatrixBase<DerivedV>& V, const Eigen::MatrixBase<DerivedF>& F,
+                              const Intersector& intersector, const Eigen::MatrixBase<DerivedO>& O,
+                              Eigen::PlainObjectBase<DerivedW>& W) {
+  (void)intersector;
+  gwn_antipodal* a = gwn_detail::antipodal_build(V.derived(), F.derived(), nullptr);
+  try {
+    gwn_detail::antipodal_query(a, O.derived(), W.derived());
+  } catch (...) {
+    gwn_antipodal_free(a);
+    throw;
+  }
+  gwn_antipodal_free(a);
 }
 
 namespace copyleft {

@@ -68,6 +68,12 @@ SIGNATURES = {
     "gwn_cloud_copy_expansion": (C.c_int, [vp, vp, vp, vp]),
     "gwn_cloud_query": (C.c_int, [vp, f32p, C.c_int64, C.c_double, f32p]),
     "gwn_cloud_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, vp, vp]),
+    "gwn_antipodal_build": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, vp, C.POINTER(vp)]),
+    "gwn_antipodal_free": (None, [vp]),
+    "gwn_antipodal_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), vp]),
+    "gwn_antipodal_query": (C.c_int, [vp, f64p, C.c_int64, f64p]),
+    "gwn_antipodal_query_device": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
+    "gwn_winding_number_antipodal": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
     "gwn_knn_columns": (C.c_int, [C.c_int64, C.c_int64]),
     "gwn_knn": (C.c_int, [f64p, C.c_int64, f64p, C.c_int64, C.c_int64, vp, C.c_int64, i32p]),
     "gwn_knn_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, C.c_int64, vp, C.c_int64, vp, vp]),

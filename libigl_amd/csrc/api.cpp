@@ -123,6 +123,18 @@ static void soup_build_common(gwn_soup* s, std::vector<float>& U, const double* 
   soup_finish_build(s, st);
 }
 
+// An order-0 soup over FP64 vertices (for the antipodal scene's rays).
+void soup_build_into(gwn_soup* s, const double* V, int64_t nv, const int32_t* F, int64_t nf, int tree_kind) {
+  GWN_CHECK(nv == 0 || V != nullptr, GWN_ERR_INVALID, "null vertices");
+  std::vector<float> U(3 * (size_t)nv);
+  for (size_t i = 0; i < U.size(); ++i) U[i] = (float)V[i];
+  soup_build_common(s, U, V, F, nv, nf, 0, tree_kind);
+}
+
+void antipodal_build(gwn_antipodal* a, const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* x0,
+                     hipStream_t st);
+void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no, double* W, hipStream_t st);
+
 }  // namespace gwn
 
 using namespace gwn;
@@ -408,6 +420,71 @@ int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type,
     GWN_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
     GWN_HIP(hipStreamSynchronize(st));
   });
+}
+
+// ---------------- antipodal method ----------------
+int gwn_antipodal_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* x0,
+                        gwn_antipodal** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(nv >= 0 && nf >= 0 && nf < (int64_t)0x7FFFFFFF && nv < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID,
+              "bad sizes");
+    ensure_device();
+    gwn_antipodal* a = new gwn_antipodal();
+    try {
+      antipodal_build(a, V, nv, F, nf, x0, nullptr);
+    } catch (...) {
+      delete a;
+      throw;
+    }
+    *out = a;
+  });
+}
+
+void gwn_antipodal_free(gwn_antipodal* a) { delete a; }
+
+int gwn_antipodal_info(const gwn_antipodal* a, int64_t* num_boundary_segments, int64_t* num_faces, double* x0) {
+  return guard([&] {
+    GWN_CHECK(a != nullptr, GWN_ERR_INVALID, "null scene");
+    if (num_boundary_segments) *num_boundary_segments = a->nseg;
+    if (num_faces) *num_faces = a->soup.nf;
+    if (x0)
+      for (int k = 0; k < 3; ++k) x0[k] = a->x0[k];
+  });
+}
+
+int gwn_antipodal_query(const gwn_antipodal* a, const double* O, int64_t no, double* W) {
+  return guard([&] {
+    GWN_CHECK(a != nullptr && no >= 0, GWN_ERR_INVALID, "bad arguments");
+    if (no == 0) return;
+    GWN_CHECK(O && W, GWN_ERR_INVALID, "null input");
+    hipStream_t st = nullptr;
+    DevBuf<double> dO, dW(no);
+    upload(dO, O, 3 * (size_t)no, st);
+    antipodal_query_device(a, dO.p, no, dW.p, st);
+    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no, double* W, void* stream) {
+  return guard([&] {
+    GWN_CHECK(a != nullptr && no >= 0, GWN_ERR_INVALID, "bad arguments");
+    if (no == 0) return;
+    GWN_CHECK(O && W, GWN_ERR_INVALID, "null input");
+    antipodal_query_device(a, O, no, W, static_cast<hipStream_t>(stream));
+  });
+}
+
+int gwn_winding_number_antipodal(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                 int64_t no, double* W) {
+  gwn_antipodal* a = nullptr;
+  int rc = gwn_antipodal_build(V, nv, F, nf, nullptr, &a);
+  if (rc != GWN_OK) return rc;
+  rc = gwn_antipodal_query(a, O, no, W);
+  gwn_antipodal_free(a);
+  return rc;
 }
 
 // ---------------- k nearest neighbours, point areas ----------------

@@ -199,6 +199,14 @@ struct gwn_soup {
   gwn::DevBuf<uint8_t> work;
 };
 
+// Antipodal-method scene: the soup's triangles, tree and slot boxes for the
+// rays, and the weighted open-boundary segments.
+struct gwn_antipodal {
+  gwn_soup soup;
+  gwn::DevBuf<double> seg;  // nseg x 8: a.xyz, b.xyz, w, 0
+  int64_t nseg = 0;
+  double x0[3] = {0, 0, 0};
+};
 struct gwn_cloud {
   int64_t n = 0, ncells = 0, nrec = 0;
   int order = 2, nterms = 39;

@@ -292,6 +292,49 @@ def octree(P):
     return point_indices, CH, CN, W
 
 
+class WindingNumberAntipodalScene:
+    """igl::WindingNumberAntipodalScene (WindingNumberAntipodalScene.h:48-256)
+    on the GPU: the mesh's weighted open boundary, and its own tree for the
+    signed ray crossings the reference asks of an intersector."""
+
+    def __init__(self, V, F, x0=None):
+        V = _rows3(V, "V", np.float64)
+        F = np.ascontiguousarray(np.asarray(F).reshape(-1, 3), dtype=np.int32)
+        xd = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64).reshape(3))
+        h = nat.vp()
+        check(lib().gwn_antipodal_build(V, V.shape[0], F, F.shape[0], None if xd is None else xd.ctypes.data,
+                                        C_byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                lib().gwn_antipodal_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def info(self):
+        import ctypes
+        nb, nf = ctypes.c_int64(), ctypes.c_int64()
+        x0 = np.zeros(3)
+        check(lib().gwn_antipodal_info(self._h, ctypes.byref(nb), ctypes.byref(nf), x0.ctypes.data))
+        return {"num_boundary_segments": nb.value, "num_faces": nf.value, "x0": x0}
+
+    def winding_number(self, O):
+        O = _rows3(O, "O", np.float64)
+        W = np.zeros(O.shape[0])
+        if O.shape[0]:
+            check(lib().gwn_antipodal_query(self._h, O, O.shape[0], W))
+        return W
+
+
+def winding_number_antipodal(V, F, O):
+    """igl::winding_number_antipodal(V,F,intersector,O,W) (the GPU counts the
+    ray crossings itself; no intersector is needed)."""
+    return WindingNumberAntipodalScene(V, F).winding_number(O)
+
+
 def octree_candidates(point_indices, CH):
     """The points igl::knn can return over an octree (knn.cpp:101-123): the
     first point of every non-empty leaf reachable from cell 0."""

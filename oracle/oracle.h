@@ -23,6 +23,13 @@
  *       triangle, a dense barycentric sampling bound, the single-tet golden
  *       value of tests/include/igl/signed_distance.cpp:4-44 (sqrt(1/12)) and
  *       sign checks on closed meshes.
+ *   oracle_winding_number_antipodal
+ *       equals the exact winding number (oracle_winding_number_bf) to 1e-9 on
+ *       closed meshes (where it is an integer, the boundary being empty),
+ *       open meshes, a single triangle and random non-manifold soups (the
+ *       reference's own test, tests/include/igl/embree/
+ *       winding_number_antipodal.cpp, asserts 1e-5 there), for the default
+ *       and for axis-aligned directions.
  *   oracle_knn / oracle_knn_bf
  *       the reference's golden vectors (tests/include/igl/knn.cpp: cube corners
  *       with a tie, and nv <= k), each against the other (best-first octree
@@ -141,6 +148,12 @@ void oracle_cloud_eval_f32(const float* P, const float* N, const float* A,
                            const oracle_octree_t* t, int order,
                            const float* CM, const float* R, const float* EC,
                            const float* Q, int64_t nq, double beta, float* WN, int nthreads);
+
+/* ---- antipodal method (WindingNumberAntipodalScene.h:48-256) ----
+ * x0 NULL: the reference's default direction.  *nseg_out (may be NULL)
+ * receives the number of weighted boundary segments. */
+int oracle_winding_number_antipodal(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* x0,
+                                    const double* O, int64_t no, double* W, int64_t* nseg_out, int nthreads);
 
 /* ---- k nearest neighbours (knn.cpp:24-127) ----
  * oracle_knn: the reference's best-first search over an FP64 octree of V

@@ -66,6 +66,8 @@ def lib():
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        L.oracle_winding_number_antipodal.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, C.c_void_p, _f64p, C.c_int64,
+                                                      _f64p, C.POINTER(C.c_int64), C.c_int]
         L.oracle_knn.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int64, C.POINTER(OctreeT), _i32p, C.c_int]
         L.oracle_knn_bf.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, _i32p,
                                     C.c_int]
@@ -474,3 +476,20 @@ def _point_areas(P, I, N, rows, Delaunay, QhullError, warnings):
             acc += _star_term(plane[f[X0]], plane[f[(X0 + 1) % 3]], plane[f[(X0 + 2) % 3]])
         A[i] = acc if np.isfinite(acc) else 0.0
     return A, T
+
+
+# ---------------- antipodal method ----------------
+def winding_number_antipodal(V, F, O, x0=None, nthreads=0):
+    """WindingNumberAntipodalScene restated (boundary term + brute-force signed
+    ray crossings).  Returns (W, number of boundary segments)."""
+    V = _c(V, np.float64).reshape(-1, 3)
+    F = _c(F, np.int32).reshape(-1, 3)
+    O = _c(O, np.float64).reshape(-1, 3)
+    W = np.zeros(O.shape[0])
+    ns = C.c_int64(0)
+    xd = None if x0 is None else _c(x0, np.float64)
+    rc = lib().oracle_winding_number_antipodal(V, V.shape[0], F, F.shape[0], None if xd is None else xd.ctypes.data,
+                                               O, O.shape[0], W, C.byref(ns), nthreads)
+    if rc != 0:
+        raise MemoryError("oracle antipodal failed")
+    return W, ns.value

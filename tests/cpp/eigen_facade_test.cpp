@@ -188,6 +188,38 @@ int main() {
     igl::copyleft::cgal::fast_winding_number(Pi, Ni, Qi, W2);
     if (W1.rows() != 2 || !(W1(0, 0) > 0.5) || !(std::fabs(W1(1, 0)) < 0.1) || W2(0, 0) != W1(0, 0)) bad |= 16384;
   }
+  // antipodal method on the single tet (closed: an integer count) and on one
+  // open triangle (tests/include/igl/embree/winding_number_antipodal.cpp:48-77)
+  {
+    struct NoIntersector {} none;
+    Eigen::MatrixXd Pa(2, 3);
+    Pa(0, 0) = 0.1, Pa(0, 1) = 0.1, Pa(0, 2) = 0.1;  // inside the tet
+    Pa(1, 0) = 2.0, Pa(1, 1) = 0.3, Pa(1, 2) = 0.2;  // outside
+    Eigen::VectorXd Wa;
+    igl::winding_number_antipodal(V, F, none, Pa, Wa);
+    if (Wa.rows() != 2 || Wa(0, 0) != 1.0 || Wa(1, 0) != 0.0) bad |= 32768;
+    igl::WindingNumberAntipodalScene<double> scene(V, F);
+    if (scene.num_boundary_segments() != 0 || scene.num_faces() != 4) bad |= 32768;
+    Eigen::Matrix<double, 1, 3> p;
+    p(0, 0) = 0.2, p(0, 1) = 0.2, p(0, 2) = 0.2;
+    if (scene.winding_number(none, p) != 1.0) bad |= 32768;
+    Eigen::MatrixXd Vt(3, 3);
+    Eigen::MatrixXi Ft(1, 3);
+    const double vt[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Vt(i, j) = vt[i][j];
+    Ft(0, 0) = 0, Ft(0, 1) = 1, Ft(0, 2) = 2;
+    Eigen::MatrixXd Pt(2, 3);
+    Pt(0, 0) = Pt(0, 1) = 1.0 / 3.0, Pt(0, 2) = 0.5;
+    Pt(1, 0) = Pt(1, 1) = 1.0 / 3.0, Pt(1, 2) = -0.5;
+    igl::WindingNumberAntipodalScene<double> tri(Vt, Ft);
+    Eigen::VectorXd Wt, Wx;
+    tri.winding_number(none, Pt, Wt);
+    igl::winding_number(Vt, Ft, Pt, Wx);
+    if (tri.num_boundary_segments() != 3 || !(Wt(0, 0) * Wt(1, 0) < 0.0) || std::fabs(Wt(0, 0) - Wx(0, 0)) > 1e-12 ||
+        std::fabs(Wt(1, 0) - Wx(1, 0)) > 1e-12)
+      bad |= 65536;
+  }
   std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
   return bad;
 }
