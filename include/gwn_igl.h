@@ -27,7 +27,11 @@
 //   igl::signed_distance(P,V,F,type,lower,upper,S,I,C,N)  signed_distance.h:61-71
 //   igl::signed_distance(P,V,F,type,S,I,C,N)          signed_distance.h:81-89
 //   igl::signed_distance_fast_winding_number(P,V,F,tree,fwn_bvh,S)
-//                                                     signed_distance.h (batch, .cpp:665-687)
+//                                                     signed_distance.h:276-289 (batch, .cpp:665-687)
+//   igl::signed_distance_fast_winding_number(q,V,F,tree,fwn_bvh)
+//                                                     signed_distance.h:300-311 (one point, .cpp:688-711)
+//   igl::signed_distance_winding_number(tree,V,F,hier,q[,s,sqrd,i,c])
+//                                                     signed_distance.h:211-242 (3D, .cpp:579-629)
 // (signed distance for the unsigned, winding-number and fast-winding-number
 // sign types; the pseudonormal type throws)
 // A program switches by including this header instead of the libigl ones and
@@ -431,6 +435,27 @@ void cloud_estimated_areas(const MP& P, const MN& N, const MQ& Q, int order, dou
     WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
 }
 
+// signed_distance.cpp:611-629 for one point q: closest point on the soup `h`
+// (squared distance recomputed from it, summed as the device sums it) and
+// s = 1 - 2 w with the exact winding number
+template <typename MV, typename MF, typename Mq>
+void signed_distance_wn_point(gwn_soup* h, const MV& V, const MF& F, const Mq& q, double& s, double& sqrd, int& i,
+                              double c[3]) {
+  const double p[3] = {static_cast<double>(q(0, 0)), static_cast<double>(q(0, 1)), static_cast<double>(q(0, 2))};
+  double S = 0.0;
+  int32_t I = -1;
+  check(gwn_signed_distance(h, p, 1, GWN_SIGN_UNSIGNED, std::numeric_limits<double>::min(),
+                            std::numeric_limits<double>::max(), &S, &I, c));
+  const double e0 = p[0] - c[0], e1 = p[1] - c[1], e2 = p[2] - c[2];
+  sqrd = (e0 * e0 + e1 * e1) + e2 * e2;
+  i = I;
+  const auto v = rows3<double>(V, "V");
+  const auto f = faces(F);
+  double w = 0.0;
+  check(gwn_exact_winding_number(v.data(), V.rows(), f.data(), F.rows(), p, 1, &w));
+  s = 1. - 2. * w;
+}
+
 // WindingNumberAntipodalScene.h:69-78: the scene (x0 NULL: the default direction)
 template <typename MV, typename MF>
 gwn_antipodal* antipodal_build(const MV& V, const MF& F, const double* x0) {
@@ -565,6 +590,65 @@ void signed_distance_fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, c
   gwn_detail::signed_distance_bvh(fwn_bvh.handle, P.derived(), GWN_SIGN_FAST_WINDING_NUMBER,
                                   std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), S.derived(),
                                   I, C);
+}
+
+// signed_distance.cpp:688-711: one point, sign from fwn_bvh (accuracy_scale 2)
+template <typename Derivedq, typename DerivedV, typename DerivedF, typename Tree>
+typename DerivedV::Scalar signed_distance_fast_winding_number(const Eigen::MatrixBase<Derivedq>& q,
+                                                              const Eigen::MatrixBase<DerivedV>& V,
+                                                              const Eigen::MatrixBase<DerivedF>& F, const Tree& tree,
+                                                              const FastWindingNumberBVH& fwn_bvh) {
+  (void)V;
+  (void)F;
+  (void)tree;
+  if (!fwn_bvh.handle) throw std::runtime_error("FastWindingNumberBVH not built");
+  const double p[3] = {static_cast<double>(q.derived()(0, 0)), static_cast<double>(q.derived()(0, 1)),
+                       static_cast<double>(q.derived()(0, 2))};
+  double S = 0.0, c[3];
+  int32_t I = -1;
+  gwn_detail::check(gwn_signed_distance(fwn_bvh.handle, p, 1, GWN_SIGN_FAST_WINDING_NUMBER,
+                                        std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), &S,
+                                        &I, c));
+  return static_cast<typename DerivedV::Scalar>(S);
+}
+
+// signed_distance.cpp:579-629 (3D): closest point and s = 1 - 2 w with the
+// exact winding number.  `tree` and `hier` stand for the reference's AABB and
+// WindingNumberAABB: a FastWindingNumberBVH built over (V, F) is used as the
+// closest-point tree; anything else (e.g. 0) makes the call build one.
+template <typename Tree, typename DerivedV, typename DerivedF, typename Hier, typename Derivedq, typename Scalar,
+          typename Derivedc>
+void signed_distance_winding_number(const Tree& tree, const Eigen::MatrixBase<DerivedV>& V,
+                                    const Eigen::MatrixBase<DerivedF>& F, const Hier& hier,
+                                    const Eigen::MatrixBase<Derivedq>& q, Scalar& s, Scalar& sqrd, int& i,
+                                    Eigen::PlainObjectBase<Derivedc>& c) {
+  (void)hier;
+  double sd = 0.0, d2 = 0.0, cc[3];
+  if constexpr (std::is_same<Tree, FastWindingNumberBVH>::value) {
+    if (!tree.handle) throw std::runtime_error("FastWindingNumberBVH not built");
+    gwn_detail::signed_distance_wn_point(tree.handle, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
+  } else {
+    FastWindingNumberBVH bvh;
+    gwn_detail::build(V.derived(), F.derived(), 0, bvh);
+    gwn_detail::signed_distance_wn_point(bvh.handle, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
+  }
+  s = static_cast<Scalar>(sd);
+  sqrd = static_cast<Scalar>(d2);
+  auto& cd = c.derived();
+  cd.resize(1, 3);
+  for (int j = 0; j < 3; ++j) cd(0, j) = static_cast<typename std::decay<decltype(cd(0, 0))>::type>(cc[j]);
+}
+
+template <typename Tree, typename DerivedV, typename DerivedF, typename Hier, typename Derivedq>
+typename DerivedV::Scalar signed_distance_winding_number(const Tree& tree, const Eigen::MatrixBase<DerivedV>& V,
+                                                         const Eigen::MatrixBase<DerivedF>& F, const Hier& hier,
+                                                         const Eigen::MatrixBase<Derivedq>& q) {
+  typedef typename DerivedV::Scalar Scalar;
+  Scalar s = 0, sqrd = 0;
+  int i = -1;
+  Eigen::Matrix<double, Eigen::Dynamic, Eigen::Dynamic> c;
+  signed_distance_winding_number(tree, V, F, hier, q, s, sqrd, i, c);
+  return s * std::sqrt(sqrd);
 }
 
 template <typename DerivedV, typename DerivedF>

@@ -44,6 +44,33 @@ int main() {
     const int tree = 0;  // the caller's AABB is not used
     igl::signed_distance_fast_winding_number(P, V, F, tree, bvh, S);
     if (S.rows() != 1 || std::fabs(S(0, 0) - Sexact) > 1e-7) bad |= 4;
+    // the single-point form (signed_distance.cpp:688-711)
+    Eigen::Matrix<double, 1, 3> q1;
+    q1(0, 0) = 0.5, q1(0, 1) = 0.5, q1(0, 2) = 0.5;
+    const double sq1 = igl::signed_distance_fast_winding_number(q1, V, F, tree, bvh);
+    if (std::fabs(sq1 - Sexact) > 1e-7) {
+      std::printf("signed_distance_fast_winding_number(q): %.17g\n", sq1);
+      bad |= 1 << 17;
+    }
+    // signed_distance_winding_number (signed_distance.cpp:579-629), with the
+    // bvh as the tree and with a tree built by the call
+    double s1 = 0, sqrd1 = 0;
+    int i1 = -1;
+    Eigen::MatrixXd c1;
+    igl::signed_distance_winding_number(bvh, V, F, 0, q1, s1, sqrd1, i1, c1);
+    // outside, w is zero up to the rounding of the solid angles: s = 1 - 2w
+    if (std::fabs(s1 - 1.0) > 1e-14 || std::fabs(sqrd1 - 1.0 / 12.0) > 1e-15 || i1 != 3 ||
+        std::fabs(c1(0, 0) - 1.0 / 3.0) > 1e-15) {
+      std::printf("signed_distance_winding_number: s %.17g sqrd %.17g i %d c %.17g\n", s1, sqrd1, i1, c1(0, 0));
+      bad |= 1 << 18;
+    }
+    Eigen::Matrix<double, 1, 3> q2;
+    q2(0, 0) = 0.1, q2(0, 1) = 0.1, q2(0, 2) = 0.2;  // inside: w = 1, s = -1
+    const double sq2 = igl::signed_distance_winding_number(0, V, F, 0, q2);
+    if (std::fabs(sq2 + 0.1) > 1e-14) {
+      std::printf("signed_distance_winding_number(q): %.17g\n", sq2);
+      bad |= 1 << 19;
+    }
     // the batch and single-point FWN overloads
     Eigen::MatrixXd Q(2, 3);
     Q(0, 0) = 0.1, Q(0, 1) = 0.1, Q(0, 2) = 0.1;
