@@ -77,7 +77,7 @@ const char* gwn_version(void);
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
  * "soup_traverse64", "exact", "cloud_traverse", "radix_sort", "closest_point",
- * "knn", "point_areas", "cloud_brute", "exact2d") with HIP events
+ * "knn", "point_areas", "cloud_brute", "exact2d", "curve_traverse") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises
  * on those events and returns the accumulated time and launch count. */
 int gwn_profile_enable(int on);
@@ -274,6 +274,48 @@ int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sig
  * V nv x 2, E ne x 2 (edges), O no x 2, W no; host buffers. */
 int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, int64_t ne, const double* O,
                                 int64_t no, double* W);
+
+/* ------------------------------------------------------------------------
+ * 2D fast winding number of a curve: UT_SubtendedAngle<float,float>
+ * (FastWindingNumberForSoups.h:6315-6370).  gwn_curve_build replaces
+ * UT_SubtendedAngle::init(nsegments, segment_points, npoints, positions,
+ * order) (7342-7604): P np x 2 float, E ne x 2 int32 (the segment point
+ * indices), order in {0,1,2}; tree_kind as for the soup (GWN_TREE_SAH4 is the
+ * reference topology, BVH<4>::init<BOX_AREA,S,2>).  The handle keeps its own
+ * device copies (the reference keeps the caller's pointers).
+ * gwn_curve_query replaces computeAngle(query_point, accuracy_scale)
+ * (7700-7828) for nq queries (Q nq x 2 float): angle[i] is the signed angle
+ * the curve subtends at Q[i] (the winding number is angle / 2 pi); an empty
+ * curve gives 0.  Concurrent queries on one handle must not overlap (the
+ * handle's query workspace is shared).
+ */
+typedef struct gwn_curve gwn_curve;
+typedef struct {
+  int64_t num_points, num_segments;
+  int64_t num_nodes, num_slots;
+  int32_t order, tree_kind, depth, reserved;
+  int64_t device_bytes;
+  float bbox[4]; /* min xy, max xy of the points */
+} gwn_curve_info;
+
+int gwn_curve_build(const float* P, int64_t np, const int32_t* E, int64_t ne, int order, int tree_kind,
+                    gwn_curve** out);
+void gwn_curve_free(gwn_curve* c);
+int gwn_curve_get_info(const gwn_curve* c, gwn_curve_info* info);
+int gwn_curve_query(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle);
+/* Device form: Q and angle are device pointers. */
+int gwn_curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
+                           void* stream);
+/* Introspection for parity checks: the node array (reference encoding, as
+ * gwn_soup_copy_nodes) and the per-node BoxData, 48 floats in the reference
+ * member order (7287-7320). */
+int gwn_curve_copy_nodes(const gwn_curve* c, uint32_t* out);
+int gwn_curve_copy_boxdata(const gwn_curve* c, float* out);
+/* Host-only (no GPU needed): the reference-topology tree of a curve, i.e. the
+ * segment boxes of UT_SubtendedAngle::init (7365-7390) followed by
+ * BVH<4>::init<BOX_AREA,S,2> (7392).  As gwn_sah4_tree. */
+int gwn_sah4_tree_2d(const float* P, int64_t np, const int32_t* E, int64_t ne, uint32_t* nodes, int64_t capacity,
+                     int64_t* num_nodes);
 
 /* ------------------------------------------------------------------------
  * Mesh files (read_triangle_mesh.h; readOFF.cpp, readOBJ.cpp,

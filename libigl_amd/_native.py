@@ -30,6 +30,12 @@ class SoupInfo(C.Structure):
                 ("reserved", C.c_int32), ("device_bytes", C.c_int64), ("bbox", C.c_float * 6)]
 
 
+class CurveInfo(C.Structure):
+    _fields_ = [("num_points", C.c_int64), ("num_segments", C.c_int64), ("num_nodes", C.c_int64),
+                ("num_slots", C.c_int64), ("order", C.c_int32), ("tree_kind", C.c_int32), ("depth", C.c_int32),
+                ("reserved", C.c_int32), ("device_bytes", C.c_int64), ("bbox", C.c_float * 4)]
+
+
 class MixedParams(C.Structure):
     _fields_ = [("near_dist", C.c_float), ("ambiguous_band", C.c_float)]
 
@@ -74,6 +80,14 @@ SIGNATURES = {
     "gwn_antipodal_query": (C.c_int, [vp, f64p, C.c_int64, f64p]),
     "gwn_antipodal_query_device": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
     "gwn_winding_number_antipodal": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
+    "gwn_curve_build": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
+    "gwn_curve_free": (None, [vp]),
+    "gwn_curve_get_info": (C.c_int, [vp, C.POINTER(CurveInfo)]),
+    "gwn_curve_query": (C.c_int, [vp, f32p, C.c_int64, C.c_float, f32p]),
+    "gwn_curve_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_float, vp, vp]),
+    "gwn_curve_copy_nodes": (C.c_int, [vp, u32p]),
+    "gwn_curve_copy_boxdata": (C.c_int, [vp, f32p]),
+    "gwn_sah4_tree_2d": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64)]),
     "gwn_knn_columns": (C.c_int, [C.c_int64, C.c_int64]),
     "gwn_knn": (C.c_int, [f64p, C.c_int64, f64p, C.c_int64, C.c_int64, vp, C.c_int64, i32p]),
     "gwn_knn_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, C.c_int64, vp, C.c_int64, vp, vp]),

@@ -334,6 +334,149 @@ int gwn_soup_copy_boxdata(const gwn_soup* s, float* out) {
   });
 }
 
+// ---------------- 2D curve (UT_SubtendedAngle) ----------------
+static void check_segments(const int32_t* E, int64_t ne, int64_t np) {
+  for (int64_t i = 0; i < 2 * ne; ++i)
+    GWN_CHECK(E[i] >= 0 && E[i] < np, GWN_ERR_INVALID, "segment index out of range at E[" + std::to_string(i) + "]");
+}
+
+// segment boxes of UT_SubtendedAngle::init (7365-7390), 6 floats each (z = 0)
+static std::vector<float> segment_boxes(const float* P, const int32_t* E, int64_t ne) {
+  std::vector<float> boxes(6 * (size_t)ne, 0.0f);
+  auto hmin = [](float a, float b) { return (a > b) ? b : a; };
+  auto hmax = [](float a, float b) { return (a < b) ? b : a; };
+  for (int64_t i = 0; i < ne; ++i)
+    for (int a = 0; a < 2; ++a) {
+      const float u = P[2 * (int64_t)E[2 * i] + a], v = P[2 * (int64_t)E[2 * i + 1] + a];
+      boxes[6 * i + 2 * a] = hmin(u, v);
+      boxes[6 * i + 2 * a + 1] = hmax(u, v);
+    }
+  return boxes;
+}
+
+int gwn_curve_build(const float* P, int64_t np, const int32_t* E, int64_t ne, int order, int tree_kind,
+                    gwn_curve** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(order >= 0 && order <= 2, GWN_ERR_INVALID, "order must be 0, 1 or 2");
+    GWN_CHECK(tree_kind == GWN_TREE_SAH4 || tree_kind == GWN_TREE_LBVH, GWN_ERR_INVALID, "unknown tree kind");
+    GWN_CHECK(np >= 0 && ne >= 0 && ne < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad sizes");
+    GWN_CHECK(np == 0 || P != nullptr, GWN_ERR_INVALID, "null points");
+    if (ne > 0) {
+      GWN_CHECK(E != nullptr && np > 0, GWN_ERR_INVALID, "null segments/points");
+      check_segments(E, ne, np);
+    }
+    ensure_device();
+    hipStream_t st = nullptr;
+    gwn_curve* c = new gwn_curve();
+    try {
+      c->np = np;
+      c->ne = ne;
+      c->order = order;
+      c->tree_kind = tree_kind;
+      float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+      for (int64_t i = 0; i < np; ++i)
+        for (int a = 0; a < 2; ++a) {
+          const float v = P[2 * i + a];
+          if (std::isfinite(v)) {
+            lo[a] = std::min(lo[a], v);
+            hi[a] = std::max(hi[a], v);
+          }
+        }
+      for (int a = 0; a < 2; ++a) {
+        c->bbox[a] = std::isfinite(lo[a]) ? lo[a] : 0.0f;
+        c->bbox[2 + a] = std::isfinite(hi[a]) ? hi[a] : 0.0f;
+      }
+      upload(c->P, P, 2 * (size_t)np, st);
+      upload(c->E, E, 2 * (size_t)ne, st);
+      curve_build_device(c, st);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+void gwn_curve_free(gwn_curve* c) { delete c; }
+
+int gwn_curve_get_info(const gwn_curve* c, gwn_curve_info* info) {
+  return guard([&] {
+    GWN_CHECK(c && info, GWN_ERR_INVALID, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->num_points = c->np;
+    info->num_segments = c->ne;
+    info->num_nodes = c->nnodes;
+    info->num_slots = c->nslots;
+    info->order = c->order;
+    info->tree_kind = c->tree_kind;
+    info->depth = c->depth;
+    info->device_bytes = (int64_t)(c->P.bytes() + c->E.bytes() + c->seg.bytes() + c->boxes.bytes() +
+                                   c->nodes.bytes() + c->boxdata.bytes() + c->slots.bytes() + c->work.bytes());
+    std::memcpy(info->bbox, c->bbox, sizeof(c->bbox));
+  });
+}
+
+int gwn_curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
+                           void* stream) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null curve");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && angle), GWN_ERR_INVALID, "null query/output");
+    curve_query_device(c, Q, nq, accuracy_scale, angle, (hipStream_t)stream);
+  });
+}
+
+int gwn_curve_query(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle) {
+  return guard([&] {
+    GWN_CHECK(c != nullptr, GWN_ERR_INVALID, "null curve");
+    GWN_CHECK(nq >= 0 && nq < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && angle), GWN_ERR_INVALID, "null query/output");
+    if (nq == 0) return;
+    hipStream_t st = nullptr;
+    DevBuf<float> dQ, dA(nq);
+    upload(dQ, Q, 2 * (size_t)nq, st);
+    curve_query_device(c, dQ.p, nq, accuracy_scale, dA.p, st);
+    GWN_HIP(hipMemcpyAsync(angle, dA.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_curve_copy_nodes(const gwn_curve* c, uint32_t* out) {
+  return guard([&] {
+    GWN_CHECK(c && out, GWN_ERR_INVALID, "null argument");
+    if (c->nnodes)
+      GWN_HIP(hipMemcpy(out, c->nodes.p, sizeof(uint32_t) * 4 * (size_t)c->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwn_curve_copy_boxdata(const gwn_curve* c, float* out) {
+  return guard([&] {
+    GWN_CHECK(c && out, GWN_ERR_INVALID, "null argument");
+    if (c->nnodes)
+      GWN_HIP(hipMemcpy(out, c->boxdata.p, sizeof(float) * 48 * (size_t)c->nnodes, hipMemcpyDeviceToHost));
+  });
+}
+
+int gwn_sah4_tree_2d(const float* P, int64_t np, const int32_t* E, int64_t ne, uint32_t* nodes, int64_t capacity,
+                     int64_t* num_nodes) {
+  return guard([&] {
+    GWN_CHECK(num_nodes != nullptr, GWN_ERR_INVALID, "null node count");
+    GWN_CHECK(np >= 0 && ne >= 0 && ne < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad sizes");
+    if (ne > 0) {
+      GWN_CHECK(P && E, GWN_ERR_INVALID, "null curve");
+      check_segments(E, ne, np);
+    }
+    const std::vector<float> boxes = segment_boxes(P, E, ne);
+    std::vector<uint32_t> out;
+    sah4_build(boxes.data(), (uint32_t)ne, out, 2);
+    *num_nodes = (int64_t)(out.size() / 4);
+    GWN_CHECK(nodes == nullptr || capacity >= *num_nodes, GWN_ERR_INVALID, "node capacity too small");
+    if (nodes && !out.empty()) std::memcpy(nodes, out.data(), sizeof(uint32_t) * out.size());
+  });
+}
+
 // ---------------- 2D exact winding number ----------------
 int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, int64_t ne, const double* O,
                                 int64_t no, double* W) {

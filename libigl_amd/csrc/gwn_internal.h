@@ -10,6 +10,7 @@
 #include "../../include/gwn.h"
 
 struct gwn_soup;
+struct gwn_curve;
 
 namespace gwn {
 
@@ -123,7 +124,9 @@ constexpr uint32_t NODE_INTERNAL = 0x80000000u;
 
 // Host build of the reference topology (UT::BVH<4>::init<BOX_AREA>).
 // boxes: nf x 6 floats (xmin,xmax,ymin,ymax,zmin,zmax); returns nodes (4 per node).
-void sah4_build(const float* boxes, uint32_t nboxes, std::vector<uint32_t>& nodes);
+// naxes = 2 builds the tree of 2D boxes (z extents ignored), as
+// UT_SubtendedAngle::init does (FastWindingNumberForSoups.h:7392).
+void sah4_build(const float* boxes, uint32_t nboxes, std::vector<uint32_t>& nodes, int naxes = 3);
 
 // ---- radix sort (LDS), device ----
 // Stable sort of (keys, values) by key; keys uint32 or uint64 with the low
@@ -153,6 +156,10 @@ struct Levels {
   int maxdepth = 0;
 };
 void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st);
+// First slot of every node in the rope (depth-first slot) layout, given the
+// slot count of each node's subtree (nsub); first[0] = 0.
+void rope_first_slots(const Levels& L, const uint32_t* nodes, const uint32_t* nsub, uint32_t* first,
+                      hipStream_t st);
 // soup queries (soup.hip)
 size_t soup_query_work_bytes(int64_t nq);
 void soup_ensure_fp64(gwn_soup* s, hipStream_t st);
@@ -170,6 +177,10 @@ void point_areas_device(const double* P, int64_t n, const int32_t* I, int64_t k,
                         double* T, hipStream_t st);
 int64_t distinct_points_device(const double* V, int64_t n, DevBuf<int32_t>& cand, hipStream_t st);
 void to_float_device(const double* x, int64_t n, float* y, hipStream_t st);
+// 2D curve (curve.hip): device setup + tree + moments, and queries
+void curve_build_device(gwn_curve* c, hipStream_t st);
+void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
+                        hipStream_t st);
 // traversal kernel: WALK_WAVE (wave-coherent, batch queries) or WALK_LANE
 // (per-lane, used by the serving engine); bitwise the same results
 constexpr int WALK_WAVE = 0, WALK_LANE = 1;
@@ -197,6 +208,24 @@ struct gwn_soup {
   gwn_mixed_params mixed{1e-3f, 0.05f};
   // per-handle scratch for queries (grow-only)
   gwn::DevBuf<uint8_t> work;
+};
+
+// 2D curve (UT_SubtendedAngle<float,float>): segments, the 4-wide tree over
+// their boxes, the reference BoxData and the rope-ordered slot records.
+struct gwn_curve {
+  int64_t np = 0, ne = 0;
+  int order = 2, tree_kind = 0, depth = 0;
+  float bbox[4] = {0, 0, 0, 0};  // min xy, max xy of the points
+  gwn::DevBuf<float> P;          // np x 2
+  gwn::DevBuf<int32_t> E;        // ne x 2
+  gwn::DevBuf<float4> seg;       // ne: a.xy, b.xy
+  gwn::DevBuf<float> boxes;      // ne x 6 segment boxes (z = 0)
+  gwn::DevBuf<uint32_t> nodes;   // nnodes x 4
+  int64_t nnodes = 0;
+  gwn::DevBuf<float> boxdata;    // nnodes x 48, reference BoxData order
+  gwn::DevBuf<float4> slots;     // nslots x 3 (rope layout)
+  int64_t nslots = 0;
+  gwn::DevBuf<uint8_t> work;     // per-handle query scratch (grow-only)
 };
 
 // Antipodal-method scene: the soup's triangles, tree and slot boxes for the

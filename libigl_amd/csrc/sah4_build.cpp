@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "gwn_internal.h"
@@ -48,8 +49,11 @@ struct Box {
       hi[a] = (hi[a] > o.hi[a]) ? hi[a] : o.hi[a];
     }
   }
-  float half_area() const {
-    const float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+  // Box::half_surface_area, NAXES == 2 (4071-4075) or 3 (4076-4080)
+  float half_area(int naxes) const {
+    const float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1];
+    if (naxes == 2) return d0 + d1;
+    const float d2 = hi[2] - lo[2];
     return d0 * d1 + d1 * d2 + d2 * d0;
   }
   float centre2(uint32_t a) const { return lo[a] + hi[a]; }
@@ -66,12 +70,24 @@ struct Box {
 
 class Builder {
  public:
-  Builder(const std::vector<Box>& boxes, std::vector<uint32_t>& out) : boxes_(boxes), out_(out) {}
+  Builder(const std::vector<Box>& boxes, std::vector<uint32_t>& out, int naxes)
+      : boxes_(boxes), out_(out), naxes_(naxes) {}
 
   void run(std::vector<uint32_t>& idx) {
     const uint32_t n = (uint32_t)idx.size();
     Box full = boxes_[idx[0]];
     for (uint32_t i = 1; i < n; ++i) full.grow(boxes_[idx[i]]);
+    // The root bound is computeFullBoundingBox (4856-4913) over the kept
+    // boxes.  From 8192 boxes on, on a host with more than one hardware
+    // thread, its parallel branch folds the boxes into per-thread boxes that
+    // UT_Array::setSize zero-fills (a POD Box, 2342-2360), so the root bound
+    // also contains the origin.  The root split depends on it.
+    if (n >= 2 * 4096 && std::thread::hardware_concurrency() > 1) {
+      for (int a = 0; a < 3; ++a) {
+        full.lo[a] = (0.0f < full.lo[a]) ? 0.0f : full.lo[a];
+        full.hi[a] = (0.0f > full.hi[a]) ? 0.0f : full.hi[a];
+      }
+    }
     out_.assign(4, NODE_EMPTY);
     build_node(0, full, idx.data(), n);
   }
@@ -79,6 +95,7 @@ class Builder {
  private:
   const std::vector<Box>& boxes_;
   std::vector<uint32_t>& out_;
+  const int naxes_;
 
   float c2(uint32_t i, uint32_t axis) const { return boxes_[i].centre2(axis); }
 
@@ -148,7 +165,7 @@ class Builder {
 
     uint32_t axis = 0;
     float len = bound.hi[0] - bound.lo[0];
-    for (uint32_t a = 1; a < 3; ++a) {
+    for (uint32_t a = 1; a < (uint32_t)naxes_; ++a) {
       const float l = bound.hi[a] - bound.lo[a];
       if (l > len) {
         axis = a;
@@ -178,7 +195,7 @@ class Builder {
         part[side].grow(local[k + 1]);
         ++count[side];
       }
-      const float cost = part[0].half_area() * (float)count[0] + part[1].half_area() * (float)count[1];
+      const float cost = part[0].half_area(naxes_) * (float)count[0] + part[1].half_area(naxes_) * (float)count[1];
       if (best == 0 || cost < best_cost) {
         best = bits;
         best_cost = cost;
@@ -218,9 +235,9 @@ class Builder {
       right[i - 1] = acc;
     }
     uint32_t best = 0;
-    float best_cost = left[0].half_area() + right[0].half_area() * (float)(n - 1);
+    float best_cost = left[0].half_area(naxes_) + right[0].half_area(naxes_) * (float)(n - 1);
     for (uint32_t s = 1; s < ns; ++s) {
-      const float cost = left[s].half_area() * (float)(s + 1) + right[s].half_area() * (float)(n - (s + 1));
+      const float cost = left[s].half_area(naxes_) * (float)(s + 1) + right[s].half_area(naxes_) * (float)(n - (s + 1));
       if (cost < best_cost) {
         best = s;
         best_cost = cost;
@@ -274,7 +291,7 @@ class Builder {
     for (uint32_t s = 0; s < kSplits; ++s) {
       const uint32_t lc = left_count[s];
       if (lc < min_count || lc > max_count) continue;
-      const float cost = (float)lc * left[s].half_area() + (float)(n - lc) * right[s].half_area();
+      const float cost = (float)lc * left[s].half_area(naxes_) + (float)(n - lc) * right[s].half_area(naxes_);
       if (cost < best_cost) {
         best_cost = cost;
         best = (int)s;
@@ -320,7 +337,7 @@ class Builder {
     part[0] = idx;
     part[2] = idx + n;
     part[1] = split(bound, idx, n, part_box);
-    float area[kN] = {part_box[0].half_area(), part_box[1].half_area(), 0, 0};
+    float area[kN] = {part_box[0].half_area(naxes_), part_box[1].half_area(naxes_), 0, 0};
     for (uint32_t np = 2; np < kN; ++np) {
       int pick = -1;
       float pick_cost = 0.0f;
@@ -344,8 +361,8 @@ class Builder {
       }
       const Box parent = part_box[c];
       part[c + 1] = split(parent, b, (uint32_t)(e - b), &part_box[c]);
-      area[c] = part_box[c].half_area();
-      area[c + 1] = part_box[c + 1].half_area();
+      area[c] = part_box[c].half_area(naxes_);
+      area[c + 1] = part_box[c + 1].half_area(naxes_);
     }
   }
 
@@ -372,7 +389,7 @@ class Builder {
 
 }  // namespace
 
-void sah4_build(const float* boxes6, uint32_t nboxes, std::vector<uint32_t>& nodes) {
+void sah4_build(const float* boxes6, uint32_t nboxes, std::vector<uint32_t>& nodes, int naxes) {
   nodes.clear();
   if (nboxes == 0) return;
   std::vector<Box> boxes(nboxes);
@@ -389,7 +406,7 @@ void sah4_build(const float* boxes6, uint32_t nboxes, std::vector<uint32_t>& nod
     if (!boxes[i].non_finite()) idx.push_back(i);
   if (idx.empty()) return;
   nodes.reserve((size_t)4 * (nboxes / 2 + nboxes / 6 + 4));
-  Builder(boxes, nodes).run(idx);
+  Builder(boxes, nodes, naxes).run(idx);
 }
 
 }  // namespace gwn

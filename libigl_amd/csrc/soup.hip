@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "device_math.h"
 #include "gwn_internal.h"
 #include "soup_moments.h"
 
@@ -368,35 +369,7 @@ __device__ __forceinline__ T tri_solid_angle(T ax, T ay, T az, T bx, T by, T bz,
   return T(2) * atan2(num, den);
 }
 
-// ---- FP32 fast path of the traversal ----
-// atan on [0,1] as a * P(a^2), P of degree 8: a weighted least-squares /
-// Lawson fit to atan(x)/x on 4000 Chebyshev nodes (computed for this
-// library); max error 1.2e-8 in exact arithmetic, ~1e-7 rad when evaluated in
-// FP32 (comparable to a correctly rounded atan2f).
-__device__ __forceinline__ float atan01(float a) {
-  const float s = a * a;
-  float p = 0.0028501534834504128f;
-  p = fmaf(p, s, -0.016069747507572174f);
-  p = fmaf(p, s, 0.04269346594810486f);
-  p = fmaf(p, s, -0.07504474371671677f);
-  p = fmaf(p, s, 0.10641028732061386f);
-  p = fmaf(p, s, -0.14203672111034393f);
-  p = fmaf(p, s, 0.1999262422323227f);
-  p = fmaf(p, s, -0.33333075046539307f);
-  p = fmaf(p, s, 1.0f);
-  return a * p;
-}
-
-// atan2 for y != 0 (the caller returns early on a zero numerator).
-__device__ __forceinline__ float atan2_fast(float y, float x) {
-  const float ax = fabsf(x), ay = fabsf(y);
-  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-  float r = atan01(mn * __builtin_amdgcn_rcpf(mx));
-  if (ay > ax) r = 1.57079632679489662f - r;
-  if (x < 0.0f) r = 3.14159265358979324f - r;
-  return copysignf(r, y);
-}
-
+// ---- FP32 fast path of the traversal (atan2_fast: device_math.h) ----
 // UTsignedSolidAngleTri (6070-6110) with hardware reciprocal square roots for
 // the normalisation; same zero-length and zero-numerator rules.
 __device__ __forceinline__ float tri_solid_angle_fast(float ax, float ay, float az, float bx, float by, float bz,
@@ -750,6 +723,19 @@ void compute_levels(const uint32_t* nodes, int64_t nn, Levels& L, hipStream_t st
   GWN_HIP(hipStreamSynchronize(st));
 }
 
+// First slot of every node in the rope layout, top-down from the root's 0.
+void rope_first_slots(const Levels& L, const uint32_t* nodes, const uint32_t* nsub, uint32_t* first,
+                      hipStream_t st) {
+  GWN_HIP(hipMemsetAsync(first, 0, sizeof(uint32_t), st));
+  for (int d = 0; d <= L.maxdepth; ++d) {
+    const int64_t cnt = L.offset[d + 1] - L.offset[d];
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_positions, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt, nodes,
+                       nsub, first);
+    check_launch("k_positions");
+  }
+}
+
 template <typename T>
 static void upward(gwn_soup* s, const Levels& L, T* boxdata, DevBuf<uint32_t>& nsub, hipStream_t st) {
   const int64_t nn = s->nnodes;
@@ -778,14 +764,7 @@ void soup_finish_build(gwn_soup* s, hipStream_t st) {
   upward<float>(s, L, s->boxdata.p, nsub, st);
   uint32_t total = 0;
   GWN_HIP(hipMemcpyAsync(&total, nsub.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  GWN_HIP(hipMemsetAsync(first.p, 0, sizeof(uint32_t), st));
-  for (int d = 0; d <= L.maxdepth; ++d) {
-    const int64_t cnt = L.offset[d + 1] - L.offset[d];
-    if (!cnt) continue;
-    hipLaunchKernelGGL(k_positions, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt,
-                       s->nodes.p, nsub.p, first.p);
-    check_launch("k_positions");
-  }
+  rope_first_slots(L, s->nodes.p, nsub.p, first.p, st);
   GWN_HIP(hipStreamSynchronize(st));
   s->nslots = total;
   s->slots.alloc(6 * (size_t)total);
@@ -803,14 +782,7 @@ void soup_ensure_fp64(gwn_soup* s, hipStream_t st) {
   DevBuf<double> bd64(92 * nn);
   DevBuf<uint32_t> nsub(nn), first(nn);
   upward<double>(s, L, bd64.p, nsub, st);
-  GWN_HIP(hipMemsetAsync(first.p, 0, sizeof(uint32_t), st));
-  for (int d = 0; d <= L.maxdepth; ++d) {
-    const int64_t cnt = L.offset[d + 1] - L.offset[d];
-    if (!cnt) continue;
-    hipLaunchKernelGGL(k_positions, dim3(div_up(cnt, 256)), dim3(256), 0, st, L.order.p + L.offset[d], cnt,
-                       s->nodes.p, nsub.p, first.p);
-    check_launch("k_positions");
-  }
+  rope_first_slots(L, s->nodes.p, nsub.p, first.p, st);
   DevBuf<double> s64(24 * (size_t)s->nslots);
   hipLaunchKernelGGL(k_write_slots64, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, bd64.p, first.p,
                      nsub.p, s64.p);

@@ -12,6 +12,8 @@ defaults and output shapes), each call going through the C ABI of libgwn.so:
   fast_winding_number(P, N, A, point_indices, CH, order) -> (CM, R, EC)   .h:44-53
   fast_winding_number(P, N, A, point_indices, CH, CM, R, EC, Q, beta)     .h:88-99
   octree(P) -> (point_indices, CH, CN, W)       octree.h
+  SubtendedAngle(P, E, order).compute_angle(Q, accuracy_scale)
+                                                UT_SubtendedAngle (FastWindingNumberForSoups.h:6315-6370)
 
 Errors: the reference asserts; here invalid input raises GWNError (or
 ValueError for shape errors caught before the C ABI).
@@ -26,7 +28,7 @@ from ._native import (PREC_FP32, PREC_FP64, PREC_MIXED, SIGN_DEFAULT, SIGN_FAST_
 
 DBL_MIN, DBL_MAX = 2.2250738585072014e-308, 1.7976931348623157e308
 
-__all__ = ["FastWindingNumberBVH", "PointCloudFWN", "fast_winding_number", "winding_number", "octree",
+__all__ = ["FastWindingNumberBVH", "PointCloudFWN", "SubtendedAngle", "fast_winding_number", "winding_number", "octree",
            "PREC_FP32", "PREC_MIXED", "PREC_FP64", "TREE_SAH4", "TREE_LBVH", "GWNError"]
 
 
@@ -327,6 +329,82 @@ class WindingNumberAntipodalScene:
         if O.shape[0]:
             check(lib().gwn_antipodal_query(self._h, O, O.shape[0], W))
         return W
+
+
+def _rows2(X, name, dtype):
+    X = np.asarray(X)
+    if X.ndim == 1 and X.shape[0] == 2:
+        X = X.reshape(1, 2)
+    if X.ndim != 2 or X.shape[1] != 2:
+        raise ValueError(f"{name} should be n x 2")
+    return np.ascontiguousarray(X, dtype=dtype)
+
+
+class SubtendedAngle:
+    """UT_SubtendedAngle<float,float> (FastWindingNumberForSoups.h:6315-6370):
+    the 2D fast winding number of a curve given as segments.  The constructor
+    is init(nsegments, segment_points, npoints, positions, order) (7342-7604)
+    with P np x 2 and E ne x 2; compute_angle(Q, accuracy_scale) is
+    computeAngle (7700-7828) for each row of Q (float angles in radians; the
+    winding number is angle / 2 pi)."""
+
+    def __init__(self, P, E, order=2, tree=TREE_SAH4):
+        P = _rows2(P, "P", np.float32)
+        E = _rows2(np.asarray(E).reshape(-1, 2), "E", np.int32)
+        h = nat.vp()
+        check(lib().gwn_curve_build(P, P.shape[0], E, E.shape[0], int(order), tree, C_byref(h)))
+        self._h = h.value
+        self.order = order
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None:
+                lib().gwn_curve_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def info(self):
+        i = nat.CurveInfo()
+        check(lib().gwn_curve_get_info(self._h, C_byref(i)))
+        return i
+
+    def nodes(self):
+        n = self.info().num_nodes
+        out = np.empty((n, 4), np.uint32)
+        if n:
+            check(lib().gwn_curve_copy_nodes(self._h, out))
+        return out
+
+    def boxdata(self):
+        n = self.info().num_nodes
+        out = np.empty((n, 48), np.float32)
+        if n:
+            check(lib().gwn_curve_copy_boxdata(self._h, out))
+        return out
+
+    def compute_angle(self, Q, accuracy_scale=2.0):
+        Q = _rows2(Q, "Q", np.float32)
+        out = np.empty(Q.shape[0], np.float32)
+        if Q.shape[0]:
+            check(lib().gwn_curve_query(self._h, Q, Q.shape[0], float(accuracy_scale), out))
+        return out
+
+    def query_device(self, Q_ptr, nq, angle_ptr, accuracy_scale=2.0, stream=None):
+        """Device pointers (e.g. torch tensor data_ptr()): Q float nq x 2, angle float nq."""
+        check(lib().gwn_curve_query_device(self._h, Q_ptr, nq, float(accuracy_scale), angle_ptr, stream))
+
+
+def sah4_tree_2d(P, E):
+    """Host-only reference-topology tree of a curve (BVH<4>::init<BOX_AREA,S,2>)."""
+    import ctypes
+    P = _rows2(P, "P", np.float32)
+    E = _rows2(np.asarray(E).reshape(-1, 2), "E", np.int32)
+    n = ctypes.c_int64()
+    check(lib().gwn_sah4_tree_2d(P, P.shape[0], E, E.shape[0], None, 0, ctypes.byref(n)))
+    out = np.empty((n.value, 4), np.uint32)
+    check(lib().gwn_sah4_tree_2d(P, P.shape[0], E, E.shape[0], out.ctypes.data, n.value, ctypes.byref(n)))
+    return out
 
 
 def winding_number_antipodal(V, F, O):

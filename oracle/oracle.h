@@ -18,6 +18,10 @@
  *   oracle_soup_*
  *       bit-exact against the compiled reference UT_SolidAngle (oracle/_ref):
  *       node arrays, BoxData and per-query Omega; plus closed-mesh invariants.
+ *   oracle_curve_*
+ *       bit-exact against the compiled reference UT_SubtendedAngle
+ *       (oracle/_ref): node arrays, BoxData and per-query angles; plus closed
+ *       curves (2 pi inside, 0 outside) and the exact segment angle sum.
  *   oracle_point_tri_sqrdist / oracle_signed_distance
  *       closed-form closest points in every Voronoi region of a right
  *       triangle, a dense barycentric sampling bound, the single-tet golden
@@ -90,6 +94,31 @@ void oracle_soup_query(const oracle_soup* s, const double* Q, int64_t nq, float 
  * calls), accepted far-field child slots and exact triangles. */
 void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale,
                        int64_t* visits, int64_t* far_accepted, int64_t* exact_tris);
+
+/* BVH::init<BOX_AREA> (FastWindingNumberForSoups.h:4579-5944) over nboxes
+ * boxes of 6 floats (x0 x1 y0 y1 z0 z1; 2D boxes carry z0 = z1 = 0 with
+ * naxes = 2).  Returns the node count; *nodes (4 uint32 per node, malloc'ed,
+ * free()d by the caller) is NULL for an empty tree. */
+uint32_t oracle_bvh4_build(const float* boxes6, uint32_t nboxes, int naxes, uint32_t** nodes);
+
+/* ---- 2D curve: UT_SubtendedAngle<float,float> over UT::BVH<4> ---- */
+typedef struct oracle_curve oracle_curve;
+/* UT_SubtendedAngle::init (FastWindingNumberForSoups.h:7342-7604): P np x 2,
+ * E ne x 2 segment point indices */
+oracle_curve* oracle_curve_build(const float* P, int32_t np, const int32_t* E, int32_t ne, int order);
+void oracle_curve_free(oracle_curve* c);
+int32_t oracle_curve_num_nodes(const oracle_curve* c);
+void oracle_curve_nodes(const oracle_curve* c, uint32_t* out);
+/* BoxData, 48 floats per node in the reference struct order (7287-7320) */
+void oracle_curve_boxdata(const oracle_curve* c, float* out);
+/* computeAngle (7700-7828) for each query (Q nq x 2 float): the signed
+ * subtended angle (the winding number is angle / 2 pi) */
+void oracle_curve_angle(const oracle_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
+                        int nthreads);
+/* Counted visits for the roofline: node pre() calls, accepted far-field child
+ * slots and exact segments of one query. */
+void oracle_curve_count(const oracle_curve* c, const float* q, float accuracy_scale, int64_t* visits,
+                        int64_t* far_accepted, int64_t* exact_segments);
 
 /* ---- signed distance (signed_distance.cpp:28-140) ---- */
 /* closest point of triangle (a,b,c) to p, point_simplex_squared_distance.cpp:43-114;

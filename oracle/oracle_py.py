@@ -66,6 +66,16 @@ def lib():
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
+        L.oracle_curve_build.restype = C.c_void_p
+        L.oracle_curve_build.argtypes = [_f32p, C.c_int32, _i32p, C.c_int32, C.c_int]
+        L.oracle_curve_free.argtypes = [C.c_void_p]
+        L.oracle_curve_num_nodes.restype = C.c_int32
+        L.oracle_curve_num_nodes.argtypes = [C.c_void_p]
+        L.oracle_curve_nodes.argtypes = [C.c_void_p, _u32p]
+        L.oracle_curve_boxdata.argtypes = [C.c_void_p, _f32p]
+        L.oracle_curve_angle.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_float, _f32p, C.c_int]
+        L.oracle_curve_count.argtypes = [C.c_void_p, _f32p, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64)]
         L.oracle_winding_number_antipodal.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, C.c_void_p, _f64p, C.c_int64,
                                                       _f64p, C.POINTER(C.c_int64), C.c_int]
         L.oracle_knn.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int64, C.POINTER(OctreeT), _i32p, C.c_int]
@@ -103,6 +113,14 @@ def ref():
         R.ref_solid_angle_tri_f64.restype = C.c_double
         R.ref_solid_angle_tri_f64.argtypes = [_f64p, _f64p, _f64p, _f64p]
         R.ref_num_threads.restype = C.c_int
+        R.ref_curve_build.restype = C.c_void_p
+        R.ref_curve_build.argtypes = [_f32p, C.c_int, _i32p, C.c_int, C.c_int]
+        R.ref_curve_free.argtypes = [C.c_void_p]
+        R.ref_curve_num_nodes.restype = C.c_int
+        R.ref_curve_num_nodes.argtypes = [C.c_void_p]
+        R.ref_curve_nodes.argtypes = [C.c_void_p, _u32p]
+        R.ref_curve_boxdata.argtypes = [C.c_void_p, _f32p]
+        R.ref_curve_angle.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_float, _f32p]
         _ref = R
     return _ref
 
@@ -240,6 +258,69 @@ class RefSoup:
         W = np.empty(Q.shape[0], np.float64)
         ref().ref_soup_query(self.h, Q, Q.shape[0], accuracy_scale, W)
         return W
+
+
+# ---------------- 2D curve ----------------
+class _CurveBase:
+    _prefix = None
+
+    def _api(self):
+        raise NotImplementedError
+
+    def __init__(self, P, E, order=2):
+        self.P = _c(P, np.float32).reshape(-1, 2)
+        self.E = _c(E, np.int32).reshape(-1, 2)
+        self.h = getattr(self._api(), self._prefix + "build")(self.P, self.P.shape[0], self.E, self.E.shape[0], order)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self._api(), self._prefix + "free")(self.h)
+            self.h = None
+
+    def nodes(self):
+        n = getattr(self._api(), self._prefix + "num_nodes")(self.h)
+        out = np.empty((n, 4), np.uint32)
+        getattr(self._api(), self._prefix + "nodes")(self.h, out)
+        return out
+
+    def boxdata(self):
+        n = getattr(self._api(), self._prefix + "num_nodes")(self.h)
+        out = np.empty((n, 48), np.float32)
+        getattr(self._api(), self._prefix + "boxdata")(self.h, out)
+        return out
+
+
+class Curve(_CurveBase):
+    """Oracle UT_SubtendedAngle<float,float> (restated)."""
+    _prefix = "oracle_curve_"
+
+    def _api(self):
+        return lib()
+
+    def angle(self, Q, accuracy_scale=2.0, nthreads=0):
+        Q = _c(Q, np.float32).reshape(-1, 2)
+        out = np.empty(Q.shape[0], np.float32)
+        lib().oracle_curve_angle(self.h, Q, Q.shape[0], accuracy_scale, out, nthreads)
+        return out
+
+    def count(self, q, accuracy_scale=2.0):
+        v, a, e = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().oracle_curve_count(self.h, _c(q, np.float32), accuracy_scale, C.byref(v), C.byref(a), C.byref(e))
+        return v.value, a.value, e.value
+
+
+class RefCurve(_CurveBase):
+    """The reference's own UT_SubtendedAngle (compiled from the reference header)."""
+    _prefix = "ref_curve_"
+
+    def _api(self):
+        return ref()
+
+    def angle(self, Q, accuracy_scale=2.0):
+        Q = _c(Q, np.float32).reshape(-1, 2)
+        out = np.empty(Q.shape[0], np.float32)
+        ref().ref_curve_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
+        return out
 
 
 # ---------------- cloud ----------------

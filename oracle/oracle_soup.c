@@ -20,6 +20,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/sysinfo.h>
 
 #define BVH_N 4
 #define NODE_EMPTY 0xFFFFFFFFu
@@ -56,10 +57,11 @@ static inline void box_combine(obox* b, const obox* s) {
     b->v[a][1] = (mx > s->v[a][1]) ? mx : s->v[a][1];
   }
 }
-/* Box::half_surface_area, NAXES==3 (4075-4080) */
-static inline float box_area(const obox* b) {
+/* Box::half_surface_area, NAXES==2 (4071-4075) and NAXES==3 (4076-4080) */
+static inline float box_area(const obox* b, int naxes) {
   const float d0 = b->v[0][1] - b->v[0][0];
   const float d1 = b->v[1][1] - b->v[1][0];
+  if (naxes == 2) return d0 + d1;
   const float d2 = b->v[2][1] - b->v[2][0];
   return d0 * d1 + d1 * d2 + d2 * d0;
 }
@@ -147,7 +149,7 @@ static void nth_element(const obox* boxes, uint32_t* indices, uint32_t* indices_
 
 /* ---------------- split (5348-5799) ---------------- */
 static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint32_t nboxes, uint32_t** split_indices,
-                   obox* split_boxes) {
+                   obox* split_boxes, int naxes) {
   if (nboxes == 2) {
     split_boxes[0] = boxes[indices[0]];
     split_boxes[1] = boxes[indices[1]];
@@ -171,7 +173,7 @@ static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint3
         box_combine(&sb[dest], &local_boxes[bit + 1]);
         ++cnt[dest];
       }
-      const float h = box_area(&sb[0]) * (float)cnt[0] + box_area(&sb[1]) * (float)cnt[1];
+      const float h = box_area(&sb[0], naxes) * (float)cnt[0] + box_area(&sb[1], naxes) * (float)cnt[1];
       if (best == 0xFFFFFFFFu || h < best_h) {
         best = bits;
         best_h = h;
@@ -192,7 +194,7 @@ static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint3
 
   uint32_t max_axis = 0;
   float max_len = axes_minmax.v[0][1] - axes_minmax.v[0][0];
-  for (uint32_t a = 1; a < 3; ++a) {
+  for (uint32_t a = 1; a < (uint32_t)naxes; ++a) {
     const float l = axes_minmax.v[a][1] - axes_minmax.v[a][0];
     if (l > max_len) {
       max_axis = a;
@@ -249,9 +251,9 @@ static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint3
       right[i - 1] = acc;
     }
     uint32_t best_split = 0;
-    float best_h = box_area(&left[0]) + box_area(&right[0]) * (float)(nboxes - 1);
+    float best_h = box_area(&left[0], naxes) + box_area(&right[0], naxes) * (float)(nboxes - 1);
     for (uint32_t s = 1; s < nsplits; ++s) {
-      const float h = box_area(&left[s]) * (float)(s + 1) + box_area(&right[s]) * (float)(nboxes - (s + 1));
+      const float h = box_area(&left[s], naxes) * (float)(s + 1) + box_area(&right[s], naxes) * (float)(nboxes - (s + 1));
       if (h < best_h) {
         best_split = s;
         best_h = h;
@@ -310,7 +312,7 @@ static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint3
     const uint32_t lc = left_counts[s];
     if (lc < min_count || lc > max_count) continue;
     const uint32_t rc = nboxes - lc;
-    const float h = (float)lc * box_area(&left[s]) + (float)rc * box_area(&right[s]);
+    const float h = (float)lc * box_area(&left[s], naxes) + (float)rc * box_area(&right[s], naxes);
     if (h < smallest) {
       smallest = h;
       split_index = s;
@@ -364,13 +366,13 @@ static void split2(obox axes_minmax, const obox* boxes, uint32_t* indices, uint3
 
 /* ---------------- multiSplit, BOX_AREA branch (5248-5346) ---------------- */
 static void multi_split(const obox* axes_minmax, const obox* boxes, uint32_t* indices, uint32_t nboxes,
-                        uint32_t* sub_indices[BVH_N + 1], obox sub_boxes[BVH_N]) {
+                        uint32_t* sub_indices[BVH_N + 1], obox sub_boxes[BVH_N], int naxes) {
   sub_indices[0] = indices;
   sub_indices[2] = indices + nboxes;
-  split2(*axes_minmax, boxes, indices, nboxes, &sub_indices[1], &sub_boxes[0]);
+  split2(*axes_minmax, boxes, indices, nboxes, &sub_indices[1], &sub_boxes[0], naxes);
   float areas[BVH_N];
-  areas[0] = box_area(&sub_boxes[0]);
-  areas[1] = box_area(&sub_boxes[1]);
+  areas[0] = box_area(&sub_boxes[0], naxes);
+  areas[1] = box_area(&sub_boxes[1], naxes);
   for (uint32_t nsub = 2; nsub < BVH_N; ++nsub) {
     uint32_t choice = 0xFFFFFFFFu;
     float maxh = 0;
@@ -390,15 +392,16 @@ static void multi_split(const obox* axes_minmax, const obox* boxes, uint32_t* in
     for (uint32_t i = nsub - 1; i > choice; --i) sub_boxes[i + 1] = sub_boxes[i];
     for (uint32_t i = nsub - 1; i > choice; --i) areas[i + 1] = areas[i];
     obox parent = sub_boxes[choice];
-    split2(parent, boxes, sel_start, (uint32_t)(sel_end - sel_start), &sub_indices[choice + 1], &sub_boxes[choice]);
-    areas[choice] = box_area(&sub_boxes[choice]);
-    areas[choice + 1] = box_area(&sub_boxes[choice + 1]);
+    split2(parent, boxes, sel_start, (uint32_t)(sel_end - sel_start), &sub_indices[choice + 1], &sub_boxes[choice],
+           naxes);
+    areas[choice] = box_area(&sub_boxes[choice], naxes);
+    areas[choice + 1] = box_area(&sub_boxes[choice + 1], naxes);
   }
 }
 
 /* ---------------- initNode (4916-5047), serial order ---------------- */
 static void init_node(nodevec* nodes, uint32_t nodei, const obox* axes_minmax, const obox* boxes, uint32_t* indices,
-                      uint32_t nboxes) {
+                      uint32_t nboxes, int naxes) {
   if (nboxes <= BVH_N) {
     for (uint32_t i = 0; i < nboxes; ++i) nodes->a[nodei].child[i] = indices[i];
     for (uint32_t i = nboxes; i < BVH_N; ++i) nodes->a[nodei].child[i] = NODE_EMPTY;
@@ -406,7 +409,7 @@ static void init_node(nodevec* nodes, uint32_t nodei, const obox* axes_minmax, c
   }
   uint32_t* sub_indices[BVH_N + 1];
   obox sub_boxes[BVH_N];
-  multi_split(axes_minmax, boxes, indices, nboxes, sub_indices, sub_boxes);
+  multi_split(axes_minmax, boxes, indices, nboxes, sub_indices, sub_boxes, naxes);
   for (uint32_t i = 0; i < BVH_N; ++i) {
     const uint32_t sn = (uint32_t)(sub_indices[i + 1] - sub_indices[i]);
     if (sn == 1) nodes->a[nodei].child[i] = sub_indices[i][0];
@@ -418,9 +421,55 @@ static void init_node(nodevec* nodes, uint32_t nodei, const obox* axes_minmax, c
     if (sn != 1) {
       const uint32_t start = nv_push(nodes);
       nodes->a[nodei].child[i] = start | NODE_INTERNAL;
-      init_node(nodes, start, &sub_boxes[i], boxes, sub_indices[i], sn);
+      init_node(nodes, start, &sub_boxes[i], boxes, sub_indices[i], sn, naxes);
     }
   }
+}
+
+/* BVH::init (4579-4634) over n boxes of 6 floats (x0 x1 y0 y1 z0 z1; a 2D box
+ * has z0 = z1 = 0 and naxes = 2).  Returns the node count and the nodes
+ * (4 uint32 each, malloc'ed, NULL for an empty tree). */
+uint32_t oracle_bvh4_build(const float* boxes6, uint32_t nboxes, int naxes, uint32_t** nodes_out) {
+  const obox* boxes = (const obox*)boxes6;
+  *nodes_out = NULL;
+  if (nboxes == 0) return 0;
+  uint32_t n = nboxes;
+  uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * n);
+  for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+  /* utExcludeNaNInfBoxIndices (4535-4577) */
+  uint32_t* end = idx + n;
+  uint32_t* p = idx;
+  for (; p != end; ++p)
+    if (box_exclude(&boxes[*p])) break;
+  if (p != end) {
+    uint32_t* nan_start = p;
+    for (++p; p != end; ++p)
+      if (!box_exclude(&boxes[*p])) *nan_start++ = *p;
+    n = (uint32_t)(nan_start - idx);
+  }
+  uint32_t count = 0;
+  if (n > 0) {
+    /* computeFullBoundingBox (4856-4913) over the kept boxes.  From 8192
+     * boxes on a host with more than one hardware thread (the reference asks
+     * std::thread::hardware_concurrency, 3670-3672) its parallel branch
+     * starts each thread's box from UT_Array::setSize's zero fill of the POD
+     * Box (2342-2360), so the origin is folded into the root bound. */
+    obox full;
+    if (n >= 2 * 4096 && get_nprocs() > 1) {
+      for (int a = 0; a < 3; ++a) full.v[a][0] = full.v[a][1] = 0.0f;
+      for (uint32_t i = 0; i < n; ++i) box_combine(&full, &boxes[idx[i]]);
+    } else {
+      full = boxes[idx[0]];
+      for (uint32_t i = 1; i < n; ++i) box_combine(&full, &boxes[idx[i]]);
+    }
+    nodevec nvv = {NULL, 0, 0};
+    nv_push(&nvv);
+    init_node(&nvv, 0, &full, boxes, idx, n, naxes);
+    *nodes_out = (uint32_t*)nvv.a;
+    count = nvv.n;
+  }
+  free(idx);
+  return count;
 }
 
 /* ---------------- precompute (6598-7128) ---------------- */
@@ -750,35 +799,7 @@ oracle_soup* oracle_soup_build(const float* V, int32_t nv, const int32_t* F, int
       }
     }
   }
-  /* BVH::init (4579-4634) */
-  s->nodes = NULL;
-  s->nnodes = 0;
-  if (nf > 0) {
-    uint32_t n = (uint32_t)nf;
-    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * n);
-    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
-    /* utExcludeNaNInfBoxIndices (4535-4577) */
-    uint32_t* end = idx + n;
-    uint32_t* p = idx;
-    for (; p != end; ++p)
-      if (box_exclude(&s->tri_boxes[*p])) break;
-    if (p != end) {
-      uint32_t* nan_start = p;
-      for (++p; p != end; ++p)
-        if (!box_exclude(&s->tri_boxes[*p])) *nan_start++ = *p;
-      n = (uint32_t)(nan_start - idx);
-    }
-    if (n > 0) {
-      obox full = s->tri_boxes[idx[0]];
-      for (uint32_t i = 1; i < n; ++i) box_combine(&full, &s->tri_boxes[idx[i]]);
-      nodevec nvv = {NULL, 0, 0};
-      nv_push(&nvv);
-      init_node(&nvv, 0, &full, s->tri_boxes, idx, n);
-      s->nodes = nvv.a;
-      s->nnodes = nvv.n;
-    }
-    free(idx);
-  }
+  s->nnodes = oracle_bvh4_build((const float*)s->tri_boxes, (uint32_t)nf, 3, (uint32_t**)&s->nodes);
   s->data = (boxdata_t*)calloc(s->nnodes > 0 ? s->nnodes : 1, sizeof(boxdata_t));
   if (s->nnodes > 0) {
     localdata_t root;

@@ -4,8 +4,10 @@
 // include/igl/FastWindingNumberForSoups.h where it lies in the reference tree
 // (see oracle/Makefile).  The output library goes to oracle/_ref/ only.  It
 // exposes the reference tree (BVH<4> nodes, 4217-4232), the per-node BoxData
-// (6477-6521) and computeSolidAngle (7148-7284), so that the C restatement in
-// oracle_soup.c and the HIP path can be compared with the reference itself.
+// (6477-6521) and computeSolidAngle (7148-7284), and the same for the 2D
+// UT_SubtendedAngle (BoxData 7287-7320, computeAngle 7700-7828), so that the C
+// restatements in oracle_soup.c / oracle_curve.c and the HIP path can be
+// compared with the reference itself.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -102,4 +104,51 @@ double ref_solid_angle_tri_f64(const double* a, const double* b, const double* c
 }
 
 int ref_num_threads(void) { return (int)igl::default_num_threads(); }
+
+// ---- UT_SubtendedAngle<float,float> (6315-6370, init 7342-7604, computeAngle 7700-7828) ----
+struct RefCurve {
+  std::vector<UT_Vector2T<float>> P;
+  std::vector<int> E;
+  UT_SubtendedAngle<float, float> sa;
+};
+
+void* ref_curve_build(const float* P, int np, const int* E, int ne, int order) {
+  RefCurve* r = new RefCurve();
+  r->P.resize(np > 0 ? np : 1);
+  for (int i = 0; i < np; ++i)
+    for (int j = 0; j < 2; ++j) r->P[i][j] = P[2 * i + j];
+  r->E.assign(E, E + 2 * (size_t)ne);
+  if (r->E.empty()) r->E.push_back(0);
+  r->sa.init(ne, r->E.data(), np, r->P.data(), order);
+  return r;
+}
+
+void ref_curve_free(void* h) { delete static_cast<RefCurve*>(h); }
+
+int ref_curve_num_nodes(void* h) { return static_cast<RefCurve*>(h)->sa.myTree.getNumNodes(); }
+
+void ref_curve_nodes(void* h, uint32_t* out) {
+  RefCurve* r = static_cast<RefCurve*>(h);
+  const int n = r->sa.myTree.getNumNodes();
+  if (n) std::memcpy(out, r->sa.myTree.getNodes(), sizeof(uint32_t) * 4 * (size_t)n);
+}
+
+// BoxData (7287-7320): 12 x v4uf = 48 floats per node
+void ref_curve_boxdata(void* h, float* out) {
+  RefCurve* r = static_cast<RefCurve*>(h);
+  const int n = r->sa.myTree.getNumNodes();
+  static_assert(sizeof(*r->sa.myData.get()) == 48 * sizeof(float), "BoxData layout");
+  if (n) std::memcpy(out, r->sa.myData.get(), sizeof(float) * 48 * (size_t)n);
+}
+
+// computeAngle for each query (float), serial
+void ref_curve_angle(void* h, const float* Q, int64_t nq, float accuracy_scale, float* angle) {
+  RefCurve* r = static_cast<RefCurve*>(h);
+  for (int64_t p = 0; p < nq; ++p) {
+    UT_Vector2T<float> q;
+    q[0] = Q[2 * p];
+    q[1] = Q[2 * p + 1];
+    angle[p] = r->sa.computeAngle(q, accuracy_scale);
+  }
+}
 }

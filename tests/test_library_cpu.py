@@ -43,6 +43,39 @@ def test_sah4_tree_matches_reference_nodes():
         assert np.array_equal(ours, theirs)
 
 
+@pytest.mark.parametrize("nf", [8191, 8192, 20000])
+def test_sah4_tree_root_bound_off_origin(nf):
+    """A mesh away from the origin: from 8192 boxes on, the reference's root
+    bound (computeFullBoundingBox's parallel branch) also contains the origin,
+    which moves the root split (FastWindingNumberForSoups.h:4856-4913)."""
+    rng = np.random.default_rng(nf)
+    V = rng.random((4000, 3)) + np.array([2.0, 0.5, -3.0])
+    F = rng.integers(0, 4000, (nf, 3)).astype(np.int32)
+    ref = O.RefSoup(V, F) if O.ref_available() else O.Soup(V, F)
+    assert np.array_equal(gwn.sah4_tree(V, F), ref.nodes())
+    Vs, Fs = D.bumpy_sphere(5, 16, 4)
+    Vs = Vs + 10.0
+    ref = O.RefSoup(Vs, Fs) if O.ref_available() else O.Soup(Vs, Fs)
+    assert np.array_equal(gwn.sah4_tree(Vs, Fs), ref.nodes())
+
+
+def test_sah4_tree_2d_matches_reference_nodes():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 5, 7, 33, 1000, 20000):
+        t = np.linspace(0, 2 * np.pi, n, endpoint=False)
+        P = np.stack([np.cos(t), np.sin(t)], 1) * (1 + 0.05 * rng.standard_normal((n, 1))) + 0.3
+        E = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+        ref = O.RefCurve(P, E) if O.ref_available() else O.Curve(P, E)
+        assert np.array_equal(gwn.sah4_tree_2d(P, E), ref.nodes())
+    P = rng.random((3000, 2)) + np.array([4.0, 1.0])
+    E = rng.integers(0, 3000, (12000, 2))
+    ref = O.RefCurve(P, E) if O.ref_available() else O.Curve(P, E)
+    assert np.array_equal(gwn.sah4_tree_2d(P, E), ref.nodes())
+    assert gwn.sah4_tree_2d(np.zeros((0, 2)), np.zeros((0, 2), np.int32)).shape == (0, 4)
+    with pytest.raises(nat.GWNError):
+        gwn.sah4_tree_2d(P, np.array([[0, 3000]]))
+
+
 def test_sah4_tree_degenerate_inputs():
     V, F = D.noisy_sphere(2, 0.01, 5)
     Vn = np.concatenate([V, [[np.nan, 0, 0]], [[np.inf, 1, 1]]])
