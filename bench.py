@@ -57,7 +57,9 @@ def main():
     ap.add_argument("--tree", choices=["sah4", "lbvh"], default="sah4")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 1.3M-triangle mixed vs FP64 comparison")
     ap.add_argument("--no-rows", action="store_true", help="skip the cloud / exact / signed-distance legs")
-    ap.add_argument("--rows", default="cloud,exact,signed_distance,point_areas,antipodal",
+    ap.add_argument("--rows-only", action="store_true",
+                    help="run only the rows named by --rows and print them as one JSON line (profiling runs)")
+    ap.add_argument("--rows", default="cloud,exact,signed_distance,point_areas,antipodal,curve",
                     help="comma-separated rows reported beside the headline")
     ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -84,6 +86,11 @@ def main():
     from libigl_amd import datagen as D
     from libigl_amd import gwn
     from libigl_amd import _native as nat
+
+    if args.rows_only:
+        torch.cuda.synchronize()
+        print(json.dumps({"rows": run_rows(args.rows, args.no_cpu)}), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -270,17 +277,7 @@ def main():
         except Exception as e:
             log(f"[bench] mixed workload skipped: {e}")
 
-    others = {}
-    if not args.no_rows:
-        wanted = set(args.rows.split(","))
-        for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance),
-                         ("point_areas", run_point_areas), ("antipodal", run_antipodal)):
-            if name not in wanted:
-                continue
-            try:
-                others[name] = fn(args.no_cpu)
-            except Exception as e:
-                log(f"[bench] {name} skipped: {e}")
+    others = {} if args.no_rows else run_rows(args.rows, args.no_cpu)
 
     serving = None
     if args.serving_duration > 0:
@@ -308,6 +305,21 @@ def main():
         "other_rows": others,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_rows(rows, no_cpu):
+    """The rows reported beside the headline, by name (comma-separated)."""
+    wanted = set(rows.split(","))
+    out = {}
+    for name, fn in (("cloud", run_cloud), ("exact", run_exact), ("signed_distance", run_signed_distance),
+                     ("point_areas", run_point_areas), ("antipodal", run_antipodal), ("curve", run_curve)):
+        if name not in wanted:
+            continue
+        try:
+            out[name] = fn(no_cpu)
+        except Exception as e:
+            log(f"[bench] {name} skipped: {e}")
+    return out
 
 
 def run_mixed_workload(beta_grid):
@@ -626,6 +638,83 @@ def run_antipodal(no_cpu):
                                "sample": f"{sub.size} queries, restatement with brute-force crossings (the reference "
                                          "counts them with Embree, which is not in the image)"}
     log(f"[bench] antipodal: {out}")
+    return out
+
+
+# counted FLOPs of the 2D path (UT_SubtendedAngle, FastWindingNumberForSoups.h
+# 7725-7809): the near/far test of one child slot (q - P, |q - P|^2, the
+# scaled radius), an accepted order-2 far field, an exact segment (atan2 as 1)
+FLOP_TEST2, FLOP_FAR2, FLOP_SEG = 6, 44, 12
+
+
+def run_curve(no_cpu):
+    """2D fast winding number (UT_SubtendedAngle<float,float>::computeAngle):
+    a 262,144-segment bumpy closed curve, 1M queries (half uniform in its box,
+    half within 1e-3 of the curve), order 2, accuracy_scale 2 (computeAngle's
+    default); checked on a sample against the oracle (bit-exact with the
+    reference) and against exact evaluation."""
+    import numpy as np
+    import torch
+
+    from libigl_amd import _native as nat
+    from libigl_amd import datagen as D
+    from libigl_amd import gwn
+
+    P, E, Q = D.curve_workload()
+    n = Q.shape[0]
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    c = gwn.SubtendedAngle(P, E, 2)
+    build_s = time.perf_counter() - tb
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    Qf = np.ascontiguousarray(Q, dtype=np.float32)
+    dQ = torch.from_numpy(Qf).to(dev)
+    dA = torch.empty(n, dtype=torch.float32, device=dev)
+    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_curve_query_device(
+        c._h, dQ.data_ptr(), n, 2.0, dA.data_ptr(), st)), 20, "curve_traverse")
+    A = dA.cpu().numpy()
+    info = c.info()
+    out = {"workload": "bumpy closed curve, 262,144 segments, 1M queries (50% uniform, 50% within 1e-3), "
+                       "order 2, accuracy_scale 2, FP32", "queries_per_s": round(n / dt, 1),
+           "kernel": "k_curve_traverse<2>", "kernel_ms": round(kms, 4), "build_s": round(build_s, 4),
+           "tree": {"nodes": int(info.num_nodes), "slots": int(info.num_slots), "depth": int(info.depth)}}
+    sub = np.arange(0, n, 1000)
+    exact = c.compute_angle(Qf[sub], 1e6)  # every segment evaluated exactly
+    out["max_abs_dw_vs_exact"] = float(np.abs(A[sub] - exact).max() / (2 * np.pi))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    co = O.Curve(P, E)
+    tot = np.zeros(3)
+    samp = np.arange(0, n, max(1, n // 4000))
+    for i in samp:
+        tot += co.count(Qf[i], 2.0)
+    visits, acc, ex = tot / samp.size
+    fpq = FLOP_TEST2 * 4 * visits + FLOP_FAR2 * acc + FLOP_SEG * ex
+    if kms > 0:
+        achieved = fpq * n / (kms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
+                           "frac": round(achieved / 157.3, 4), "flop_per_query": round(fpq, 1),
+                           "per_query": {"node_visits": round(visits, 2), "far_slots": round(acc, 2),
+                                         "exact_segments": round(ex, 2)}}
+    Ao = co.angle(Qf[sub], 2.0)
+    out["max_abs_dw_vs_oracle"] = float(np.abs(A[sub].astype(np.float64) - Ao).max() / (2 * np.pi))
+    if not no_cpu:
+        # the reference has no parallel 2D wrapper: computeAngle called point by
+        # point, as a caller would, on one core
+        if O.ref_available():
+            ref, kind = O.RefCurve(P, E), "reference"
+        else:
+            ref, kind = co, "port"
+        csub = np.arange(0, n, 4)
+        t = time.perf_counter()
+        Ar = ref.angle(Qf[csub], 2.0) if kind == "reference" else ref.angle(Qf[csub], 2.0, 1)
+        cdt = time.perf_counter() - t
+        out["matches_reference_sample_max_abs_dw"] = float(np.abs(A[csub].astype(np.float64) - Ar).max() / (2 * np.pi))
+        out["cpu_baseline"] = {"value": round(csub.size / cdt, 1), "unit": "queries/s", "cores": 1, "kind": kind,
+                               "sample": f"{csub.size} queries (every 4th) of the same workload, computeAngle "
+                                         "per point (serial)"}
+    log(f"[bench] curve: {out}")
     return out
 
 

@@ -16,6 +16,8 @@
 //   igl::fast_winding_number(P,N,A,Q,WN)              fast_winding_number.h:126-131
 //   struct igl::FastWindingNumberBVH                  fast_winding_number.h:140-152
 //   igl::read_triangle_mesh(str,V,F)                   read_triangle_mesh.h (.off / .obj)
+//   igl::FastWindingNumber::HDK_Sample::UT_SubtendedAngle<float,float>
+//                                                     FastWindingNumberForSoups.h:6315-6370 (2D curves)
 //   igl::winding_number_antipodal(V,F,intersector,O,W) winding_number_antipodal.h:43-53
 //   igl::WindingNumberAntipodalScene<Scalar>          WindingNumberAntipodalScene.h:48-256
 //   igl::knn(P,k,point_indices,CH,CN,W,I)             knn.h:33-45
@@ -496,6 +498,86 @@ bool read_triangle_mesh(const std::string& path, MV& V, MF& F) {
 }
 
 }  // namespace gwn_detail
+
+// FastWindingNumberForSoups.h's 2D fast winding number, UT_SubtendedAngle
+// (6315-6370), in the reference's namespace.  Only <float,float> is
+// instantiated, as in the reference.  Unlike the reference, the object copies
+// the segments and positions to the GPU at init, so the caller's arrays need
+// not outlive it.
+namespace FastWindingNumber {
+namespace HDK_Sample {
+
+// the reference's UT_Vector2T is UT_FixedVector<T,2>: two components with []
+template <typename T>
+struct UT_Vector2T {
+  T vec[2];
+  UT_Vector2T() = default;
+  UT_Vector2T(T x, T y) : vec{x, y} {}
+  T& operator[](int i) { return vec[i]; }
+  const T& operator[](int i) const { return vec[i]; }
+};
+
+template <typename T, typename S>
+class UT_SubtendedAngle {
+  static_assert(std::is_same<T, float>::value && std::is_same<S, float>::value,
+                "UT_SubtendedAngle is provided for <float,float>");
+
+ public:
+  UT_SubtendedAngle() = default;
+  UT_SubtendedAngle(const int nsegments, const int* const segment_points, const int npoints,
+                    const UT_Vector2T<S>* const positions, const int order = 2) {
+    init(nsegments, segment_points, npoints, positions, order);
+  }
+  UT_SubtendedAngle(const UT_SubtendedAngle&) = delete;
+  UT_SubtendedAngle& operator=(const UT_SubtendedAngle&) = delete;
+  ~UT_SubtendedAngle() { clear(); }
+
+  // init (7342-7604); may be called again to re-initialise
+  void init(const int nsegments, const int* const segment_points, const int npoints,
+            const UT_Vector2T<S>* const positions, const int order = 2) {
+    clear();
+    std::vector<float> P(2 * (size_t)std::max(npoints, 0));
+    for (int i = 0; i < npoints; ++i) {
+      P[2 * (size_t)i] = positions[i][0];
+      P[2 * (size_t)i + 1] = positions[i][1];
+    }
+    gwn_detail::check(gwn_curve_build(P.data(), npoints, segment_points, nsegments, order, GWN_TREE_SAH4, &handle_));
+    nsegments_ = nsegments;
+  }
+  void clear() {
+    if (handle_) gwn_curve_free(handle_);
+    handle_ = nullptr;
+    nsegments_ = 0;
+  }
+  bool isClear() const { return nsegments_ == 0; }
+
+  // computeAngle (7700-7828): the signed angle the curve subtends at the point
+  T computeAngle(const UT_Vector2T<T>& query_point, const T accuracy_scale = T(2.0)) const {
+    T out = T(0);
+    computeAngles(1, &query_point, &out, accuracy_scale);
+    return out;
+  }
+  // batch form (no counterpart in the reference, which is called per point):
+  // one GPU call for nq points
+  void computeAngles(int64_t nq, const UT_Vector2T<T>* query_points, T* angles,
+                     const T accuracy_scale = T(2.0)) const {
+    if (nq <= 0) return;
+    if (!handle_) {
+      std::fill(angles, angles + nq, T(0));
+      return;
+    }
+    static_assert(sizeof(UT_Vector2T<T>) == 2 * sizeof(T), "packed 2-vectors");
+    gwn_detail::check(
+        gwn_curve_query(handle_, reinterpret_cast<const float*>(query_points), nq, accuracy_scale, angles));
+  }
+
+ private:
+  gwn_curve* handle_ = nullptr;
+  int nsegments_ = 0;
+};
+
+}  // namespace HDK_Sample
+}  // namespace FastWindingNumber
 }  // namespace igl
 
 #if defined(__has_include)

@@ -11,12 +11,12 @@
 //                    BoxData, LocalData for the parent, subtree slot count
 //   rope_first_slots top-down first slot of every node (soup.hip)
 //   k_curve_slots    one 48-byte record per non-empty child slot
-// Query (gwn_curve_query*): 2D Morton order of the queries (LDS radix sort),
-// then k_curve_traverse walks the rope layout wave-coherently, as the soup's
-// k_traverse does: a far child slot (|q-P|^2 > maxP2*beta^2 and a finite
-// Taylor value, AngleFunctors::pre 7725-7800) adds its order-0/1/2 term and
-// jumps past its subtree, a near internal slot steps into its first child, a
-// near segment adds the exact angle (UTsignedAngleSegment, 6279-6307).
+// Query (gwn_curve_query*): large batches are put in 2D Morton order (LDS radix
+// sort); k_curve_traverse walks the rope layout, one query per lane: a far
+// child slot (|q-P|^2 > maxP2*beta^2 and a finite Taylor value,
+// AngleFunctors::pre 7725-7800) adds its order-0/1/2 term and jumps past its
+// subtree, a near internal slot steps into its first child, a near segment
+// adds the exact angle (UTsignedAngleSegment, 6279-6307).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +30,8 @@ namespace gwn {
 namespace {
 
 constexpr uint32_t kLinkLeaf = 0x80000000u;
+// batches of at least this many queries are put in Morton order first
+constexpr int64_t kCurveSortMin = 262144;
 
 __device__ __forceinline__ float cmin(float a, float b) { return (a > b) ? b : a; }
 __device__ __forceinline__ float cmax(float a, float b) { return (a < b) ? b : a; }
@@ -295,10 +297,14 @@ __device__ __forceinline__ float segment_angle(const float4 s, float qx, float q
   return atan2_fast(num, den);
 }
 
-// Wave-coherent rope walk (see soup.hip, k_traverse): the wavefront walks one
-// shared record index; a lane takes part in a record when it is the lane's
-// next one, so each lane sums exactly the terms of its own walk, while every
-// record and segment is fetched once per wave through the scalar cache.
+// Rope walk, one query per lane: each lane follows its own skip pointers and
+// gathers its own 48-byte records (a far child slot adds its Taylor value and
+// skips its subtree, a near internal slot steps into its first child, a near
+// segment adds the exact angle).  A wave-coherent walk over a shared record
+// index (as the soup's k_traverse) measured slower here at every batch size
+// (1M queries on a 262,144-segment curve: 127 us against 115 us): the 2D
+// records are small and a lane's path is short (~11 nodes), so the shared
+// index mostly serialises lanes whose paths differ.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_curve_traverse(const float4* __restrict__ slots, uint32_t nslots,
                                                         const float4* __restrict__ seg, const float* __restrict__ Q,
@@ -309,47 +315,40 @@ __global__ __launch_bounds__(256) void k_curve_traverse(const float4* __restrict
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[2 * qi], qy = Q[2 * qi + 1];
   float sum = 0.0f;
-  uint32_t skip = 0;
   uint32_t i = 0;
   while (i < nslots) {
     const float4* r = slots + 3 * (size_t)i;
     const float4 a = r[0], b = r[1];
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (ORDER >= 2) c = r[2];
-    if (ORDER >= 2) asm volatile("" ::"s"(a.x), "s"(b.w), "s"(c.w));
-    else asm volatile("" ::"s"(a.x), "s"(b.w));
     const uint32_t link = __float_as_uint(a.w);
     const bool leaf = (link & kLinkLeaf) != 0;
-    if (skip == i) {
-      const float x = qx - a.x, y = qy - a.y;
-      // |q-P|^2 rounded as the reference rounds it, so every near/far
-      // decision is the reference's
-      const float l2 = __fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y));
-      bool exact = leaf;
-      skip = i + 1;
-      if (!(l2 <= __fmul_rn(a.z, acc2))) {
-        const float o = far_field2<ORDER>(x, y, l2, b, c);
-        if (isfinite(o)) {  // otherwise the reference descends / evaluates the item
-          sum = __fadd_rn(sum, o);
-          exact = false;
-          if (!leaf) skip = link;
-        }
+    const float x = qx - a.x, y = qy - a.y;
+    const float l2 = __fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y));
+    bool exact = leaf;
+    uint32_t next = i + 1;
+    if (!(l2 <= __fmul_rn(a.z, acc2))) {
+      const float o = far_field2<ORDER>(x, y, l2, b, c);
+      if (isfinite(o)) {
+        sum = __fadd_rn(sum, o);
+        exact = false;
+        if (!leaf) next = link;
       }
-      if (exact) sum = __fadd_rn(sum, segment_angle(seg[link & ~kLinkLeaf], qx, qy));
     }
-    const bool step = leaf || __ballot(skip == i + 1) != 0;
-    i = __builtin_amdgcn_readfirstlane(step ? i + 1 : link);
+    if (exact) sum = __fadd_rn(sum, segment_angle(seg[link & ~kLinkLeaf], qx, qy));
+    i = next;
   }
   angle[qi] = sum;
 }
 
-// 32-bit 2D Morton key (16 bits per axis) of a query over the curve's box
-__device__ __forceinline__ uint32_t spread16(uint32_t v) {
-  v &= 0xFFFFu;
-  v = (v | (v << 8)) & 0x00FF00FFu;
-  v = (v | (v << 4)) & 0x0F0F0F0Fu;
-  v = (v | (v << 2)) & 0x33333333u;
-  v = (v | (v << 1)) & 0x55555555u;
+// 16-bit 2D Morton key (8 bits per axis: a 256 x 256 grid over the curve's
+// box) of a query; two radix passes order a large batch well enough (wider
+// keys cost more in the sort than they save in the walk)
+__device__ __forceinline__ uint32_t spread8(uint32_t v) {
+  v &= 0xFFu;
+  v = (v | (v << 4)) & 0x0F0Fu;
+  v = (v | (v << 2)) & 0x3333u;
+  v = (v | (v << 1)) & 0x5555u;
   return v;
 }
 __global__ void k_morton2(const float* __restrict__ Q, int64_t nq, float lx, float ly, float sx, float sy,
@@ -357,9 +356,9 @@ __global__ void k_morton2(const float* __restrict__ Q, int64_t nq, float lx, flo
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   float fx = (Q[2 * i] - lx) * sx, fy = (Q[2 * i + 1] - ly) * sy;
-  fx = fminf(fmaxf(fx, 0.0f), 65535.0f);
-  fy = fminf(fmaxf(fy, 0.0f), 65535.0f);
-  keys[i] = (spread16((uint32_t)fx) << 1) | spread16((uint32_t)fy);
+  fx = fminf(fmaxf(fx, 0.0f), 255.0f);
+  fy = fminf(fmaxf(fy, 0.0f), 255.0f);
+  keys[i] = (spread8((uint32_t)fx) << 1) | spread8((uint32_t)fy);
   vals[i] = (uint32_t)i;
 }
 
@@ -432,20 +431,24 @@ void curve_query_device(const gwn_curve* cc, const float* Q, int64_t nq, float a
     GWN_HIP(hipMemsetAsync(angle, 0, sizeof(float) * nq, st));
     return;
   }
-  const size_t a = 256;
-  auto up = [&](size_t b) { return (b + a - 1) / a * a; };
-  const size_t tmpb = radix_sort_temp_bytes(nq, false);
-  c->work.reserve(2 * up(4 * (size_t)nq) + up(tmpb));
-  uint32_t* keys = (uint32_t*)c->work.p;
-  uint32_t* perm = (uint32_t*)(c->work.p + up(4 * (size_t)nq));
-  void* tmp = c->work.p + 2 * up(4 * (size_t)nq);
-  const float ex = std::max(c->bbox[2] - c->bbox[0], 1e-30f), ey = std::max(c->bbox[3] - c->bbox[1], 1e-30f);
-  hipLaunchKernelGGL(k_morton2, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, c->bbox[0], c->bbox[1],
-                     65536.0f / ex, 65536.0f / ey, keys, perm);
-  check_launch("k_morton2");
-  {
+  // Large batches are walked in Morton order (neighbouring lanes then follow
+  // similar paths); below kCurveSortMin the sort costs more than it saves
+  // (65,536 queries: 32 us unsorted, 88 us sorted).
+  uint32_t* perm = nullptr;
+  if (nq >= kCurveSortMin) {
+    const size_t a = 256;
+    auto up = [&](size_t b) { return (b + a - 1) / a * a; };
+    const size_t tmpb = radix_sort_temp_bytes(nq, false);
+    c->work.reserve(2 * up(4 * (size_t)nq) + up(tmpb));
+    uint32_t* keys = (uint32_t*)c->work.p;
+    perm = (uint32_t*)(c->work.p + up(4 * (size_t)nq));
+    void* tmp = c->work.p + 2 * up(4 * (size_t)nq);
+    const float ex = std::max(c->bbox[2] - c->bbox[0], 1e-30f), ey = std::max(c->bbox[3] - c->bbox[1], 1e-30f);
+    hipLaunchKernelGGL(k_morton2, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, c->bbox[0], c->bbox[1],
+                       256.0f / ex, 256.0f / ey, keys, perm);
+    check_launch("k_morton2");
     ProfileScope prof("radix_sort", st);
-    radix_sort_u32(keys, perm, nq, 32, tmp, tmpb, st);
+    radix_sort_u32(keys, perm, nq, 16, tmp, tmpb, st);
   }
   const float acc2 = accuracy_scale * accuracy_scale;
   const unsigned g = div_up(nq, 256);

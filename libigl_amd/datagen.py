@@ -209,3 +209,37 @@ def serving_meshes(nmeshes=64, lo=1_000, hi=500_000, seed=55):
         V, F = uv_sphere(int(c), nbumps=8, seed=seed * 1000 + i)
         meshes.append((V, F))
     return meshes
+
+
+def bumpy_curve(nseg: int, nbumps: int = 16, seed: int = 66):
+    """A closed counter-clockwise curve of nseg segments: the unit circle with
+    its radius displaced by nbumps Gaussian bumps, r(t) = 1 + sum a_j exp(-d^2/w_j^2)
+    (d the angular distance to the bump's centre), a_j ~ U(-0.3, 0.3),
+    w_j ~ U(0.1, 0.4)."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0.0, 2.0 * np.pi, nseg, endpoint=False)
+    r = np.ones(nseg)
+    for _ in range(nbumps):
+        c, a, w = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.3), rng.uniform(0.1, 0.4)
+        d = np.angle(np.exp(1j * (t - c)))
+        r += a * np.exp(-(d * d) / (w * w))
+    P = np.stack([r * np.cos(t), r * np.sin(t)], 1)
+    E = np.stack([np.arange(nseg), (np.arange(nseg) + 1) % nseg], 1).astype(np.int32)
+    return P, E
+
+
+def curve_workload(nseg=262_144, nq=1_000_000, seed=66, near_dist=1e-3):
+    """2D fast winding number workload: bumpy_curve(nseg), half the queries
+    uniform in its box, half within near_dist of a segment (a random point of
+    a random segment moved along the segment's normal)."""
+    P, E = bumpy_curve(nseg, 16, seed)
+    rng = np.random.default_rng(seed + 1000)
+    nu = nq // 2
+    Qu = rng.uniform(P.min(0), P.max(0), (nu, 2))
+    k = rng.integers(0, nseg, nq - nu)
+    a, b = P[E[k, 0]], P[E[k, 1]]
+    s = rng.random((nq - nu, 1))
+    d = b - a
+    n = np.stack([d[:, 1], -d[:, 0]], 1) / np.maximum(np.linalg.norm(d, axis=1, keepdims=True), 1e-300)
+    Qn = a + s * d + n * rng.uniform(-near_dist, near_dist, (nq - nu, 1))
+    return P, E, np.concatenate([Qu, Qn], 0)

@@ -113,6 +113,31 @@ int main() {
                                      std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), S, I, C);
     if (S.rows() != 1 || std::fabs(S(0, 0) - std::sqrt(1.0 / 12.0)) > 1e-7 || I(0, 0) != 3) bad |= 1024;
   }
+  // UT_SubtendedAngle (2D): a closed 64-gon, counter-clockwise: 2 pi inside,
+  // 0 outside; the batch form agrees with the per-point one
+  {
+    using igl::FastWindingNumber::HDK_Sample::UT_SubtendedAngle;
+    using igl::FastWindingNumber::HDK_Sample::UT_Vector2T;
+    const int n = 64;
+    std::vector<UT_Vector2T<float>> P(n);
+    std::vector<int> E(2 * n);
+    for (int i = 0; i < n; ++i) {
+      const double t = 2 * 3.14159265358979323846 * i / n;
+      P[i] = UT_Vector2T<float>((float)std::cos(t), (float)std::sin(t));
+      E[2 * i] = i;
+      E[2 * i + 1] = (i + 1) % n;
+    }
+    UT_SubtendedAngle<float, float> sa(n, E.data(), n, P.data(), 2);
+    const float in = sa.computeAngle(UT_Vector2T<float>(0.1f, -0.2f), 2.0f);
+    const float out = sa.computeAngle(UT_Vector2T<float>(3.0f, 0.5f), 2.0f);
+    if (std::fabs(in / (2 * 3.14159265f) - 1.0f) > 2e-3f || std::fabs(out) > 2e-2f) bad |= 2048;
+    const UT_Vector2T<float> qs[2] = {UT_Vector2T<float>(0.1f, -0.2f), UT_Vector2T<float>(3.0f, 0.5f)};
+    float both[2];
+    sa.computeAngles(2, qs, both, 2.0f);
+    if (both[0] != in || both[1] != out || sa.isClear()) bad |= 2048;
+    UT_SubtendedAngle<float, float> empty;
+    if (!empty.isClear() || empty.computeAngle(UT_Vector2T<float>(0.0f, 0.0f)) != 0.0f) bad |= 2048;
+  }
   // bad input is reported, not asserted
   Mat<int> Fbad(1, 3);
   Fbad(0, 0) = 0, Fbad(0, 1) = 1, Fbad(0, 2) = 9;
