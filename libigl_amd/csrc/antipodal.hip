@@ -16,7 +16,7 @@
 //              non-zero nets become segments oriented so the weight is > 0
 //   frac       per query, the boundary sum in FP64 (segments staged in LDS)
 //   crossings  all rays share the direction x0, so the queries are taken in
-//              Morton order and a wavefront walks the soup's rope of slot
+//              Hilbert order and a wavefront walks the soup's rope of slot
 //              boxes coherently (as the closest-point search does): a lane
 //              enters a slot whose box its ray meets (FP64 slab test on boxes
 //              rounded outward) and counts its triangle with the watertight
@@ -295,7 +295,7 @@ void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no,
   DevBuf<uint32_t> keys(no), perm(no);
   hipLaunchKernelGGL(k_ap_narrow, dim3(div_up(3 * no, 256)), dim3(256), 0, st, O, no, Of.p);
   check_launch("k_ap_narrow");
-  morton_keys(Of.p, no, s->bbox, keys.p, perm.p, st);
+  hilbert_keys(Of.p, no, s->bbox, keys.p, perm.p, st);
   {
     const size_t tb = radix_sort_temp_bytes(no, false);
     DevBuf<uint8_t> tmp(tb);

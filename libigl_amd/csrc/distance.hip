@@ -270,7 +270,7 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
   DevBuf<int32_t> face(np);
   hipLaunchKernelGGL(k_narrow3, dim3(div_up(3 * np, 256)), dim3(256), 0, st, P, np, Pf.p);
   check_launch("k_narrow3");
-  morton_keys(Pf.p, np, s->bbox, keys.p, perm.p, st);
+  hilbert_keys(Pf.p, np, s->bbox, keys.p, perm.p, st);
   {
     const size_t tb = radix_sort_temp_bytes(np, false);
     DevBuf<uint8_t> tmp(tb);

@@ -167,6 +167,8 @@ void soup_ensure_dist(gwn_soup* s, hipStream_t st);
 // .off / .obj file as a triangle mesh (polygons split into fans)
 void read_triangle_mesh(const char* path, std::vector<double>& V, std::vector<int32_t>& F);
 void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
+// 30-bit Hilbert keys (10 bits per axis over bbox) + identity values: query order for the tree walks
+void hilbert_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
                             double* S, int32_t* I, double* C, hipStream_t st);
 // k nearest neighbours and point areas (knn.hip)
