@@ -729,25 +729,6 @@ __global__ __launch_bounds__(256) void k_cloud_brute(const float* __restrict__ P
   if (ok) WN[q] = (float)wn;
 }
 
-// queries along a Morton curve (shared with the soup path's ordering)
-__device__ __forceinline__ uint32_t cl_expand10(uint32_t v) {
-  v = (v * 0x00010001u) & 0xFF0000FFu;
-  v = (v * 0x00000101u) & 0x0F00F00Fu;
-  v = (v * 0x00000011u) & 0xC30C30C3u;
-  v = (v * 0x00000005u) & 0x49249249u;
-  return v;
-}
-__global__ void k_cloud_morton_q(const float* __restrict__ Q, int64_t nq, float lx, float ly, float lz, float sx,
-                                 float sy, float sz, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  float fx = fminf(fmaxf((Q[3 * i] - lx) * sx, 0.0f), 1023.0f);
-  float fy = fminf(fmaxf((Q[3 * i + 1] - ly) * sy, 0.0f), 1023.0f);
-  float fz = fminf(fmaxf((Q[3 * i + 2] - lz) * sz, 0.0f), 1023.0f);
-  keys[i] = (cl_expand10((uint32_t)fx) << 2) | (cl_expand10((uint32_t)fy) << 1) | cl_expand10((uint32_t)fz);
-  vals[i] = (uint32_t)i;
-}
-
 // ---------------- host orchestration ----------------
 // CM, R, EC of every cell.  fold_cm: split cells of this library's own
 // octree take their centres from the per-level folds (their point lists are
@@ -1220,12 +1201,7 @@ void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double 
   uint32_t* keys = (uint32_t*)c->work.p;
   uint32_t* perm = keys + (4 * (size_t)nq + 255) / 256 * 64;
   void* tmp = (uint8_t*)perm + (4 * (size_t)nq + 255) / 256 * 256;
-  const float* bb = c->bbox;
-  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
-              ez = std::max(bb[5] - bb[2], 1e-30f);
-  hipLaunchKernelGGL(k_cloud_morton_q, dim3(g), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
-                     1024.0f / ey, 1024.0f / ez, keys, perm);
-  check_launch("k_cloud_morton_q");
+  hilbert_keys(Q, nq, c->bbox, keys, perm, st);
   radix_sort_u32(keys, perm, nq, 30, tmp, tb, st);
   const bool rl = c->root_leaf;
   const float* Ps = c->Ps.p + (rl ? 3 * c->root_start : 0);
