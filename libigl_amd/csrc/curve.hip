@@ -11,7 +11,7 @@
 //                    BoxData, LocalData for the parent, subtree slot count
 //   rope_first_slots top-down first slot of every node (soup.hip)
 //   k_curve_slots    one 48-byte record per non-empty child slot
-// Query (gwn_curve_query*): large batches are put in 2D Morton order (LDS radix
+// Query (gwn_curve_query*): large batches are put in 2D Hilbert order (LDS radix
 // sort); k_curve_traverse walks the rope layout, one query per lane: a far
 // child slot (|q-P|^2 > maxP2*beta^2 and a finite Taylor value,
 // AngleFunctors::pre 7725-7800) adds its order-0/1/2 term and jumps past its
@@ -30,7 +30,7 @@ namespace gwn {
 namespace {
 
 constexpr uint32_t kLinkLeaf = 0x80000000u;
-// batches of at least this many queries are put in Morton order first
+// batches of at least this many queries are put in Hilbert order first
 constexpr int64_t kCurveSortMin = 262144;
 
 __device__ __forceinline__ float cmin(float a, float b) { return (a > b) ? b : a; }
@@ -341,24 +341,35 @@ __global__ __launch_bounds__(128) void k_curve_traverse(const float4* __restrict
   angle[qi] = sum;
 }
 
-// 16-bit 2D Morton key (8 bits per axis: a 256 x 256 grid over the curve's
-// box) of a query; two radix passes order a large batch well enough (wider
-// keys cost more in the sort than they save in the walk)
-__device__ __forceinline__ uint32_t spread8(uint32_t v) {
-  v &= 0xFFu;
-  v = (v | (v << 4)) & 0x0F0Fu;
-  v = (v | (v << 2)) & 0x3333u;
-  v = (v | (v << 1)) & 0x5555u;
-  return v;
+// 16-bit 2D Hilbert key of a query (a 256 x 256 grid over the curve's box;
+// the classic rotate-and-reflect walk down the quadrants): two radix passes
+// order a large batch well enough (wider keys cost more in the sort than they
+// save in the walk)
+__device__ __forceinline__ uint32_t hilbert2_8(uint32_t x, uint32_t y) {
+  uint32_t d = 0;
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = 255u - x;
+        y = 255u - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
 }
-__global__ void k_morton2(const float* __restrict__ Q, int64_t nq, float lx, float ly, float sx, float sy,
-                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_order2(const float* __restrict__ Q, int64_t nq, float lx, float ly, float sx, float sy,
+                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   float fx = (Q[2 * i] - lx) * sx, fy = (Q[2 * i + 1] - ly) * sy;
   fx = fminf(fmaxf(fx, 0.0f), 255.0f);
   fy = fminf(fmaxf(fy, 0.0f), 255.0f);
-  keys[i] = (spread8((uint32_t)fx) << 1) | spread8((uint32_t)fy);
+  keys[i] = hilbert2_8((uint32_t)fx, (uint32_t)fy);
   vals[i] = (uint32_t)i;
 }
 
@@ -431,7 +442,7 @@ void curve_query_device(const gwn_curve* cc, const float* Q, int64_t nq, float a
     GWN_HIP(hipMemsetAsync(angle, 0, sizeof(float) * nq, st));
     return;
   }
-  // Large batches are walked in Morton order (neighbouring lanes then follow
+  // Large batches are walked in Hilbert order (neighbouring lanes then follow
   // similar paths); below kCurveSortMin the sort costs more than it saves
   // (65,536 queries: 32 us unsorted, 88 us sorted).
   uint32_t* perm = nullptr;
@@ -444,9 +455,9 @@ void curve_query_device(const gwn_curve* cc, const float* Q, int64_t nq, float a
     perm = (uint32_t*)(c->work.p + up(4 * (size_t)nq));
     void* tmp = c->work.p + 2 * up(4 * (size_t)nq);
     const float ex = std::max(c->bbox[2] - c->bbox[0], 1e-30f), ey = std::max(c->bbox[3] - c->bbox[1], 1e-30f);
-    hipLaunchKernelGGL(k_morton2, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, c->bbox[0], c->bbox[1],
+    hipLaunchKernelGGL(k_order2, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, c->bbox[0], c->bbox[1],
                        256.0f / ex, 256.0f / ey, keys, perm);
-    check_launch("k_morton2");
+    check_launch("k_order2");
     ProfileScope prof("radix_sort", st);
     radix_sort_u32(keys, perm, nq, 16, tmp, tmpb, st);
   }
