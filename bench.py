@@ -37,7 +37,7 @@ FLOP_EXACT = 76   # exact triangle (6078-6109)
 # SAH4 tree), keyed by accuracy_scale: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
 # for a 256 MiB device copy (fetch x2.000, write x1.000).
-MEASURED_TRAFFIC = {4.9: int((180.7 + 33.4) * 2**20)}
+MEASURED_TRAFFIC = {4.9: int((181.0 + 33.4) * 2**20)}
 
 
 def log(*a):
