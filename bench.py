@@ -76,8 +76,8 @@ def main():
         print(json.dumps({"mode": "copy", "bytes": 4 * n, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries would shard by "
-                                   "Morton range with a replicated tree; see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (more GPUs would shard the "
+                                   "queries by curve-order range against a replicated tree)"}))
         sys.exit(2)
 
     import numpy as np

@@ -6,7 +6,7 @@ reference's golden values and brute force):
   * CM and R bit-identical (FP32 sums in ascending point order, as the
     reference), EC within 2e-5 relative (accumulated in FP64 on the GPU);
   * W within 1e-4 at beta = 2 and identical 0.5 classification outside a 1e-4
-    band (north-star tolerance);
+    band (the fast path's winding-number tolerance);
   * beta <= 0 brute force within 1e-5.
 """
 import numpy as np

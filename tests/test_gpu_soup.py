@@ -4,7 +4,7 @@ Against the reference (oracle/_ref, the reference's own UT_SolidAngle) and the
 bit-exact C restatement:
   * tree: identical node arrays (SAH4 mode);
   * moments: BoxData computed by the GPU upward pass identical bit for bit;
-  * W: max |W_gpu - W_ref| <= 1e-4 (observed ~1e-6,
+  * W: max |W_gpu - W_ref| <= 1e-4 (the fast path's tolerance; observed ~1e-6,
     from the per-slot summation order and the device atan2f);
   * identical inside/outside classification at 0.5 wherever the reference is
     more than 1e-4 away from 0.5.
