@@ -13,10 +13,12 @@
 // child node precedes its own subtree (4916-5047).  Floating-point steps keep
 // the reference's operation order (the file is built with -ffp-contract=off).
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <future>
 #include <limits>
 #include <thread>
 #include <vector>
@@ -68,10 +70,17 @@ struct Box {
   }
 };
 
+// Subtrees of at least this many boxes are built on their own threads when a
+// node has two or more of them (the reference's initNode does the same,
+// 4942-5031); the node array comes out as the serial order: every child node
+// precedes its own subtree, children in slot order.
+constexpr uint32_t kParallelBoxes = 1024;
+
 class Builder {
  public:
   Builder(const std::vector<Box>& boxes, std::vector<uint32_t>& out, int naxes)
-      : boxes_(boxes), out_(out), naxes_(naxes) {}
+      : boxes_(boxes), out_(out), naxes_(naxes),
+        max_threads_(std::min(32u, std::max(1u, std::thread::hardware_concurrency()))) {}
 
   void run(std::vector<uint32_t>& idx) {
     const uint32_t n = (uint32_t)idx.size();
@@ -89,13 +98,15 @@ class Builder {
       }
     }
     out_.assign(4, NODE_EMPTY);
-    build_node(0, full, idx.data(), n);
+    build_node(out_, 0, full, idx.data(), n);
   }
 
  private:
   const std::vector<Box>& boxes_;
   std::vector<uint32_t>& out_;
   const int naxes_;
+  const unsigned max_threads_;
+  std::atomic<unsigned> threads_{1};
 
   float c2(uint32_t i, uint32_t axis) const { return boxes_[i].centre2(axis); }
 
@@ -366,23 +377,56 @@ class Builder {
     }
   }
 
-  void build_node(uint32_t node, const Box& bound, uint32_t* idx, uint32_t n) {
+  // Builds node `node` of `out` over idx[0, n) and appends its subtree.
+  void build_node(std::vector<uint32_t>& out, uint32_t node, const Box& bound, uint32_t* idx, uint32_t n) {
     if (n <= kN) {
-      for (uint32_t i = 0; i < kN; ++i) out_[4 * node + i] = (i < n) ? idx[i] : NODE_EMPTY;
+      for (uint32_t i = 0; i < kN; ++i) out[4 * node + i] = (i < n) ? idx[i] : NODE_EMPTY;
       return;
     }
     uint32_t* part[kN + 1];
     Box part_box[kN];
     split4(bound, idx, n, part, part_box);
     for (uint32_t i = 0; i < kN; ++i)
-      if (part[i + 1] - part[i] == 1) out_[4 * node + i] = part[i][0];
+      if (part[i + 1] - part[i] == 1) out[4 * node + i] = part[i][0];
+    // large children on threads of their own, each into a local array whose
+    // node 0 is the child; spliced in below at the child's serial position
+    std::vector<uint32_t> local[kN];
+    std::future<void> task[kN];
+    int nlarge = 0;
+    for (uint32_t i = 0; i < kN; ++i) nlarge += (uint32_t)(part[i + 1] - part[i]) >= kParallelBoxes;
+    if (nlarge >= 2) {
+      for (uint32_t i = 0; i < kN; ++i) {
+        const uint32_t cnt = (uint32_t)(part[i + 1] - part[i]);
+        if (cnt < kParallelBoxes) continue;
+        local[i].reserve((size_t)4 * (cnt / 2 + cnt / 6 + 4));
+        local[i].assign(4, NODE_EMPTY);
+        if (threads_.fetch_add(1) < max_threads_) {
+          task[i] = std::async(std::launch::async, [this, &local, &part_box, &part, i, cnt] {
+            build_node(local[i], 0, part_box[i], part[i], cnt);
+            threads_.fetch_sub(1);
+          });
+        } else {
+          threads_.fetch_sub(1);
+          build_node(local[i], 0, part_box[i], part[i], cnt);
+        }
+      }
+    }
     for (uint32_t i = 0; i < kN; ++i) {
       const uint32_t cnt = (uint32_t)(part[i + 1] - part[i]);
       if (cnt == 1) continue;
-      const uint32_t child = (uint32_t)(out_.size() / 4);
-      out_.insert(out_.end(), 4, NODE_EMPTY);
-      out_[4 * node + i] = child | NODE_INTERNAL;
-      build_node(child, part_box[i], part[i], cnt);
+      const uint32_t child = (uint32_t)(out.size() / 4);
+      out[4 * node + i] = child | NODE_INTERNAL;
+      if (!local[i].empty()) {
+        if (task[i].valid()) task[i].get();
+        const size_t at = out.size();
+        out.insert(out.end(), local[i].begin(), local[i].end());
+        for (size_t k = at; k < out.size(); ++k)
+          if (out[k] != NODE_EMPTY && (out[k] & NODE_INTERNAL)) out[k] += child;
+        std::vector<uint32_t>().swap(local[i]);
+      } else {
+        out.insert(out.end(), 4, NODE_EMPTY);
+        build_node(out, child, part_box[i], part[i], cnt);
+      }
     }
   }
 };

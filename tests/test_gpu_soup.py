@@ -44,6 +44,24 @@ def test_tree_and_moments_bit_exact(level, nb, seed):
     assert same.all()
 
 
+def test_tree_off_origin_large_mesh():
+    """20,480 triangles not straddling the origin: the reference's root bound
+    for 8,192+ boxes includes the origin; the GPU build's tree, moments and W
+    are still the reference's."""
+    V, F = D.bumpy_sphere(5, 16, 4)
+    V = V + np.array([10.0, -3.0, 2.5])
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
+    ref = _ref_soup(V, F)
+    assert np.array_equal(bvh.nodes(), ref.nodes())
+    assert np.array_equal(bvh.boxdata().view(np.uint32), ref.boxdata().view(np.uint32))
+    rng = np.random.default_rng(4)
+    Q = np.concatenate([rng.uniform(V.min(0) - 0.3, V.max(0) + 0.3, (20000, 3)),
+                        D.near_surface_queries(V, F, 20000, 1e-3, rng)])
+    W, Wr = bvh.query(Q, 2.0), ref.query(Q, 2.0)
+    assert np.abs(W - Wr).max() <= TOL_REF
+    assert _classify_same(W, Wr)
+
+
 def test_soup_cfg2_vs_reference():
     V, F, Q = D.soup_workload(nq=1_000_000)  # 81,920 tris, 1M queries (half near the surface)
     bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
