@@ -69,27 +69,56 @@ __global__ void k_parent(const uint32_t* __restrict__ nodes, int64_t nn, int32_t
   }
 }
 
-__global__ void k_depth(const int32_t* __restrict__ parent, int64_t nn, int32_t* __restrict__ depth,
-                        uint32_t* __restrict__ hist, int32_t* __restrict__ maxdepth) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  int d = 0;
-  int32_t p = parent[i];
-  while (p >= 0 && d < 4096) {
-    ++d;
-    p = parent[p];
+// Depth of every node and the per-depth counts.  Counts are gathered per
+// block in LDS first (depths below 64; deeper ones go straight to memory):
+// a global atomic per node on the few per-depth counters serialised.
+constexpr int kLdsDepths = 64;
+
+__global__ __launch_bounds__(256) void k_depth(const int32_t* __restrict__ parent, int64_t nn,
+                                               int32_t* __restrict__ depth, uint32_t* __restrict__ hist,
+                                               int32_t* __restrict__ maxdepth) {
+  __shared__ uint32_t h[kLdsDepths];
+  __shared__ int32_t dmax;
+  if (threadIdx.x < kLdsDepths) h[threadIdx.x] = 0;
+  if (threadIdx.x == 0) dmax = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) {
+    int d = 0;
+    int32_t p = parent[i];
+    while (p >= 0 && d < 4096) {
+      ++d;
+      p = parent[p];
+    }
+    depth[i] = d;
+    if (d < kLdsDepths)
+      atomicAdd(&h[d], 1u);
+    else
+      atomicAdd(&hist[d], 1u);
+    atomicMax(&dmax, d);
   }
-  depth[i] = d;
-  atomicAdd(&hist[d], 1u);
-  atomicMax(maxdepth, d);
+  __syncthreads();
+  if (threadIdx.x < kLdsDepths && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 0) atomicMax(maxdepth, dmax);
 }
 
-__global__ void k_bucket(const int32_t* __restrict__ depth, int64_t nn, uint32_t* __restrict__ cursor,
-                         int32_t* __restrict__ order) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  const uint32_t pos = atomicAdd(&cursor[depth[i]], 1u);
-  order[pos] = (int32_t)i;
+// order[] grouped by depth: each block reserves one range per depth with a
+// single atomic and places its nodes inside it (the order within a depth is
+// immaterial: the nodes of a level are processed independently)
+__global__ __launch_bounds__(256) void k_bucket(const int32_t* __restrict__ depth, int64_t nn,
+                                                uint32_t* __restrict__ cursor, int32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kLdsDepths], base[kLdsDepths];
+  if (threadIdx.x < kLdsDepths) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = i < nn ? depth[i] : -1;
+  uint32_t local = 0;
+  if (d >= 0 && d < kLdsDepths) local = atomicAdd(&cnt[d], 1u);
+  __syncthreads();
+  if (threadIdx.x < kLdsDepths && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (d >= kLdsDepths) order[atomicAdd(&cursor[d], 1u)] = (int32_t)i;
+  else if (d >= 0) order[base[d] + local] = (int32_t)i;
 }
 
 // ---------------------------------------------------------------------------
