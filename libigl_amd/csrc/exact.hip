@@ -5,12 +5,13 @@
 // through its WindingNumberAABB hierarchy (cap triangles for boxes that do
 // not contain the query); in real arithmetic both sums are equal.
 //
-// Kernel: one query per thread (QPT queries per thread for ILP), 256 threads;
-// triangles are staged through LDS in tiles of 256 (9 doubles each, 18 KB),
-// so every triangle is read from HBM once per block and broadcast from LDS
-// to all lanes.  FP64 VALU bound.  atan2 terms are summed and divided by 2 pi
-// once per query (the reference divides each term; the difference is a few
-// ulps of the FP64 result).
+// Kernel: one query per thread, 256 threads; triangles are staged through LDS
+// in tiles of 256 (9 doubles each, 18 KB), so every triangle is read from
+// HBM once per block and broadcast from LDS to all lanes.  A small query set
+// also splits the triangles into chunks (grid.y) so that the GPU is full;
+// the chunk partials are then added in chunk order.  FP64 VALU bound.  atan2
+// terms are summed and divided by 2 pi once per query (the reference divides
+// each term; the difference is a few ulps of the FP64 result).
 #include <hip/hip_runtime.h>
 
 #include "gwn_internal.h"
@@ -19,7 +20,7 @@ namespace gwn {
 
 constexpr int EX_THREADS = 256;
 constexpr int EX_TILE = 256;
-constexpr int EX_QPT = 2;
+constexpr int EX_QPT = 1;
 constexpr double EX_PI = 3.1415926535897932384626433832795;
 
 __device__ __forceinline__ double vos_atan2(const double* t, double px, double py, double pz) {
@@ -46,9 +47,14 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact_gather(const double* __res
     for (int a = 0; a < 3; ++a) T[9 * f + 3 * k + a] = V[3 * (int64_t)F[3 * f + k] + a];
 }
 
-__global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__ T, int64_t nf,
+// Block (bx, by): queries [bx * EX_THREADS * EX_QPT, ...) against the
+// triangles of chunk by, [by * chunk, (by + 1) * chunk); the partial sum of
+// query q over chunk by goes to part[by * no + q] (or to W / 2 pi directly
+// when there is one chunk).  Splitting the triangles gives a small query set
+// enough blocks to fill the GPU; the partials are added in chunk order.
+__global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__ T, int64_t nf, int64_t chunk,
                                                        const double* __restrict__ O, int64_t no,
-                                                       double* __restrict__ W) {
+                                                       double* __restrict__ part, double* __restrict__ W) {
   __shared__ double tile[EX_TILE * 9];
   const int64_t q0 = ((int64_t)blockIdx.x * EX_THREADS) * EX_QPT + threadIdx.x;
   double px[EX_QPT], py[EX_QPT], pz[EX_QPT], acc[EX_QPT];
@@ -60,8 +66,10 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__
     pz[k] = ok ? O[3 * q + 2] : 0.0;
     acc[k] = 0.0;
   }
-  for (int64_t f0 = 0; f0 < nf; f0 += EX_TILE) {
-    const int cnt = (int)((nf - f0) < EX_TILE ? (nf - f0) : EX_TILE);
+  const int64_t fb = (int64_t)blockIdx.y * chunk;
+  const int64_t fe = (fb + chunk) < nf ? (fb + chunk) : nf;
+  for (int64_t f0 = fb; f0 < fe; f0 += EX_TILE) {
+    const int cnt = (int)((fe - f0) < EX_TILE ? (fe - f0) : EX_TILE);
     __syncthreads();
     for (int e = threadIdx.x; e < cnt * 9; e += EX_THREADS) tile[e] = T[9 * f0 + e];
     __syncthreads();
@@ -73,8 +81,29 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__
   }
   for (int k = 0; k < EX_QPT; ++k) {
     const int64_t q = q0 + (int64_t)k * EX_THREADS;
-    if (q < no) W[q] = acc[k] / (2.0 * EX_PI);
+    if (q < no) {
+      if (part)
+        part[(int64_t)blockIdx.y * no + q] = acc[k];
+      else
+        W[q] = acc[k] / (2.0 * EX_PI);
+    }
   }
+}
+
+__global__ void k_exact_sum(const double* __restrict__ part, int64_t nchunks, int64_t no, double* __restrict__ W) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= no) return;
+  double s = 0.0;
+  for (int64_t c = 0; c < nchunks; ++c) s += part[c * no + q];
+  W[q] = s / (2.0 * EX_PI);
+}
+
+// Triangle chunks per launch: enough blocks for ~16 wavefronts per SIMD
+// (256 CUs x 4 SIMDs), chunks of at least 1024 triangles.
+static int64_t exact_chunks(int64_t nf, int64_t qblocks) {
+  const int64_t want = (16 * 1024 * 64 / EX_THREADS + qblocks - 1) / qblocks;
+  const int64_t most = (nf + 1023) / 1024;
+  return want < 1 ? 1 : (want < most ? want : most);
 }
 
 // ---- 2D: winding number of a curve of edges (winding_number.cpp:62-89 with
@@ -145,13 +174,23 @@ void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, con
     GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * no, st));
     return;
   }
-  tri_scratch.reserve(9 * (size_t)nf);
+  const int64_t qblocks = div_up(no, EX_THREADS * EX_QPT);
+  const int64_t nch = exact_chunks(nf, qblocks);
+  GWN_CHECK(nch <= 65535, GWN_ERR_INVALID, "exact: too many triangle chunks");
+  const int64_t chunk = (nf + nch - 1) / nch;
+  // scratch: the triangles (9 doubles each), then the partial sums
+  tri_scratch.reserve(9 * (size_t)nf + (nch > 1 ? (size_t)nch * (size_t)no : 0));
+  double* part = nch > 1 ? tri_scratch.p + 9 * (size_t)nf : nullptr;
   hipLaunchKernelGGL(k_exact_gather, dim3(div_up(nf, EX_THREADS)), dim3(EX_THREADS), 0, st, V, F, nf, tri_scratch.p);
   check_launch("k_exact_gather");
   ProfileScope prof("exact", st);
-  hipLaunchKernelGGL(k_exact, dim3(div_up(no, EX_THREADS * EX_QPT)), dim3(EX_THREADS), 0, st, tri_scratch.p, nf, O, no,
-                     W);
+  hipLaunchKernelGGL(k_exact, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, tri_scratch.p, nf, chunk,
+                     O, no, part, W);
   check_launch("k_exact");
+  if (part) {
+    hipLaunchKernelGGL(k_exact_sum, dim3(div_up(no, 256)), dim3(256), 0, st, part, nch, no, W);
+    check_launch("k_exact_sum");
+  }
 }
 
 }  // namespace gwn

@@ -53,6 +53,19 @@ def test_exact_ragged_sizes():
             assert np.abs(W - Wo).max() <= 1e-12, (nf, nq)
 
 
+def test_exact_split_over_triangle_chunks():
+    # few queries against thousands of triangles: the kernel splits the
+    # triangles into chunks (1,025 -> 2, 3,001 -> 3 of 1,001, 5,120 -> 5) and
+    # adds the partial sums afterwards
+    V, F = D.noisy_sphere(4, 0.01, 5)
+    for nf in (1025, 3001, 5120):
+        for nq in (1, 300, 4097):
+            Q = np.random.default_rng(nf + 7 * nq).uniform(-1.3, 1.3, (nq, 3))
+            W = gwn.winding_number(V, F[:nf], Q)
+            Wo = O.winding_number_bf(V, F[:nf], Q)
+            assert np.abs(W - Wo).max() <= 1e-12, (nf, nq)
+
+
 def test_exact_rejects_bad_index():
     V, F = D.noisy_sphere(1, 0.01, 3)
     F = F.copy()
