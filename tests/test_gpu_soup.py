@@ -62,14 +62,14 @@ def test_tree_off_origin_large_mesh():
     assert _classify_same(W, Wr)
 
 
-def test_soup_cfg2_vs_reference():
+def test_soup_81920tri_1Mq_vs_reference():
     V, F, Q = D.soup_workload(nq=1_000_000)  # 81,920 tris, 1M queries (half near the surface)
     bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
     W = bvh.query(Q, 2.0)
     ref = _ref_soup(V, F)
     Wr = ref.query(Q, 2.0)
     err = np.abs(W - Wr)
-    print(f"cfg2 beta=2: max|W-W_ref| {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}")
+    print(f"81,920 tris, 1M queries, beta=2: max|W-W_ref| {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}")
     assert err.max() <= TOL_REF
     assert _classify_same(W, Wr)
 
