@@ -439,6 +439,9 @@ def run_cloud(no_cpu):
     tb = time.perf_counter()
     c = gwn.PointCloudFWN(P, N, A, 2)
     build_s = time.perf_counter() - tb
+    tb = time.perf_counter()
+    c = gwn.PointCloudFWN(P, N, A, 2)  # again, now that the device code is loaded
+    build_warm_s = time.perf_counter() - tb
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream(dev).cuda_stream
     nq = Q.shape[0]
@@ -449,7 +452,8 @@ def run_cloud(no_cpu):
                             5, "cloud_traverse")
     W = dW.cpu().numpy()
     out = {"workload": "2M points, 160^3 grid queries, order 2, beta 2", "cells": int(c.ncells()),
-           "build_s": round(build_s, 4), "queries_per_s": round(nq / dt, 1), "kernel_ms": round(kms, 4)}
+           "build_s": round(build_s, 4), "build_warm_s": round(build_warm_s, 4), "queries_per_s": round(nq / dt, 1),
+           "kernel_ms": round(kms, 4)}
     if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

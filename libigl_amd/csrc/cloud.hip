@@ -203,13 +203,16 @@ __global__ void k_permute_points(const float* __restrict__ P, const float* __res
 // sorted by (cell ordinal, point index), the terms are laid out in that
 // order, and each cell's four sums are folded in it: a wavefront per cell on
 // levels of few (large) cells, a thread per cell on levels of many.
+// Block (j, y): cell j's pairs y * 64 + lane, stepping by gridDim.y * 64, so
+// that a level of few large cells still spreads over many blocks.
 __global__ void k_lvl_emit(const SplitCell* __restrict__ lvl, const uint32_t* __restrict__ off,
                            const int32_t* __restrict__ perm, uint64_t* __restrict__ keys) {
   const uint32_t j = blockIdx.x;
   const SplitCell sc = lvl[j];
   const int64_t cnt = sc.end - sc.start;
   const uint32_t o = off[j];
-  for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x)
+  const int64_t step = (int64_t)gridDim.y * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; t < cnt; t += step)
     keys[o + t] = ((uint64_t)j << 32) | (uint32_t)perm[sc.start + t];
 }
 
@@ -245,11 +248,36 @@ __global__ void k_lvl_fold(const SplitCell* __restrict__ lvl, uint32_t ncell, co
   CM[3 * (int64_t)sc.cell + 2] = m2 / at;
 }
 
-// One wavefront per cell (levels of few, large cells).  The lanes stage 512
-// terms at a time in LDS, the next 512 already in flight, and lanes 0-3 each
-// fold one of the four running sums in order: the same sequential FP32 sums,
-// one add per term and lane instead of a dependent load chain per term.
-constexpr int FOLD_CHUNK = 512;
+constexpr int FOLD_CHUNK = 512;   // terms staged in LDS per round
+constexpr int FOLD_GROUP = 16;    // float4 LDS reads in flight per lane
+
+// s + b[0] + b[1] + ... + b[n-1], added in that order (FP32).  The lane reads
+// 64 of its terms from LDS before adding them, so the LDS latency is paid
+// once per 64 dependent adds, not once per 4.
+__device__ __forceinline__ float fold_in_order(const float* b, int n, float s) {
+  const float4* b4 = reinterpret_cast<const float4*>(b);
+  const int n4 = n >> 2;
+  int g = 0;
+  for (; g + FOLD_GROUP <= n4; g += FOLD_GROUP) {
+    float4 q[FOLD_GROUP];
+#pragma unroll
+    for (int k = 0; k < FOLD_GROUP; ++k) q[k] = b4[g + k];
+#pragma unroll
+    for (int k = 0; k < FOLD_GROUP; ++k) {
+      s += q[k].x;
+      s += q[k].y;
+      s += q[k].z;
+      s += q[k].w;
+    }
+  }
+  for (int e = 4 * g; e < n; ++e) s += b[e];
+  return s;
+}
+
+// One wavefront per cell (levels of few, large cells).  The lanes stage
+// FOLD_CHUNK terms at a time in LDS, the next FOLD_CHUNK already in flight,
+// and lanes 0-3 each fold one of the four running sums in order: the same
+// sequential FP32 sums, one add per term and lane.
 __global__ __launch_bounds__(64) void k_lvl_fold_wave(const SplitCell* __restrict__ lvl,
                                                       const uint32_t* __restrict__ off,
                                                       const float4* __restrict__ terms, float* __restrict__ CM) {
@@ -281,18 +309,7 @@ __global__ __launch_bounds__(64) void k_lvl_fold_wave(const SplitCell* __restric
 #pragma unroll
     for (int u = 0; u < PER; ++u)
       if (nb + lane + 64 * u < cnt) r[u] = v[nb + lane + 64 * u];
-    if (lane < 4) {
-      const float* b = buf[lane];
-      int e = 0;
-      for (; e + 4 <= n; e += 4) {
-        const float4 q = *reinterpret_cast<const float4*>(b + e);
-        s += q.x;
-        s += q.y;
-        s += q.z;
-        s += q.w;
-      }
-      for (; e < n; ++e) s += b[e];
-    }
+    if (lane < 4) s = fold_in_order(buf[lane], n, s);
     __syncthreads();
   }
   const float at = __shfl(s, 0);
@@ -417,18 +434,7 @@ __global__ __launch_bounds__(64) void k_cm_fold_wave(const int32_t* __restrict__
     }
     __syncthreads();
     gather(base + FOLD_CHUNK);
-    if (lane < 4) {
-      const float* bb = buf[lane];
-      int e = 0;
-      for (; e + 4 <= n; e += 4) {
-        const float4 q = *reinterpret_cast<const float4*>(bb + e);
-        s += q.x;
-        s += q.y;
-        s += q.z;
-        s += q.w;
-      }
-      for (; e < n; ++e) s += bb[e];
-    }
+    if (lane < 4) s = fold_in_order(buf[lane], n, s);
     __syncthreads();
   }
   const float at = __shfl(s, 0);
@@ -761,29 +767,37 @@ static void cell_moments(gwn_cloud* c, bool fold_cm, const SplitCell* all, const
     hipLaunchKernelGGL(k_cell_empty, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->ccount.p, nt, c->CM.p,
                        c->R.p, c->EC.p);
     check_launch("k_cell_empty");
-    // split-cell centres, level by level
-    for (size_t d = 0; fold_cm && d + 1 < lvl_off.size(); ++d) {
+    // split-cell centres, level by level.  The work buffers are allocated
+    // once for the largest level: a level's cells are disjoint ranges of the
+    // points, so it has at most n (cell, point) pairs.
+    uint32_t maxm = 0;
+    for (size_t d = 0; d + 1 < lvl_off.size(); ++d) maxm = std::max(maxm, lvl_off[d + 1] - lvl_off[d]);
+    const bool any_level = fold_cm && maxm > 0;
+    const int64_t np = c->n;
+    DevBuf<uint32_t> cnt(any_level ? maxm : 0), off(any_level ? maxm : 0), tot(any_level ? 1 : 0);
+    DevBuf<uint8_t> sct(any_level ? scan_temp_bytes(maxm) : 0);
+    DevBuf<uint64_t> keys(any_level ? np : 0);
+    DevBuf<uint32_t> dummy(any_level ? np : 0);
+    DevBuf<float4> terms(any_level ? np : 0);
+    const size_t tb = any_level ? radix_sort_temp_bytes(np, true) : 0;
+    DevBuf<uint8_t> tmp(tb);
+    for (size_t d = 0; any_level && d + 1 < lvl_off.size(); ++d) {
       const uint32_t b = lvl_off[d], e = lvl_off[d + 1];
       if (e == b) continue;
       const uint32_t m = e - b;
-      DevBuf<uint32_t> cnt(m), off(m), tot(1);
       hipLaunchKernelGGL(k_split_counts, dim3(div_up(m, 256)), dim3(256), 0, st, all + b, m, cnt.p);
       check_launch("k_split_counts");
-      DevBuf<uint8_t> sct(scan_temp_bytes(m));
       exclusive_scan_u32(cnt.p, off.p, m, tot.p, sct.p, st);
       uint32_t T = 0;
       GWN_HIP(hipMemcpyAsync(&T, tot.p, 4, hipMemcpyDeviceToHost, st));
       GWN_HIP(hipStreamSynchronize(st));
-      DevBuf<uint64_t> keys(T);
-      DevBuf<uint32_t> dummy(T);
-      hipLaunchKernelGGL(k_lvl_emit, dim3(m), dim3(64), 0, st, all + b, off.p, c->perm.p, keys.p);
+      GWN_CHECK((int64_t)T <= np, GWN_ERR_INVALID, "cloud build: a level holds more pairs than points");
+      const unsigned ysplit = m < 4096 ? (unsigned)div_up(4096, m) : 1u;
+      hipLaunchKernelGGL(k_lvl_emit, dim3(m, ysplit), dim3(64), 0, st, all + b, off.p, c->perm.p, keys.p);
       check_launch("k_lvl_emit");
       int obits = 1;
       while ((1u << obits) < m) ++obits;
-      const size_t tb = radix_sort_temp_bytes(T, true);
-      DevBuf<uint8_t> tmp(tb);
       radix_sort_u64(keys.p, dummy.p, T, 32 + obits, tmp.p, tb, st);
-      DevBuf<float4> terms(T);
       hipLaunchKernelGGL(k_lvl_terms, dim3(div_up(T, 256)), dim3(256), 0, st, (int64_t)T, keys.p, c->P.p, c->A.p,
                          terms.p);
       check_launch("k_lvl_terms");
