@@ -96,10 +96,13 @@ __device__ double tri_closest(const double* p, const double* A, const double* B,
 
 // squared distance from p to a box (0 inside), AlignedBox::squaredExteriorDistance
 __device__ __forceinline__ double box_d2(const double* p, const float4& lo, const float4& hi) {
+  // per axis: l - p below the box, p - h above it, 0 inside.  For a box
+  // with l <= h at most one of the two differences is positive, so the max
+  // picks the same value as the three-way branch (and 0 for a NaN p).
   const double l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
   double s = 0.0;
   for (int k = 0; k < 3; ++k) {
-    const double e = p[k] < l[k] ? l[k] - p[k] : (p[k] > h[k] ? p[k] - h[k] : 0.0);
+    const double e = fmax(fmax(l[k] - p[k], p[k] - h[k]), 0.0);
     s += e * e;
   }
   return s;

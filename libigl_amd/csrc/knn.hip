@@ -79,10 +79,13 @@ __device__ __forceinline__ double kn_d2(const double* q, double x, double y, dou
 }
 
 __device__ __forceinline__ double kn_box_d2(const double* q, const float4& lo, const float4& hi) {
+  // per axis: l - q below the box, q - h above it, 0 inside.  For a box
+  // with l <= h at most one of the two differences is positive, so the max
+  // picks the same value as the three-way branch (and 0 for a NaN q).
   const double l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
   double s = 0.0;
   for (int k = 0; k < 3; ++k) {
-    const double e = q[k] < l[k] ? l[k] - q[k] : (q[k] > h[k] ? q[k] - h[k] : 0.0);
+    const double e = fmax(fmax(l[k] - q[k], q[k] - h[k]), 0.0);
     s += e * e;
   }
   return s;
