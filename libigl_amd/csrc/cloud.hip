@@ -625,7 +625,9 @@ __device__ __forceinline__ float expansion(float lx, float ly, float lz, const f
     const float d = (float)(1.0 / (double)r3);
     const float s = (float)(-3.0 / (double)r5);
     float acc = 0.0f;
+#pragma unroll
     for (int v = 0; v < 3; ++v)
+#pragma unroll
       for (int u = 0; u < 3; ++u) {
         float S = (loc[u] * loc[v]) * s;
         if (u == v) S += d;
@@ -636,10 +638,13 @@ __device__ __forceinline__ float expansion(float lx, float ly, float lz, const f
   if (NT > 12) {
     const float r7 = r5 * r * r;
     const float s = (float)(-3.0 / (double)r5);
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
       const float f15 = (float)(15.0 * (double)loc[i]);
       float acc = 0.0f;
+#pragma unroll
       for (int v = 0; v < 3; ++v)
+#pragma unroll
         for (int u = 0; u < 3; ++u) {
           float rcd = 0.0f;
           if (u == v) rcd += loc[i];
@@ -682,6 +687,11 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
     uint32_t i = 0, skip = 0;
     while (i < nrec) {
       const uint2 m = meta[i];
+      const float4 h = head[i];
+      // the record's meta and head in one scalar-cache round trip (head is
+      // not used on a leaf, but the compiler would otherwise wait for meta
+      // before requesting it)
+      asm volatile("" ::"s"(m.x), "s"(m.y), "s"(h.x), "s"(h.w));
       const bool leaf = (m.x & REC_LEAF) != 0;
       if (skip == i) {
         skip = i + 1;
@@ -693,7 +703,6 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
                          NAs[3 * k + 2]);
           }
         } else {
-          const float4 h = head[i];
           const float lx = h.x - qx, ly = h.y - qy, lz = h.z - qz;
           const float nrm = sqrtf(lx * lx + ly * ly + lz * lz);
           if ((double)nrm > beta * (double)h.w) {

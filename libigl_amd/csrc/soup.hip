@@ -664,13 +664,17 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
   while (i < nslots) {
     const double* r = s64 + 24 * (size_t)i;
     const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+    const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    // the link and the record's point and radius in one scalar-cache round
+    // trip (the compiler would otherwise wait for the link first)
+    asm volatile("" ::"s"(link), "s"(r0), "s"(r3));
     const bool leaf = (link & LINK_LEAF) != 0;
     if (skip == i) {
-      const double x = qx - r[0], y = qy - r[1], z = qz - r[2];
+      const double x = qx - r0, y = qy - r1, z = qz - r2;
       const double l2 = x * x + y * y + z * z;
       bool exact = leaf;
       skip = i + 1;
-      if (!(l2 <= r[3] * acc2)) {
+      if (!(l2 <= r3 * acc2)) {
         const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
             e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
         const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
