@@ -405,9 +405,6 @@ __global__ __launch_bounds__(256) void k_knn(const float4* __restrict__ rec, uin
   uint32_t skip = 0, i = 0;
   while (i < nrec) {
     const float4 lo = rec[2 * (size_t)i], hi = rec[2 * (size_t)i + 1];
-    // the whole 32-byte record in one scalar-cache round trip, not the link
-    // first and the box after it
-    asm volatile("" ::"s"(lo.x), "s"(lo.w), "s"(hi.z));
     const uint32_t link = __float_as_uint(lo.w);
     const bool leaf = (link & KN_LEAF) != 0;
     if (skip == i) {
