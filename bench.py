@@ -710,6 +710,30 @@ def run_curve(no_cpu):
                                          "exact_segments": round(ex, 2)}}
     Ao = co.angle(Qf[sub], 2.0)
     out["max_abs_dw_vs_oracle"] = float(np.abs(A[sub].astype(np.float64) - Ao).max() / (2 * np.pi))
+    # exact 2D winding number (igl::winding_number with F.cols() == 2) on the
+    # same curve: 20,000 of the queries against all 262,144 edges, FP64
+    Oe = np.ascontiguousarray(Q[::50][:20_000], dtype=np.float64)
+    gwn.winding_number(P, E, Oe)
+    nat.lib().gwn_profile_reset()
+    nat.lib().gwn_profile_enable(1)
+    te = time.perf_counter()
+    for _ in range(3):
+        We = gwn.winding_number(P, E, Oe)
+    ete = (time.perf_counter() - te) / 3
+    nat.lib().gwn_profile_enable(0)
+    ekms = _kernel_ms("exact2d")
+    ex2 = {"queries": int(Oe.shape[0]), "edges": int(E.shape[0]), "kernel_ms": round(ekms, 4),
+           "pairs_per_s": round(Oe.shape[0] * E.shape[0] / (ekms * 1e-3), 1) if ekms > 0 else None,
+           "call_s_with_host_copies": round(ete, 4)}
+    esub = np.arange(0, Oe.shape[0], 100)
+    te = time.perf_counter()
+    Weo = O.winding_number_2d(P, E, Oe[esub])
+    ecdt = time.perf_counter() - te
+    ex2["max_abs_dw_vs_oracle"] = float(np.abs(We[esub] - Weo).max())
+    if not no_cpu:
+        ex2["cpu_baseline"] = {"value": round(esub.size / ecdt, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": f"{esub.size} queries, signed_angle summed over every edge (restatement)"}
+    out["exact_2d"] = ex2
     if not no_cpu:
         # the reference has no parallel 2D wrapper: computeAngle called point by
         # point, as a caller would, on one core

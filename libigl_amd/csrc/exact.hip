@@ -90,16 +90,19 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact(const double* __restrict__
   }
 }
 
-__global__ void k_exact_sum(const double* __restrict__ part, int64_t nchunks, int64_t no, double* __restrict__ W) {
+// W[q] = the chunk partials of query q added in chunk order, divided by 2 pi
+// when the terms do not already carry it (3D)
+__global__ void k_exact_sum(const double* __restrict__ part, int64_t nchunks, int64_t no, bool div2pi,
+                            double* __restrict__ W) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= no) return;
   double s = 0.0;
   for (int64_t c = 0; c < nchunks; ++c) s += part[c * no + q];
-  W[q] = s / (2.0 * EX_PI);
+  W[q] = div2pi ? s / (2.0 * EX_PI) : s;
 }
 
-// Triangle chunks per launch: enough blocks for ~16 wavefronts per SIMD
-// (256 CUs x 4 SIMDs), chunks of at least 1024 triangles.
+// Triangle (or edge) chunks per launch: enough blocks for ~16 wavefronts per SIMD
+// (256 CUs x 4 SIMDs), chunks of at least 1024 items.
 static int64_t exact_chunks(int64_t nf, int64_t qblocks) {
   const int64_t want = (16 * 1024 * 64 / EX_THREADS + qblocks - 1) / qblocks;
   const int64_t most = (nf + 1023) / 1024;
@@ -133,22 +136,32 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact2d_gather(const double* __r
     for (int a = 0; a < 2; ++a) T[4 * f + 2 * k + a] = V[2 * (int64_t)E[2 * f + k] + a];
 }
 
-__global__ __launch_bounds__(EX_THREADS) void k_exact2d(const double* __restrict__ T, int64_t ne,
+// Block (bx, by): queries of block bx against the edges of chunk by, as in
+// k_exact; the terms already carry the 1 / 2 pi, so the partials are summed
+// as they are.
+__global__ __launch_bounds__(EX_THREADS) void k_exact2d(const double* __restrict__ T, int64_t ne, int64_t chunk,
                                                          const double* __restrict__ O, int64_t no,
-                                                         double* __restrict__ W) {
+                                                         double* __restrict__ part, double* __restrict__ W) {
   __shared__ double tile[EX_TILE * 4];
   const int64_t q = (int64_t)blockIdx.x * EX_THREADS + threadIdx.x;
   const bool ok = q < no;
   const double px = ok ? O[2 * q] : 0.0, py = ok ? O[2 * q + 1] : 0.0;
   double acc = 0.0;
-  for (int64_t f0 = 0; f0 < ne; f0 += EX_TILE) {
-    const int cnt = (int)((ne - f0) < EX_TILE ? (ne - f0) : EX_TILE);
+  const int64_t fb = (int64_t)blockIdx.y * chunk;
+  const int64_t fe = (fb + chunk) < ne ? (fb + chunk) : ne;
+  for (int64_t f0 = fb; f0 < fe; f0 += EX_TILE) {
+    const int cnt = (int)((fe - f0) < EX_TILE ? (fe - f0) : EX_TILE);
     __syncthreads();
     for (int e = threadIdx.x; e < cnt * 4; e += EX_THREADS) tile[e] = T[4 * f0 + e];
     __syncthreads();
     for (int j = 0; j < cnt; ++j) acc += signed_angle_2d(tile + 4 * j, px, py);
   }
-  if (ok) W[q] = acc;
+  if (ok) {
+    if (part)
+      part[(int64_t)blockIdx.y * no + q] = acc;
+    else
+      W[q] = acc;
+  }
 }
 
 void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no, double* W,
@@ -158,12 +171,23 @@ void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double*
     GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * no, st));
     return;
   }
-  scratch.reserve(4 * (size_t)ne);
+  const int64_t qblocks = div_up(no, EX_THREADS);
+  const int64_t nch = exact_chunks(ne, qblocks);
+  GWN_CHECK(nch <= 65535, GWN_ERR_INVALID, "exact 2D: too many edge chunks");
+  const int64_t chunk = (ne + nch - 1) / nch;
+  // scratch: the edges (4 doubles each), then the partial sums
+  scratch.reserve(4 * (size_t)ne + (nch > 1 ? (size_t)nch * (size_t)no : 0));
+  double* part = nch > 1 ? scratch.p + 4 * (size_t)ne : nullptr;
   hipLaunchKernelGGL(k_exact2d_gather, dim3(div_up(ne, EX_THREADS)), dim3(EX_THREADS), 0, st, V, E, ne, scratch.p);
   check_launch("k_exact2d_gather");
   ProfileScope prof("exact2d", st);
-  hipLaunchKernelGGL(k_exact2d, dim3(div_up(no, EX_THREADS)), dim3(EX_THREADS), 0, st, scratch.p, ne, O, no, W);
+  hipLaunchKernelGGL(k_exact2d, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, scratch.p, ne, chunk,
+                     O, no, part, W);
   check_launch("k_exact2d");
+  if (part) {
+    hipLaunchKernelGGL(k_exact_sum, dim3(div_up(no, 256)), dim3(256), 0, st, part, nch, no, false, W);
+    check_launch("k_exact_sum");
+  }
 }
 
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
@@ -188,7 +212,7 @@ void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, con
                      O, no, part, W);
   check_launch("k_exact");
   if (part) {
-    hipLaunchKernelGGL(k_exact_sum, dim3(div_up(no, 256)), dim3(256), 0, st, part, nch, no, W);
+    hipLaunchKernelGGL(k_exact_sum, dim3(div_up(no, 256)), dim3(256), 0, st, part, nch, no, true, W);
     check_launch("k_exact_sum");
   }
 }
