@@ -33,15 +33,22 @@ BETA_GRID = [2.0, 3.0, 3.5, 4.0, 4.25, 4.5, 4.6, 4.7, 4.8, 4.9, 5.0, 5.25, 5.5, 
 FLOP_TEST = 10    # |q-P|^2 <= maxP2 * beta^2 per child slot (7179-7191)
 FLOP_FAR = 83     # accepted order-2 far-field slot (7202-7241)
 FLOP_EXACT = 76   # exact triangle (6078-6109)
-# HBM bytes moved per k_traverse launch on the default workload (1M queries,
-# SAH4 tree), keyed by accuracy_scale: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads
-# for a 256 MiB device copy (fetch x2.000, write x1.000).
-MEASURED_TRAFFIC = {4.9: int((181.0 + 33.4) * 2**20)}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+_STREAM = None
+
+
+def _stream():
+    """The one HIP stream every timed launch of this run goes on."""
+    global _STREAM
+    if _STREAM is None:
+        from libigl_amd import devmem as dm
+        _STREAM = dm.Stream()
+    return _STREAM
 
 
 def main():
@@ -64,15 +71,22 @@ def main():
     ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run 3 device-to-device copies of this many bytes (counter calibration)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="with --beta: after the timed steps, 3 device-to-device copies of this many bytes")
+    ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
+                    help="measure the headline kernel's HBM bytes with two rocprofv3 counter passes at the end")
+    ap.add_argument("--no-boundary", action="store_true", help="skip the host-array call timings")
     args = ap.parse_args()
     if args.copy_calibration > 0:
-        import torch
+        import numpy as np
+
+        from libigl_amd import devmem as dm
         n = args.copy_calibration // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
+        src = dm.DeviceArray.from_numpy(np.ones(n, np.float32))
+        dst = dm.DeviceArray((n,), np.float32)
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+            dst.copy_from(src)  # hipMemcpy device to device: the runtime's copy kernel
+        dm.synchronize()
         print(json.dumps({"mode": "copy", "bytes": 4 * n, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
@@ -81,34 +95,32 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     from libigl_amd import datagen as D
+    from libigl_amd import devmem as dm
     from libigl_amd import gwn
     from libigl_amd import _native as nat
 
     if args.rows_only:
-        torch.cuda.synchronize()
+        dm.synchronize()
         print(json.dumps({"rows": run_rows(args.rows, args.no_cpu)}), flush=True)
         return
-
-    dev = torch.device("cuda:0")
     t0 = time.time()
     V, F, Q = D.soup_workload(nq=args.nq)
     log(f"[bench] workload: {F.shape[0]} tris, {Q.shape[0]} queries ({time.time() - t0:.1f}s)")
 
     bvh = gwn.FastWindingNumberBVH()
-    torch.cuda.synchronize()
+    dm.synchronize()
     tb = time.perf_counter()
     tree = gwn.TREE_SAH4 if args.tree == "sah4" else gwn.TREE_LBVH
     bvh.build(V, F, 2, tree)
-    torch.cuda.synchronize()
+    dm.synchronize()
     build_s = time.perf_counter() - tb
     # the first build of the process also loads the library's device code;
     # a second build of the same mesh times the build alone
     tb = time.perf_counter()
     gwn.FastWindingNumberBVH().build(V, F, 2, tree)
-    torch.cuda.synchronize()
+    dm.synchronize()
     build_warm_s = time.perf_counter() - tb
     info = bvh.info()
     log(f"[bench] {args.tree.upper()} tree: {info.num_nodes} nodes, {info.num_slots} slots, depth {info.depth}, "
@@ -116,38 +128,45 @@ def main():
 
     nq = Q.shape[0]
     Qf = Q.astype(np.float32)
-    dQ = torch.from_numpy(Qf).to(dev)
-    dW = torch.empty(nq, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    dQ = dm.DeviceArray.from_numpy(Qf)
+    dW = dm.DeviceArray((nq,), np.float64)
+    stream = _stream()
 
     def run(beta, prec=gwn.PREC_FP32):
-        bvh.query_device(dQ.data_ptr(), nq, dW.data_ptr(), beta, prec, stream.cuda_stream)
+        bvh.query_device(dQ.data_ptr, nq, dW.data_ptr, beta, prec, stream.ptr)
 
     if args.beta > 0:  # profiling mode: only the timed steps at one beta
         for _ in range(args.warmup):
             run(args.beta)
-        torch.cuda.synchronize()
+        dm.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
             run(args.beta)
-        torch.cuda.synchronize()
+        dm.synchronize()
         dt = (time.perf_counter() - t) / args.steps
+        if args.calibrate_copy > 0:
+            n = args.calibrate_copy // 4
+            src = dm.DeviceArray.from_numpy(np.ones(n, np.float32))
+            dst = dm.DeviceArray((n,), np.float32)
+            for _ in range(3):
+                dst.copy_from(src)
+            dm.synchronize()
         print(json.dumps({"mode": "profile", "beta": args.beta, "tree": args.tree, "ms_per_step": 1e3 * dt,
                           "queries_per_s": nq / dt}), flush=True)
         return
 
     # FP64 exact reference on the GPU for all queries
     # the oracle sees the original FP64 mesh and queries (the FWN path narrows both to float)
-    dV = torch.from_numpy(np.ascontiguousarray(V, dtype=np.float64)).to(dev)
-    dF = torch.from_numpy(F).to(dev)
-    dQ64 = torch.from_numpy(np.ascontiguousarray(Q, dtype=np.float64)).to(dev)
-    dWx = torch.empty(nq, dtype=torch.float64, device=dev)
+    dV = dm.DeviceArray.from_numpy(np.ascontiguousarray(V, dtype=np.float64))
+    dF = dm.DeviceArray.from_numpy(F)
+    dQ64 = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q, dtype=np.float64))
+    dWx = dm.DeviceArray((nq,), np.float64)
     te = time.perf_counter()
-    nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr(), V.shape[0], dF.data_ptr(), F.shape[0],
-                                                       dQ64.data_ptr(), nq, dWx.data_ptr(), stream.cuda_stream))
-    torch.cuda.synchronize()
+    nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0],
+                                                       dQ64.data_ptr, nq, dWx.data_ptr, stream.ptr))
+    dm.synchronize()
     exact_s = time.perf_counter() - te
-    Wx = dWx.cpu().numpy()
+    Wx = dWx.numpy()
     log(f"[bench] FP64 exact (GPU) for all queries: {exact_s:.2f}s")
 
     # confirm the GPU exact sum against the CPU FP64 oracle on a subsample
@@ -167,8 +186,8 @@ def main():
     beta_acc = None
     for beta in BETA_GRID:
         run(beta)
-        torch.cuda.synchronize()
-        e = np.abs(dW.cpu().numpy() - Wx)
+        dm.synchronize()
+        e = np.abs(dW.numpy() - Wx)
         err_at[beta] = (float(e.max()), float(np.quantile(e, 0.99)))
         log(f"[bench] beta {beta}: max|dw| {err_at[beta][0]:.3e} p99 {err_at[beta][1]:.3e}")
         if err_at[beta][0] <= ACC_TARGET:
@@ -180,13 +199,13 @@ def main():
     def timed(beta, steps, warmup, prec=gwn.PREC_FP32):
         for _ in range(warmup):
             run(beta, prec)
-        torch.cuda.synchronize()
+        dm.synchronize()
         nat.lib().gwn_profile_reset()
         nat.lib().gwn_profile_enable(1)
         t = time.perf_counter()
         for _ in range(steps):
             run(beta, prec)
-        torch.cuda.synchronize()
+        dm.synchronize()
         dt = time.perf_counter() - t
         nat.lib().gwn_profile_enable(0)
         import ctypes
@@ -206,11 +225,11 @@ def main():
     dtm, km_ms, _ = timed(beta_acc, max(3, args.steps // 2), 1, gwn.PREC_MIXED)
     dtd, kd_ms, _ = timed(beta_acc, max(3, args.steps // 4), 1, gwn.PREC_FP64)
     run(beta_acc, gwn.PREC_MIXED)
-    torch.cuda.synchronize()
-    err_mixed = float(np.abs(dW.cpu().numpy() - Wx).max())
+    dm.synchronize()
+    err_mixed = float(np.abs(dW.numpy() - Wx).max())
     run(beta_acc, gwn.PREC_FP64)
-    torch.cuda.synchronize()
-    err_fp64 = float(np.abs(dW.cpu().numpy() - Wx).max())
+    dm.synchronize()
+    err_fp64 = float(np.abs(dW.numpy() - Wx).max())
 
     # counted FLOPs per query from the oracle's instrumented traversal of the
     # same tree on a sample of the queries
@@ -229,29 +248,14 @@ def main():
     except Exception as e:
         log(f"[bench] flop count skipped: {e}")
 
-    roofline = None
-    if flops_per_q is not None and k_ms > 0:
-        achieved = flops_per_q * nq / (k_ms * 1e-3) / 1e12
-        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
-                    "frac": round(achieved / 157.3, 4),
-                    "traffic": (MEASURED_TRAFFIC.get(beta_acc)
-                                if (nq == 1_000_000 and args.tree == "sah4") else None),
-                    "hbm_GBps": (round(MEASURED_TRAFFIC[beta_acc] / (k_ms * 1e-3) / 1e9, 1)
-                                 if (beta_acc in MEASURED_TRAFFIC and nq == 1_000_000 and args.tree == "sah4")
-                                 else None),
-                    "hbm_peak_GBps": 8000.0,
-                    "kernel": "k_traverse<2,false>", "kernel_ms": round(k_ms, 4),
-                    "flop_per_query": round(flops_per_q, 1),
-                    "note": "FP32 VALU; no MFMA on this path; packed FP32 disabled so 78.6 TFLOP/s is the "
-                            "attainable scalar-FP32 ceiling"}
-
     cpu = None
+    host = host_info()
     if not args.no_cpu:
         try:
-            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            threads = cpu_threads(args)
             os.environ["IGL_NUM_THREADS"] = str(threads)
-            if O.ref_available():
-                ref = O.RefSoup(V, F)
+            if O.ref_available(release=True):
+                ref = O.RefSoup(V, F, release=True)
                 kind = "reference"
             else:
                 ref = O.Soup(V, F)
@@ -269,13 +273,22 @@ def main():
                 if sum(runs) > 20:
                     break
             cdt = float(np.median(runs))
-            nthr = O.ref().ref_num_threads() if kind == "reference" else threads
+            nthr = O.ref(True).ref_num_threads() if kind == "reference" else threads
             cpu = {"value": round(csub / cdt, 1), "unit": "queries/s", "cores": int(nthr), "kind": kind,
                    "sample": f"{csub} queries of the same workload at beta={beta_acc}, median of {len(runs)}; "
-                             f"UT_SolidAngle + igl::parallel_for"}
+                             f"UT_SolidAngle + igl::parallel_for built with the reference's Release flags "
+                             f"(-O3 -DNDEBUG); host: {host['cpu_model']}, {host['nproc']} cores visible, "
+                             f"IGL_NUM_THREADS={threads}"}
             log(f"[bench] CPU ({kind}, {nthr} threads): {csub / cdt:.3e} q/s")
         except Exception as e:
             log(f"[bench] cpu baseline skipped: {e}")
+
+    boundary = None
+    if not args.no_boundary:
+        try:
+            boundary = run_boundary(bvh, Q, beta_acc)
+        except Exception as e:
+            log(f"[bench] boundary timing skipped: {e}")
 
     mixed_1m_tri = None
     if not args.no_mixed:
@@ -293,13 +306,31 @@ def main():
         except Exception as e:
             log(f"[bench] serving skipped: {e}")
 
+    traffic, traffic_note = None, "not measured (--traffic off)"
+    if args.traffic != "off":
+        traffic, traffic_note = measure_traffic(args, beta_acc)
+        log(f"[bench] measured traffic: {traffic} ({traffic_note})")
+
+    roofline = None
+    if flops_per_q is not None and k_ms > 0:
+        achieved = flops_per_q * nq / (k_ms * 1e-3) / 1e12
+        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
+                    "frac": round(achieved / 157.3, 4), "traffic": traffic,
+                    "hbm_GBps": round(traffic / (k_ms * 1e-3) / 1e9, 1) if traffic else None,
+                    "hbm_peak_GBps": 8000.0, "traffic_note": traffic_note,
+                    "kernel": "k_traverse<2,false>", "kernel_ms": round(k_ms, 4),
+                    "flop_per_query": round(flops_per_q, 1),
+                    "note": "FP32 VALU; no MFMA on this path; packed FP32 disabled so 78.6 TFLOP/s is the "
+                            "attainable scalar-FP32 ceiling; FLOPs counted on the tree the kernel walks"}
+
+    tree_label = "SAH4 (reference topology, host build)" if args.tree == "sah4" else "LBVH (device build)"
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded generator)",
         "config": {"workload": "soup fast_winding_number, level-6 icosphere 81,920 tris, 16 bumps, 1M queries "
                                "(50% uniform, 50% within 1e-3 of surface), order 2",
-                   "accuracy_scale": beta_acc, "tree": "SAH4 (reference topology)", "parallelism": "single GPU"},
+                   "accuracy_scale": beta_acc, "tree": tree_label, "parallelism": "single GPU"},
         "accuracy": {"max_abs_dw_vs_fp64": err_at[beta_acc][0], "p99_abs_dw_vs_fp64": err_at[beta_acc][1],
                      "gpu_exact_vs_cpu_oracle": oracle_check, "by_beta": {str(k): v for k, v in err_at.items()}},
         "beta2_queries_per_s": round(qps_b2, 1),
@@ -308,10 +339,136 @@ def main():
         "fp64": {"queries_per_s": round(nq / (dtd / max(3, args.steps // 4)), 1), "max_abs_dw": err_fp64,
                  "kernel_ms": round(kd_ms, 4)},
         "build_s": round(build_s, 4), "build_warm_s": round(build_warm_s, 4), "sort_ms": round(sort_ms, 4),
-        "roofline": roofline, "cpu_baseline": cpu, "serving": serving, "mixed_vs_fp64": mixed_1m_tri,
-        "other_rows": others,
+        "roofline": roofline, "cpu_baseline": cpu, "host": host, "boundary": boundary, "serving": serving,
+        "mixed_vs_fp64": mixed_1m_tri, "other_rows": others,
     }
     print(json.dumps(out), flush=True)
+
+
+def host_info():
+    """The host the CPU legs run on: cores this process may use, os.cpu_count
+    and the CPU model (/proc/cpuinfo, as lscpu reports it)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count()
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_threads(args):
+    """Threads for the CPU legs: --cpu-threads, else the cores this process
+    may use, at most 16 (one GPU's share of the GPU box's host)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    return max(1, min(16, host_info()["nproc"] or 1))
+
+
+def run_boundary(bvh, Q, beta):
+    """The drop-in call as users make it: igl::fast_winding_number(bvh,
+    accuracy_scale, Q, W) with host arrays (gwn_soup_query: staging copies,
+    narrowing on the device, the query, the copy back), at nq = 1, 1e3 and
+    1e6; and 8 host threads querying one BVH at once, one point per call, as
+    signed_distance.cpp:90-121 does inside igl::parallel_for."""
+    import threading
+
+    import numpy as np
+    out = {}
+    for n, reps in ((1, 400), (1000, 100), (1_000_000, 5)):
+        Qn = np.ascontiguousarray(Q[:n])
+        bvh.query(Qn, beta)
+        t = time.perf_counter()
+        for _ in range(reps):
+            bvh.query(Qn, beta)
+        dt = (time.perf_counter() - t) / reps
+        out[f"nq_{n}"] = {"call_ms": round(1e3 * dt, 4), "queries_per_s": round(n / dt, 1)}
+    per, nth = 250, 8
+    Wt = np.zeros(per * nth)
+    errs = []
+
+    def worker(k):
+        try:
+            for i in range(k * per, (k + 1) * per):
+                Wt[i] = bvh.query(Q[i:i + 1], beta)[0]
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(nth)]
+    t = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    dt = time.perf_counter() - t
+    same = (not errs) and bool(np.array_equal(Wt, bvh.query(Q[:per * nth], beta)))
+    out["threads_8_single_point"] = {"calls": per * nth, "calls_per_s": round(per * nth / dt, 1),
+                                     "bit_equal_to_batch": same}
+    log(f"[bench] boundary: {out}")
+    return out
+
+
+def measure_traffic(args, beta):
+    """HBM bytes per k_traverse<2,false> launch of the headline run, measured
+    now: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch) over a child run of this script at the same accuracy_scale and
+    tree, which also makes three 256 MiB device-to-device copies; each counter
+    is scaled by what it reads for the copy (256 MiB read and 256 MiB
+    written).  Returns (bytes, note) or (None, reason)."""
+    import csv
+    import glob
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    rp = shutil.which("rocprofv3")
+    if rp is None:
+        return None, "rocprofv3 not on PATH"
+    copy_bytes = 1 << 28
+    child = [sys.executable, os.path.abspath(__file__), "--beta", repr(float(beta)), "--steps", "2", "--warmup", "1",
+             "--tree", args.tree, "--nq", str(args.nq), "--calibrate-copy", str(copy_bytes)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    kre = re.compile(r"k_traverse<\s*2\s*,\s*false\s*>")
+    vals = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="gwn_pmc_", dir="/tmp")
+        cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--"] + child
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            return None, f"{ctr} pass timed out"
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None, f"{ctr}: no counter file (rocprofv3 rc {r.returncode}: {r.stderr[-300:]!r})"
+        kern, copy = [], []
+        with open(files[0], newline="") as f:
+            for row in csv.DictReader(f):
+                name, v = row.get("Kernel_Name", ""), row.get("Counter_Value")
+                if v is None or row.get("Counter_Name", ctr) != ctr:
+                    continue
+                if kre.search(name):
+                    kern.append(float(v))
+                elif "copyBuffer" in name:
+                    copy.append(float(v))
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not copy:
+            return None, f"{ctr}: kernel or copy dispatches missing from the counter file"
+        vals[ctr] = (sum(kern) / len(kern), sum(copy) / len(copy))
+    fk, fc = vals["FETCH_SIZE"]
+    wk, wc = vals["WRITE_SIZE"]
+    fscale, wscale = (copy_bytes / 1024) / fc, (copy_bytes / 1024) / wc
+    total = int(round((fk * fscale + wk * wscale) * 1024))
+    note = (f"rocprofv3 --pmc in this run: FETCH_SIZE {fk:.0f} KiB x {fscale:.3f} + WRITE_SIZE {wk:.0f} KiB x "
+            f"{wscale:.3f} per dispatch (scales from a 256 MiB device copy)")
+    return total, note
 
 
 def run_rows(rows, no_cpu):
@@ -334,9 +491,9 @@ def run_mixed_workload(beta_grid):
     (1,310,720 triangles, 64 bumps), 10M queries (80% uniform, 20% within 1e-4
     of the surface).  |dw| against the FP64 exact sum on a 100k-query sample."""
     import numpy as np
-    import torch
 
     from libigl_amd import datagen as D
+    from libigl_amd import devmem as dm
     from libigl_amd import gwn
     from libigl_amd import _native as nat
 
@@ -347,28 +504,27 @@ def run_mixed_workload(beta_grid):
     tb = time.perf_counter()
     bvh.build(V, F, 2, gwn.TREE_SAH4)
     build_s = time.perf_counter() - tb
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev)
-    dQ = torch.from_numpy(Q.astype(np.float32)).to(dev)
-    dW = torch.empty(nq, dtype=torch.float64, device=dev)
+    st = _stream()
+    dQ = dm.DeviceArray.from_numpy(Q.astype(np.float32))
+    dW = dm.DeviceArray((nq,), np.float64)
     sub = np.arange(0, nq, nq // 100_000)
-    dV = torch.from_numpy(np.ascontiguousarray(V, dtype=np.float64)).to(dev)
-    dF = torch.from_numpy(F).to(dev)
-    dQs = torch.from_numpy(np.ascontiguousarray(Q[sub], dtype=np.float64)).to(dev)
-    dWx = torch.empty(sub.size, dtype=torch.float64, device=dev)
-    nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr(), V.shape[0], dF.data_ptr(), F.shape[0],
-                                                       dQs.data_ptr(), sub.size, dWx.data_ptr(), st.cuda_stream))
-    torch.cuda.synchronize()
-    Wx = dWx.cpu().numpy()
+    dV = dm.DeviceArray.from_numpy(np.ascontiguousarray(V, dtype=np.float64))
+    dF = dm.DeviceArray.from_numpy(F)
+    dQs = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q[sub], dtype=np.float64))
+    dWx = dm.DeviceArray((sub.size,), np.float64)
+    nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0],
+                                                       dQs.data_ptr, sub.size, dWx.data_ptr, st.ptr))
+    dm.synchronize()
+    Wx = dWx.numpy()
     log(f"[bench] mixed workload: {F.shape[0]} tris, {nq} queries, build {build_s:.2f}s, setup {time.time() - t0:.1f}s")
 
     def run(beta, prec):
-        bvh.query_device(dQ.data_ptr(), nq, dW.data_ptr(), beta, prec, st.cuda_stream)
+        bvh.query_device(dQ.data_ptr, nq, dW.data_ptr, beta, prec, st.ptr)
 
     def err(beta, prec):
         run(beta, prec)
-        torch.cuda.synchronize()
-        e = np.abs(dW.cpu().numpy()[sub] - Wx)
+        dm.synchronize()
+        e = np.abs(dW.numpy()[sub] - Wx)
         return float(e.max()), float(np.quantile(e, 0.99))
 
     beta = beta_grid[-1]
@@ -379,11 +535,11 @@ def run_mixed_workload(beta_grid):
 
     def rate(prec, steps):
         run(beta, prec)
-        torch.cuda.synchronize()
+        dm.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
             run(beta, prec)
-        torch.cuda.synchronize()
+        dm.synchronize()
         return nq * steps / (time.perf_counter() - t)
 
     out = {"workload": "level-8 icosphere 1,310,720 tris, 64 bumps, 10M queries (80% uniform, 20% within 1e-4)",
@@ -409,17 +565,17 @@ def _kernel_ms(name):
 def _timed_device(fn, steps, kernel):
     """warm-up call, then `steps` calls between synchronizations; returns
     (seconds per call, average kernel ms from HIP events)."""
-    import torch
+    from libigl_amd import devmem as dm
 
     from libigl_amd import _native as nat
     fn()
-    torch.cuda.synchronize()
+    dm.synchronize()
     nat.lib().gwn_profile_reset()
     nat.lib().gwn_profile_enable(1)
     t = time.perf_counter()
     for _ in range(steps):
         fn()
-    torch.cuda.synchronize()
+    dm.synchronize()
     dt = (time.perf_counter() - t) / steps
     nat.lib().gwn_profile_enable(0)
     return dt, _kernel_ms(kernel)
@@ -429,9 +585,9 @@ def run_cloud(no_cpu):
     """Point-cloud path: 2M oriented points on the noisy level-5 sphere, 160^3
     grid queries, order 2, beta 2; W checked against the oracle on a sample."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
     from libigl_amd import gwn
 
@@ -442,15 +598,14 @@ def run_cloud(no_cpu):
     tb = time.perf_counter()
     c = gwn.PointCloudFWN(P, N, A, 2)  # again, now that the device code is loaded
     build_warm_s = time.perf_counter() - tb
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
+    st = _stream().ptr
     nq = Q.shape[0]
-    dQ = torch.from_numpy(Q.astype(np.float32)).to(dev)
-    dW = torch.empty(nq, dtype=torch.float32, device=dev)
-    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_cloud_query_device(c._h, dQ.data_ptr(), nq, 2.0,
-                                                                               dW.data_ptr(), st)),
+    dQ = dm.DeviceArray.from_numpy(Q.astype(np.float32))
+    dW = dm.DeviceArray((nq,), np.float32)
+    dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_cloud_query_device(c._h, dQ.data_ptr, nq, 2.0,
+                                                                               dW.data_ptr, st)),
                             5, "cloud_traverse")
-    W = dW.cpu().numpy()
+    W = dW.numpy()
     out = {"workload": "2M points, 160^3 grid queries, order 2, beta 2", "cells": int(c.ncells()),
            "build_s": round(build_s, 4), "build_warm_s": round(build_warm_s, 4), "queries_per_s": round(nq / dt, 1),
            "kernel_ms": round(kms, 4)}
@@ -459,7 +614,7 @@ def run_cloud(no_cpu):
         import oracle_py as O
         oc = O.Cloud(P.astype(np.float32), N.astype(np.float32), A.astype(np.float32), 2, np.float32)
         sub = np.arange(0, nq, nq // 20_000)
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, host_info()['nproc'] or 1))
         t = time.perf_counter()
         Wo = oc.eval(Q[sub].astype(np.float32), 2.0, threads)
         cdt = time.perf_counter() - t
@@ -474,22 +629,21 @@ def run_exact(no_cpu):
     """Exact FP64 winding number: level-5 noisy icosphere (20,480 triangles),
     100k queries uniform in [-1.5,1.5]^3."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
 
     V, F, Q = D.exact_workload()
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
-    dV = torch.from_numpy(np.ascontiguousarray(V)).to(dev)
-    dF = torch.from_numpy(F).to(dev)
-    dQ = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
-    dW = torch.empty(Q.shape[0], dtype=torch.float64, device=dev)
+    st = _stream().ptr
+    dV = dm.DeviceArray.from_numpy(np.ascontiguousarray(V))
+    dF = dm.DeviceArray.from_numpy(F)
+    dQ = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q))
+    dW = dm.DeviceArray((Q.shape[0],), np.float64)
     dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_exact_winding_number_device(
-        dV.data_ptr(), V.shape[0], dF.data_ptr(), F.shape[0], dQ.data_ptr(), Q.shape[0], dW.data_ptr(), st)),
+        dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0], dQ.data_ptr, Q.shape[0], dW.data_ptr, st)),
         5, "exact")
-    W = dW.cpu().numpy()
+    W = dW.numpy()
     pairs = Q.shape[0] * F.shape[0]
     out = {"workload": "20,480 tris, 100k queries, FP64", "queries_per_s": round(Q.shape[0] / dt, 1),
            "kernel_ms": round(kms, 4), "pairs_per_s": round(pairs / (kms * 1e-3), 1)}
@@ -497,13 +651,15 @@ def run_exact(no_cpu):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         sub = np.arange(0, Q.shape[0], 50)
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, host_info()['nproc'] or 1))
         t = time.perf_counter()
         Wo = O.winding_number_bf(V, F, Q[sub], threads)
         cdt = time.perf_counter() - t
         out["max_abs_dw_vs_oracle"] = float(np.abs(W[sub] - Wo).max())
         out["cpu_baseline"] = {"value": round(sub.size / cdt, 1), "unit": "queries/s", "cores": threads,
-                               "kind": "port", "sample": f"{sub.size} queries, brute-force FP64 restatement"}
+                               "kind": "port",
+                               "sample": f"{sub.size} queries; brute-force FP64 port (every triangle per query), "
+                                         "not the reference's WindingNumberAABB hierarchy"}
     log(f"[bench] exact: {out}")
     return out
 
@@ -512,26 +668,25 @@ def run_signed_distance(no_cpu):
     """igl::signed_distance with the fast-winding-number sign on the 81,920-
     triangle workload mesh, 1M points (closest point + FWN sign)."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
     from libigl_amd import gwn
 
     V, F, Q = D.soup_workload()
     bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
+    st = _stream().ptr
     n = Q.shape[0]
-    dP = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
-    dS = torch.empty(n, dtype=torch.float64, device=dev)
-    dI = torch.empty(n, dtype=torch.int32, device=dev)
-    dC = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    dP = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q))
+    dS = dm.DeviceArray((n,), np.float64)
+    dI = dm.DeviceArray((n,), np.int32)
+    dC = dm.DeviceArray((n, 3), np.float64)
     lo, hi = gwn.DBL_MIN, gwn.DBL_MAX
     dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_signed_distance_device(
-        bvh.handle, dP.data_ptr(), n, gwn.SIGN_FAST_WINDING_NUMBER, lo, hi, dS.data_ptr(), dI.data_ptr(),
-        dC.data_ptr(), st)), 3, "closest_point")
-    S = dS.cpu().numpy()
+        bvh.handle, dP.data_ptr, n, gwn.SIGN_FAST_WINDING_NUMBER, lo, hi, dS.data_ptr, dI.data_ptr,
+        dC.data_ptr, st)), 3, "closest_point")
+    S = dS.numpy()
     out = {"workload": "81,920 tris, 1M points, FAST_WINDING_NUMBER sign", "points_per_s": round(n / dt, 1),
            "closest_point_kernel_ms": round(kms, 4)}
     if not no_cpu:
@@ -550,29 +705,28 @@ def run_point_areas(no_cpu):
     k = 21 over all points, point areas, and the whole estimated-area cloud
     build (coincident-point filter, knn, areas, octree, moments)."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
     from libigl_amd import gwn
 
     P, N, _, _ = D.cloud_workload()
     n, k = P.shape[0], 21
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
-    dP = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
-    dN = torch.from_numpy(np.ascontiguousarray(N)).to(dev)
-    dI = torch.empty((n, k), dtype=torch.int32, device=dev)
-    dA = torch.empty(n, dtype=torch.float64, device=dev)
-    dT = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    st = _stream().ptr
+    dP = dm.DeviceArray.from_numpy(np.ascontiguousarray(P))
+    dN = dm.DeviceArray.from_numpy(np.ascontiguousarray(N))
+    dI = dm.DeviceArray((n, k), np.int32)
+    dA = dm.DeviceArray((n,), np.float64)
+    dT = dm.DeviceArray((n, 3), np.float64)
     dt_knn, kms_knn = _timed_device(lambda: nat.check(nat.lib().gwn_knn_device(
-        dP.data_ptr(), n, dP.data_ptr(), n, k, None, 0, dI.data_ptr(), st)), 3, "knn")
+        dP.data_ptr, n, dP.data_ptr, n, k, None, 0, dI.data_ptr, st)), 3, "knn")
     dt_pa, kms_pa = _timed_device(lambda: nat.check(nat.lib().gwn_point_areas_device(
-        dP.data_ptr(), n, dI.data_ptr(), k, dN.data_ptr(), dA.data_ptr(), dT.data_ptr(), st)), 3, "point_areas")
+        dP.data_ptr, n, dI.data_ptr, k, dN.data_ptr, dA.data_ptr, dT.data_ptr, st)), 3, "point_areas")
     tb = time.perf_counter()
     c = gwn.PointCloudFWN.with_estimated_areas(P, N, 2)
     build_s = time.perf_counter() - tb
-    I, A = dI.cpu().numpy(), dA.cpu().numpy()
+    I, A = dI.numpy(), dA.numpy()
     out = {"workload": "2M oriented surface points (noisy level-5 sphere), k = 21",
            "knn_points_per_s": round(n / dt_knn, 1), "knn_kernel_ms": round(kms_knn, 4),
            "point_areas_points_per_s": round(n / dt_pa, 1), "point_areas_kernel_ms": round(kms_pa, 4),
@@ -584,7 +738,7 @@ def run_point_areas(no_cpu):
     if not no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, host_info()['nproc'] or 1))
         tree = O.Octree(P, np.float64)
         sub = np.arange(0, n, 50)
         t = time.perf_counter()
@@ -612,9 +766,9 @@ def run_antipodal(no_cpu):
     is the signed ray-crossing count), 1M queries; checked on a sample against
     the exact FP64 sum."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
     from libigl_amd import gwn
 
@@ -622,14 +776,13 @@ def run_antipodal(no_cpu):
     tb = time.perf_counter()
     scene = gwn.WindingNumberAntipodalScene(V, F)
     build_s = time.perf_counter() - tb
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
+    st = _stream().ptr
     n = Q.shape[0]
-    dQ = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
-    dW = torch.empty(n, dtype=torch.float64, device=dev)
+    dQ = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q))
+    dW = dm.DeviceArray((n,), np.float64)
     dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_antipodal_query_device(
-        scene._h, dQ.data_ptr(), n, dW.data_ptr(), st)), 5, "antipodal_crossings")
-    W = dW.cpu().numpy()
+        scene._h, dQ.data_ptr, n, dW.data_ptr, st)), 5, "antipodal_crossings")
+    W = dW.numpy()
     out = {"workload": "81,920 tris (closed), 1M queries, FP64", "queries_per_s": round(n / dt, 1),
            "crossings_kernel_ms": round(kms, 4), "build_s": round(build_s, 4),
            "integer_valued": bool(np.all(W == np.round(W)))}
@@ -637,7 +790,7 @@ def run_antipodal(no_cpu):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         sub = np.arange(0, n, 1000)
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, host_info()['nproc'] or 1))
         We = O.winding_number_bf(V, F, Q[sub], threads)
         out["max_abs_dw_vs_exact"] = float(np.abs(W[sub] - We).max())
         t = time.perf_counter()
@@ -665,26 +818,25 @@ def run_curve(no_cpu):
     default); checked on a sample against the oracle (bit-exact with the
     reference) and against exact evaluation."""
     import numpy as np
-    import torch
 
     from libigl_amd import _native as nat
+    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
     from libigl_amd import gwn
 
     P, E, Q = D.curve_workload()
     n = Q.shape[0]
-    torch.cuda.synchronize()
+    dm.synchronize()
     tb = time.perf_counter()
     c = gwn.SubtendedAngle(P, E, 2)
     build_s = time.perf_counter() - tb
-    dev = torch.device("cuda:0")
-    st = torch.cuda.current_stream(dev).cuda_stream
+    st = _stream().ptr
     Qf = np.ascontiguousarray(Q, dtype=np.float32)
-    dQ = torch.from_numpy(Qf).to(dev)
-    dA = torch.empty(n, dtype=torch.float32, device=dev)
+    dQ = dm.DeviceArray.from_numpy(Qf)
+    dA = dm.DeviceArray((n,), np.float32)
     dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_curve_query_device(
-        c._h, dQ.data_ptr(), n, 2.0, dA.data_ptr(), st)), 20, "curve_traverse")
-    A = dA.cpu().numpy()
+        c._h, dQ.data_ptr, n, 2.0, dA.data_ptr, st)), 20, "curve_traverse")
+    A = dA.numpy()
     info = c.info()
     out = {"workload": "bumpy closed curve, 262,144 segments, 1M queries (50% uniform, 50% within 1e-3), "
                        "order 2, accuracy_scale 2, FP32", "queries_per_s": round(n / dt, 1),
@@ -735,20 +887,28 @@ def run_curve(no_cpu):
                                "sample": f"{esub.size} queries, signed_angle summed over every edge (restatement)"}
     out["exact_2d"] = ex2
     if not no_cpu:
-        # the reference has no parallel 2D wrapper: computeAngle called point by
-        # point, as a caller would, on one core
-        if O.ref_available():
-            ref, kind = O.RefCurve(P, E), "reference"
-        else:
-            ref, kind = co, "port"
+        # the reference has no batch 2D wrapper: computeAngle per point, spread
+        # over the host cores with the reference's own igl::parallel_for, from
+        # the Release-flag build of the reference
+        threads = max(1, min(16, host_info()['nproc'] or 1))
+        os.environ["IGL_NUM_THREADS"] = str(threads)
         csub = np.arange(0, n, 4)
-        t = time.perf_counter()
-        Ar = ref.angle(Qf[csub], 2.0) if kind == "reference" else ref.angle(Qf[csub], 2.0, 1)
-        cdt = time.perf_counter() - t
+        if O.ref_available(release=True):
+            ref = O.RefCurveRelease(P, E)
+            t = time.perf_counter()
+            Ar = ref.angle_parallel(Qf[csub], 2.0)
+            cdt = time.perf_counter() - t
+            kind, cores = "reference", O.ref(True).ref_num_threads()
+        else:
+            t = time.perf_counter()
+            Ar = co.angle(Qf[csub], 2.0, threads)
+            cdt = time.perf_counter() - t
+            kind, cores = "port", threads
         out["matches_reference_sample_max_abs_dw"] = float(np.abs(A[csub].astype(np.float64) - Ar).max() / (2 * np.pi))
-        out["cpu_baseline"] = {"value": round(csub.size / cdt, 1), "unit": "queries/s", "cores": 1, "kind": kind,
-                               "sample": f"{csub.size} queries (every 4th) of the same workload, computeAngle "
-                                         "per point (serial)"}
+        out["cpu_baseline"] = {"value": round(csub.size / cdt, 1), "unit": "queries/s", "cores": int(cores),
+                               "kind": kind,
+                               "sample": f"{csub.size} queries (every 4th) of the same workload, computeAngle per "
+                                         "point under igl::parallel_for, Release flags"}
     log(f"[bench] curve: {out}")
     return out
 
@@ -804,7 +964,7 @@ def run_serving(duration, beta_acc):
     try:  # the reference's CPU path on a sample of the same requests, same host
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(16, host_info()['nproc'] or 1))
         os.environ["IGL_NUM_THREADS"] = str(threads)
         if O.ref_available():
             r2 = np.random.default_rng(9)

@@ -10,6 +10,12 @@
  *   - *_device entry points take device pointers and a hipStream_t (passed as
  *     void*, NULL = the default stream) and are asynchronous;
  *     the others take host pointers and return when the result is in place;
+ *   - queries take a handle by const pointer and are reentrant and thread-safe,
+ *     as the reference's const queries are (fast_winding_number.h:191-206):
+ *     any number of threads and streams may query one handle at once.  Each
+ *     call borrows its temporary device memory from a pool that orders reuse
+ *     by stream events, so concurrent calls share no scratch and, once the
+ *     pool is warm, a call allocates nothing;
  *   - an empty mesh gives W = 0 (the reference returns an uninitialised value,
  *     FastWindingNumberForSoups.h:7281-7283).
  */
@@ -42,10 +48,17 @@ typedef enum {
 
 /* Arithmetic of a soup query. */
 typedef enum {
-  GWN_PREC_FP32 = 0,  /* as the reference: FP32 far field and FP32 atan2 */
-  /* FP32 far field; FP64 VOS solid angle for triangles within near_dist of the
-   * query; FP64 running sum; queries with |w-0.5| < ambiguous_band are
-   * re-evaluated entirely in FP64. */
+  /* FP32 throughout, as the reference (same near/far decisions, same FP32
+   * moments); the far field is the reference's Taylor polynomial with its
+   * coefficients pre-combined, 1/|q| from the hardware reciprocal square
+   * root, and the exact angle uses a fitted polynomial atan2 (about 1e-7
+   * rad, the size of atan2f's own rounding) with a hardware reciprocal. */
+  GWN_PREC_FP32 = 0,
+  /* FP32 far field and FP32 running sum; the VOS solid angles of triangles
+   * that may lie within near_dist of the query are computed and summed in
+   * FP64 (added to the FP32 sum at the end, in FP64); queries with
+   * |w-0.5| < ambiguous_band are then re-evaluated entirely in FP64
+   * (moments, far field, atan2, sum). */
   GWN_PREC_MIXED = 1,
   GWN_PREC_FP64 = 2   /* everything in FP64 (moments, far field, atan2) */
 } gwn_precision;
@@ -110,8 +123,10 @@ int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info);
 
 /* Replaces igl::fast_winding_number(fwn_bvh, accuracy_scale, Q, W),
  * fast_winding_number.h:191-196 (fast_winding_number.cpp:418-437): each query
- * is narrowed to float, W = Omega / (4 pi) in double.  The single-point
- * overload (fast_winding_number.h:203-206) is nq = 1. */
+ * is narrowed to float (on the device), W = Omega / (4 pi) in double.  The
+ * single-point overload (fast_winding_number.h:203-206) is nq = 1: batches
+ * under 4,096 queries skip the query sort and walk the tree one lane per
+ * query, so a single point costs one kernel and two small copies. */
 int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accuracy_scale, int precision,
                    double* W);
 /* Device form: Q float nq x 3, W double nq. */
@@ -286,8 +301,7 @@ int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, i
  * gwn_curve_query replaces computeAngle(query_point, accuracy_scale)
  * (7700-7828) for nq queries (Q nq x 2 float): angle[i] is the signed angle
  * the curve subtends at Q[i] (the winding number is angle / 2 pi); an empty
- * curve gives 0.  Concurrent queries on one handle must not overlap (the
- * handle's query workspace is shared).
+ * curve gives 0.
  */
 typedef struct gwn_curve gwn_curve;
 typedef struct {

@@ -290,20 +290,17 @@ void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no,
     check_launch("k_ap_zero");
   }
   const gwn_soup* s = &a->soup;
-  if (s->nf == 0 || s->nslots == 0) {
-    GWN_HIP(hipStreamSynchronize(st));
-    return;
-  }
-  DevBuf<float> Of(3 * (size_t)no);
-  DevBuf<uint32_t> keys(no), perm(no);
-  hipLaunchKernelGGL(k_ap_narrow, dim3(div_up(3 * no, 256)), dim3(256), 0, st, O, no, Of.p);
+  if (s->nf == 0 || s->nslots == 0) return;
+  const size_t tb = radix_sort_temp_bytes(no, false);
+  Scratch ws(scratch_bytes({12 * (size_t)no, 4 * (size_t)no, 4 * (size_t)no, tb}), st);
+  float* Of = ws.take_n<float>(3 * (size_t)no);
+  uint32_t* keys = ws.take_n<uint32_t>((size_t)no);
+  uint32_t* perm = ws.take_n<uint32_t>((size_t)no);
+  void* tmp = ws.take(tb);
+  hipLaunchKernelGGL(k_ap_narrow, dim3(div_up(3 * no, 256)), dim3(256), 0, st, O, no, Of);
   check_launch("k_ap_narrow");
-  hilbert_keys(Of.p, no, s->bbox, keys.p, perm.p, st);
-  {
-    const size_t tb = radix_sort_temp_bytes(no, false);
-    DevBuf<uint8_t> tmp(tb);
-    radix_sort_u32(keys.p, perm.p, no, 30, tmp.p, tb, st);
-  }
+  hilbert_keys(Of, no, s->bbox, keys, perm, st);
+  radix_sort_u32(keys, perm, no, 30, tmp, tb, st);
   Ray r;
   int kz = 0;
   for (int k = 1; k < 3; ++k)
@@ -323,10 +320,9 @@ void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no,
   {
     ProfileScope prof("antipodal_crossings", st);
     hipLaunchKernelGGL(k_ap_crossings, dim3(g), dim3(256), 0, st, s->dbox.p, (uint32_t)s->nslots, s->V64.p, s->F.p,
-                       O, perm.p, no, r, W);
+                       O, perm, no, r, W);
     check_launch("k_ap_crossings");
   }
-  GWN_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace gwn

@@ -29,9 +29,9 @@ void ensure_device() {
 
 // defined in the .hip translation units
 void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no, double* W,
-                    hipStream_t st, DevBuf<double>& scratch);
+                    hipStream_t st);
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
-                  hipStream_t st, DevBuf<double>& tri_scratch);
+                  hipStream_t st);
 void soup_setup(gwn_soup* s, hipStream_t st);
 void soup_finish_build(gwn_soup* s, hipStream_t st);
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
@@ -63,6 +63,19 @@ template <typename T>
 static void upload(DevBuf<T>& d, const T* h, size_t n, hipStream_t st) {
   d.alloc(n);
   if (n) GWN_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+}
+
+// Host-array entry points stage their inputs and outputs in pooled scratch
+// (no allocation per call once warm) on the calling thread's stream.
+template <typename T>
+static T* stage_in(Scratch& ws, const T* h, size_t n, hipStream_t st) {
+  T* d = ws.take_n<T>(n);
+  if (n) GWN_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+  return d;
+}
+template <typename T>
+static void stage_out(T* h, const T* d, size_t n, hipStream_t st) {
+  if (n) GWN_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st));
 }
 
 static void check_faces(const int32_t* F, int64_t nf, int64_t nv) {
@@ -161,8 +174,7 @@ int gwn_exact_winding_number_device(const double* V, int64_t nv, const int32_t* 
     GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
     GWN_CHECK(nf == 0 || (V && F), GWN_ERR_INVALID, "null mesh");
     ensure_device();
-    static thread_local DevBuf<double> scratch;
-    exact_device(V, nv, F, nf, O, no, W, (hipStream_t)stream, scratch);
+    exact_device(V, nv, F, nf, O, no, W, (hipStream_t)stream);
   });
 }
 
@@ -178,14 +190,14 @@ int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int6
     }
     ensure_device();
     hipStream_t st = nullptr;
-    DevBuf<double> dV, dO, dW, scratch;
-    DevBuf<int32_t> dF;
-    upload(dV, V, 3 * (size_t)nv, st);
-    upload(dF, F, 3 * (size_t)nf, st);
-    upload(dO, O, 3 * (size_t)no, st);
-    dW.alloc(no);
-    exact_device(dV.p, nv, dF.p, nf, dO.p, no, dW.p, st, scratch);
-    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    const size_t v = 3 * (size_t)nv, f = 3 * (size_t)nf, o = 3 * (size_t)no;
+    Scratch io(scratch_bytes({8 * v, 4 * f, 8 * o, 8 * (size_t)no}), st);
+    const double* dV = stage_in(io, V, v, st);
+    const int32_t* dF = stage_in(io, F, f, st);
+    const double* dO = stage_in(io, O, o, st);
+    double* dW = io.take_n<double>((size_t)no);
+    exact_device(dV, nv, dF, nf, dO, no, dW, st);
+    stage_out(W, dW, (size_t)no, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -243,7 +255,7 @@ int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info) {
     info->depth = s->depth;
     info->device_bytes = (int64_t)(s->U.bytes() + s->F.bytes() + s->tri.bytes() + s->boxes.bytes() +
                                    s->nodes.bytes() + s->boxdata.bytes() + s->slots.bytes() + s->slots64.bytes() +
-                                   s->V64.bytes() + s->dbox.bytes() + s->work.bytes());
+                                   s->V64.bytes() + s->dbox.bytes());
     std::memcpy(info->bbox, s->bbox, sizeof(s->bbox));
   });
 }
@@ -274,15 +286,16 @@ int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accurac
     GWN_CHECK(nq == 0 || (Q && W), GWN_ERR_INVALID, "null query/output");
     GWN_CHECK(precision >= GWN_PREC_FP32 && precision <= GWN_PREC_FP64, GWN_ERR_INVALID, "unknown precision");
     if (nq == 0) return;
-    // fast_winding_number.cpp:431-434: each query coordinate narrowed to float
-    std::vector<float> q(3 * (size_t)nq);
-    for (size_t i = 0; i < q.size(); ++i) q[i] = (float)Q[i];
     hipStream_t st = nullptr;
-    DevBuf<float> dQ;
-    DevBuf<double> dW(nq);
-    upload(dQ, q.data(), q.size(), st);
-    soup_query_device(s, dQ.p, nq, accuracy_scale, precision, dW.p, st);
-    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
+    const size_t n = (size_t)nq;
+    Scratch io(scratch_bytes({24 * n, 12 * n, 8 * n}), st);
+    const double* dQ64 = stage_in(io, Q, 3 * n, st);
+    float* dQ = io.take_n<float>(3 * n);
+    double* dW = io.take_n<double>(n);
+    // fast_winding_number.cpp:431-434: each query coordinate narrowed to float
+    to_float_device(dQ64, 3 * nq, dQ, st);
+    soup_query_device(s, dQ, nq, accuracy_scale, precision, dW, st);
+    stage_out(W, dW, n, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -413,7 +426,7 @@ int gwn_curve_get_info(const gwn_curve* c, gwn_curve_info* info) {
     info->tree_kind = c->tree_kind;
     info->depth = c->depth;
     info->device_bytes = (int64_t)(c->P.bytes() + c->E.bytes() + c->seg.bytes() + c->boxes.bytes() +
-                                   c->nodes.bytes() + c->boxdata.bytes() + c->slots.bytes() + c->work.bytes());
+                                   c->nodes.bytes() + c->boxdata.bytes() + c->slots.bytes());
     std::memcpy(info->bbox, c->bbox, sizeof(c->bbox));
   });
 }
@@ -435,10 +448,11 @@ int gwn_curve_query(const gwn_curve* c, const float* Q, int64_t nq, float accura
     GWN_CHECK(nq == 0 || (Q && angle), GWN_ERR_INVALID, "null query/output");
     if (nq == 0) return;
     hipStream_t st = nullptr;
-    DevBuf<float> dQ, dA(nq);
-    upload(dQ, Q, 2 * (size_t)nq, st);
-    curve_query_device(c, dQ.p, nq, accuracy_scale, dA.p, st);
-    GWN_HIP(hipMemcpyAsync(angle, dA.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    Scratch io(scratch_bytes({8 * (size_t)nq, 4 * (size_t)nq}), st);
+    const float* dQ = stage_in(io, Q, 2 * (size_t)nq, st);
+    float* dA = io.take_n<float>((size_t)nq);
+    curve_query_device(c, dQ, nq, accuracy_scale, dA, st);
+    stage_out(angle, dA, (size_t)nq, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -491,13 +505,14 @@ int gwn_exact_winding_number_2d(const double* V, int64_t nv, const int32_t* E, i
     }
     ensure_device();
     hipStream_t st = nullptr;
-    DevBuf<double> dV, dO, dW(no), scratch;
-    DevBuf<int32_t> dE;
-    upload(dV, V, 2 * (size_t)nv, st);
-    upload(dE, E, 2 * (size_t)ne, st);
-    upload(dO, O, 2 * (size_t)no, st);
-    exact2d_device(dV.p, dE.p, ne, dO.p, no, dW.p, st, scratch);
-    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    const size_t v = 2 * (size_t)nv, e = 2 * (size_t)ne, o = 2 * (size_t)no;
+    Scratch io(scratch_bytes({8 * v, 4 * e, 8 * o, 8 * (size_t)no}), st);
+    const double* dV = stage_in(io, V, v, st);
+    const int32_t* dE = stage_in(io, E, e, st);
+    const double* dO = stage_in(io, O, o, st);
+    double* dW = io.take_n<double>((size_t)no);
+    exact2d_device(dV, dE, ne, dO, no, dW, st);
+    stage_out(W, dW, (size_t)no, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -554,13 +569,16 @@ int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type,
     sd_check(s, np, sign_type, P, S, I, C);
     if (np == 0) return;
     hipStream_t st = nullptr;
-    DevBuf<double> dP, dS(np), dC(3 * (size_t)np);
-    DevBuf<int32_t> dI(np);
-    upload(dP, P, 3 * (size_t)np, st);
-    signed_distance_device(s, dP.p, np, sign_type, lower_bound, upper_bound, dS.p, dI.p, dC.p, st);
-    GWN_HIP(hipMemcpyAsync(S, dS.p, sizeof(double) * np, hipMemcpyDeviceToHost, st));
-    GWN_HIP(hipMemcpyAsync(I, dI.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st));
-    GWN_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
+    const size_t n = (size_t)np;
+    Scratch io(scratch_bytes({24 * n, 8 * n, 4 * n, 24 * n}), st);
+    const double* dP = stage_in(io, P, 3 * n, st);
+    double* dS = io.take_n<double>(n);
+    int32_t* dI = io.take_n<int32_t>(n);
+    double* dC = io.take_n<double>(3 * n);
+    signed_distance_device(s, dP, np, sign_type, lower_bound, upper_bound, dS, dI, dC, st);
+    stage_out(S, dS, n, st);
+    stage_out(I, dI, n, st);
+    stage_out(C, dC, 3 * n, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -603,10 +621,11 @@ int gwn_antipodal_query(const gwn_antipodal* a, const double* O, int64_t no, dou
     if (no == 0) return;
     GWN_CHECK(O && W, GWN_ERR_INVALID, "null input");
     hipStream_t st = nullptr;
-    DevBuf<double> dO, dW(no);
-    upload(dO, O, 3 * (size_t)no, st);
-    antipodal_query_device(a, dO.p, no, dW.p, st);
-    GWN_HIP(hipMemcpyAsync(W, dW.p, sizeof(double) * no, hipMemcpyDeviceToHost, st));
+    Scratch io(scratch_bytes({24 * (size_t)no, 8 * (size_t)no}), st);
+    const double* dO = stage_in(io, O, 3 * (size_t)no, st);
+    double* dW = io.take_n<double>((size_t)no);
+    antipodal_query_device(a, dO, no, dW, st);
+    stage_out(W, dW, (size_t)no, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }
@@ -874,10 +893,11 @@ int gwn_cloud_query(const gwn_cloud* c, const float* Q, int64_t nq, double beta,
     GWN_CHECK(!c->octree_only, GWN_ERR_INVALID, "cloud was built without normals and areas");
     if (nq == 0) return;
     hipStream_t st = nullptr;
-    DevBuf<float> dQ, dW(nq);
-    upload(dQ, Q, 3 * (size_t)nq, st);
-    cloud_query_device(c, dQ.p, nq, beta, dW.p, st);
-    GWN_HIP(hipMemcpyAsync(WN, dW.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    Scratch io(scratch_bytes({12 * (size_t)nq, 4 * (size_t)nq}), st);
+    const float* dQ = stage_in(io, Q, 3 * (size_t)nq, st);
+    float* dW = io.take_n<float>((size_t)nq);
+    cloud_query_device(c, dQ, nq, beta, dW, st);
+    stage_out(WN, dW, (size_t)nq, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
 }

@@ -1204,11 +1204,10 @@ void cloud_build_from_octree_device(gwn_cloud* c, const int32_t* hCH, int64_t nc
   rope_records(c, all.p, lvl_off, dls.p, st);
 }
 
-void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st) {
-  gwn_cloud* c = const_cast<gwn_cloud*>(cc);
+void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st) {
   if (nq <= 0) return;
   if (c->n == 0) {
-    GWN_HIP(hipMemsetAsync(WN, 0, sizeof(float) * nq, st));
+    fill_device<float>(WN, nq, 0.0f, st);
     return;
   }
   const unsigned g = div_up(nq, 256);
@@ -1218,12 +1217,12 @@ void cloud_query_device(const gwn_cloud* cc, const float* Q, int64_t nq, double 
     check_launch("k_cloud_brute");
     return;
   }
+  // scratch of this call: Hilbert keys, the query permutation, sort temp
   const size_t tb = radix_sort_temp_bytes(nq, false);
-  const size_t need = 2 * ((4 * (size_t)nq + 255) / 256 * 256) + (tb + 255) / 256 * 256;
-  c->work.reserve(need);
-  uint32_t* keys = (uint32_t*)c->work.p;
-  uint32_t* perm = keys + (4 * (size_t)nq + 255) / 256 * 64;
-  void* tmp = (uint8_t*)perm + (4 * (size_t)nq + 255) / 256 * 256;
+  Scratch ws(scratch_bytes({4 * (size_t)nq, 4 * (size_t)nq, tb}), st);
+  uint32_t* keys = ws.take_n<uint32_t>((size_t)nq);
+  uint32_t* perm = ws.take_n<uint32_t>((size_t)nq);
+  void* tmp = ws.take(tb);
   hilbert_keys(Q, nq, c->bbox, keys, perm, st);
   radix_sort_u32(keys, perm, nq, 30, tmp, tb, st);
   const bool rl = c->root_leaf;

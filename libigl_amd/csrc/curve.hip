@@ -434,26 +434,25 @@ void curve_build_device(gwn_curve* c, hipStream_t st) {
   GWN_HIP(hipStreamSynchronize(st));
 }
 
-void curve_query_device(const gwn_curve* cc, const float* Q, int64_t nq, float accuracy_scale, float* angle,
+void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
                         hipStream_t st) {
-  gwn_curve* c = const_cast<gwn_curve*>(cc);
   if (nq <= 0) return;
   if (c->nslots == 0) {
-    GWN_HIP(hipMemsetAsync(angle, 0, sizeof(float) * nq, st));
+    fill_device<float>(angle, nq, 0.0f, st);
     return;
   }
   // Large batches are walked in Hilbert order (neighbouring lanes then follow
   // similar paths); below kCurveSortMin the sort costs more than it saves
   // (65,536 queries: 32 us unsorted, 88 us sorted).
   uint32_t* perm = nullptr;
-  if (nq >= kCurveSortMin) {
-    const size_t a = 256;
-    auto up = [&](size_t b) { return (b + a - 1) / a * a; };
-    const size_t tmpb = radix_sort_temp_bytes(nq, false);
-    c->work.reserve(2 * up(4 * (size_t)nq) + up(tmpb));
-    uint32_t* keys = (uint32_t*)c->work.p;
-    perm = (uint32_t*)(c->work.p + up(4 * (size_t)nq));
-    void* tmp = c->work.p + 2 * up(4 * (size_t)nq);
+  const bool sorted = nq >= kCurveSortMin;
+  const size_t tmpb = sorted ? radix_sort_temp_bytes(nq, false) : 0;
+  // scratch of this call (sorted batches): keys, the permutation, sort temp
+  Scratch ws(sorted ? scratch_bytes({4 * (size_t)nq, 4 * (size_t)nq, tmpb}) : 1, st);
+  if (sorted) {
+    uint32_t* keys = ws.take_n<uint32_t>((size_t)nq);
+    perm = ws.take_n<uint32_t>((size_t)nq);
+    void* tmp = ws.take(tmpb);
     const float ex = std::max(c->bbox[2] - c->bbox[0], 1e-30f), ey = std::max(c->bbox[3] - c->bbox[1], 1e-30f);
     hipLaunchKernelGGL(k_order2, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, c->bbox[0], c->bbox[1],
                        256.0f / ex, 256.0f / ey, keys, perm);

@@ -246,7 +246,7 @@ __global__ void k_sd_empty(int64_t np, double* __restrict__ S, int32_t* __restri
 }  // namespace
 
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
-                  hipStream_t st, DevBuf<double>& tri_scratch);
+                  hipStream_t st);
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
                        hipStream_t st);
 
@@ -270,39 +270,39 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
   soup_ensure_dist(s, st);
   double low2, up2;
   sqr_bounds(lower, upper, low2, up2);
-  DevBuf<float> Pf(3 * (size_t)np);
-  DevBuf<uint32_t> keys(np), perm(np);
-  DevBuf<double> best(np), cl(3 * (size_t)np), W;
-  DevBuf<int32_t> face(np);
-  hipLaunchKernelGGL(k_narrow3, dim3(div_up(3 * np, 256)), dim3(256), 0, st, P, np, Pf.p);
+  const bool want_w = sign_type == GWN_SIGN_FAST_WINDING_NUMBER || sign_type == GWN_SIGN_WINDING_NUMBER ||
+                      sign_type == GWN_SIGN_DEFAULT;
+  const size_t tb = radix_sort_temp_bytes(np, false);
+  const size_t n = (size_t)np;
+  Scratch ws(scratch_bytes({12 * n, 4 * n, 4 * n, tb, 8 * n, 24 * n, 4 * n, want_w ? 8 * n : 8}), st);
+  float* Pf = ws.take_n<float>(3 * n);
+  uint32_t* keys = ws.take_n<uint32_t>(n);
+  uint32_t* perm = ws.take_n<uint32_t>(n);
+  void* tmp = ws.take(tb);
+  double* best = ws.take_n<double>(n);
+  double* cl = ws.take_n<double>(3 * n);
+  int32_t* face = ws.take_n<int32_t>(n);
+  double* W = want_w ? ws.take_n<double>(n) : nullptr;
+  hipLaunchKernelGGL(k_narrow3, dim3(div_up(3 * np, 256)), dim3(256), 0, st, P, np, Pf);
   check_launch("k_narrow3");
-  hilbert_keys(Pf.p, np, s->bbox, keys.p, perm.p, st);
-  {
-    const size_t tb = radix_sort_temp_bytes(np, false);
-    DevBuf<uint8_t> tmp(tb);
-    radix_sort_u32(keys.p, perm.p, np, 30, tmp.p, tb, st);
-  }
+  hilbert_keys(Pf, np, s->bbox, keys, perm, st);
+  radix_sort_u32(keys, perm, np, 30, tmp, tb, st);
   {
     ProfileScope prof("closest_point", st);
     hipLaunchKernelGGL(k_closest, dim3(g), dim3(256), 0, st, s->dbox.p, (uint32_t)s->nslots, s->V64.p, s->F.p, P,
-                       perm.p, np, up2, best.p, face.p, cl.p);
+                       perm, np, up2, best, face, cl);
     check_launch("k_closest");
   }
   if (sign_type == GWN_SIGN_FAST_WINDING_NUMBER) {
     // signed_distance.cpp:125-130: fast_winding_number(fwn_bvh, 2, q.cast<float>())
-    W.alloc(np);
-    soup_query_device(s, Pf.p, np, 2.0f, GWN_PREC_FP32, W.p, st);
+    soup_query_device(s, Pf, np, 2.0f, GWN_PREC_FP32, W, st);
   } else if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT) {
     // signed_distance.cpp:121-124: the exact winding number
-    W.alloc(np);
-    DevBuf<double> scratch;
-    exact_device(s->V64.p, s->nv, s->F.p, s->nf, P, np, W.p, st, scratch);
-    GWN_HIP(hipStreamSynchronize(st));
+    exact_device(s->V64.p, s->nv, s->F.p, s->nf, P, np, W, st);
   }
-  hipLaunchKernelGGL(k_sd_combine, dim3(g), dim3(256), 0, st, np, best.p, face.p, cl.p, W.p, sign_type, low2, up2,
+  hipLaunchKernelGGL(k_sd_combine, dim3(g), dim3(256), 0, st, np, best, face, cl, W, sign_type, low2, up2,
                      s->nf, S, I, C);
   check_launch("k_sd_combine");
-  GWN_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace gwn

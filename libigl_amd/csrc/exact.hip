@@ -112,7 +112,10 @@ static int64_t exact_chunks(int64_t nf, int64_t qblocks) {
 // ---- 2D: winding number of a curve of edges (winding_number.cpp:62-89 with
 // F.cols() == 2): W(o) = sum over edges (a,b) of signed_angle(a, b, o)
 // (signed_angle.cpp:8-41): -atan2(cross, dot) of the unit vectors from a and b
-// to o, each term divided by 2 pi and summed in edge order, as the reference.
+// to o, each term divided by 2 pi.  Terms are summed in edge order within
+// each chunk of the edge list; when a small query set splits the edges into
+// chunks (exact_chunks), the chunk partials are then added in chunk order, so
+// the last bits can differ from the reference's single sum in edge order.
 __device__ __forceinline__ double signed_angle_2d(const double* e, double px, double py) {
   double ax = px - e[0], ay = py - e[1], bx = px - e[2], by = py - e[3];
   const double la = sqrt(ax * ax + ay * ay), lb = sqrt(bx * bx + by * by);
@@ -165,23 +168,25 @@ __global__ __launch_bounds__(EX_THREADS) void k_exact2d(const double* __restrict
 }
 
 void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double* O, int64_t no, double* W,
-                    hipStream_t st, DevBuf<double>& scratch) {
+                    hipStream_t st) {
   if (no <= 0) return;
   if (ne <= 0) {
-    GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * no, st));
+    fill_device<double>(W, no, 0.0, st);
     return;
   }
   const int64_t qblocks = div_up(no, EX_THREADS);
   const int64_t nch = exact_chunks(ne, qblocks);
   GWN_CHECK(nch <= 65535, GWN_ERR_INVALID, "exact 2D: too many edge chunks");
   const int64_t chunk = (ne + nch - 1) / nch;
-  // scratch: the edges (4 doubles each), then the partial sums
-  scratch.reserve(4 * (size_t)ne + (nch > 1 ? (size_t)nch * (size_t)no : 0));
-  double* part = nch > 1 ? scratch.p + 4 * (size_t)ne : nullptr;
-  hipLaunchKernelGGL(k_exact2d_gather, dim3(div_up(ne, EX_THREADS)), dim3(EX_THREADS), 0, st, V, E, ne, scratch.p);
+  // scratch of this call: the edges (4 doubles each), then the partial sums
+  const size_t npart = nch > 1 ? (size_t)nch * (size_t)no : 0;
+  Scratch ws(scratch_bytes({8 * 4 * (size_t)ne, 8 * npart}), st);
+  double* edges = ws.take_n<double>(4 * (size_t)ne);
+  double* part = nch > 1 ? ws.take_n<double>(npart) : nullptr;
+  hipLaunchKernelGGL(k_exact2d_gather, dim3(div_up(ne, EX_THREADS)), dim3(EX_THREADS), 0, st, V, E, ne, edges);
   check_launch("k_exact2d_gather");
   ProfileScope prof("exact2d", st);
-  hipLaunchKernelGGL(k_exact2d, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, scratch.p, ne, chunk,
+  hipLaunchKernelGGL(k_exact2d, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, edges, ne, chunk,
                      O, no, part, W);
   check_launch("k_exact2d");
   if (part) {
@@ -191,24 +196,26 @@ void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double*
 }
 
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
-                  hipStream_t st, DevBuf<double>& tri_scratch) {
+                  hipStream_t st) {
   (void)nv;
   if (no <= 0) return;
   if (nf <= 0) {
-    GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * no, st));
+    fill_device<double>(W, no, 0.0, st);
     return;
   }
   const int64_t qblocks = div_up(no, EX_THREADS * EX_QPT);
   const int64_t nch = exact_chunks(nf, qblocks);
   GWN_CHECK(nch <= 65535, GWN_ERR_INVALID, "exact: too many triangle chunks");
   const int64_t chunk = (nf + nch - 1) / nch;
-  // scratch: the triangles (9 doubles each), then the partial sums
-  tri_scratch.reserve(9 * (size_t)nf + (nch > 1 ? (size_t)nch * (size_t)no : 0));
-  double* part = nch > 1 ? tri_scratch.p + 9 * (size_t)nf : nullptr;
-  hipLaunchKernelGGL(k_exact_gather, dim3(div_up(nf, EX_THREADS)), dim3(EX_THREADS), 0, st, V, F, nf, tri_scratch.p);
+  // scratch of this call: the triangles (9 doubles each), then the partial sums
+  const size_t npart = nch > 1 ? (size_t)nch * (size_t)no : 0;
+  Scratch ws(scratch_bytes({8 * 9 * (size_t)nf, 8 * npart}), st);
+  double* tris = ws.take_n<double>(9 * (size_t)nf);
+  double* part = nch > 1 ? ws.take_n<double>(npart) : nullptr;
+  hipLaunchKernelGGL(k_exact_gather, dim3(div_up(nf, EX_THREADS)), dim3(EX_THREADS), 0, st, V, F, nf, tris);
   check_launch("k_exact_gather");
   ProfileScope prof("exact", st);
-  hipLaunchKernelGGL(k_exact, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, tri_scratch.p, nf, chunk,
+  hipLaunchKernelGGL(k_exact, dim3((unsigned)qblocks, (unsigned)nch), dim3(EX_THREADS), 0, st, tris, nf, chunk,
                      O, no, part, W);
   check_launch("k_exact");
   if (part) {

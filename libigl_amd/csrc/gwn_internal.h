@@ -2,8 +2,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <initializer_list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -81,6 +84,40 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// ---- per-call device scratch (scratch.cpp) ----
+// Borrows an idle pooled buffer of at least `bytes` for work enqueued on
+// `st` (the stream is made to wait for the buffer's previous user); the
+// destructor returns it, recording an event on `st` that orders the next
+// borrower after this call's work.  No host synchronisation; no allocation
+// once the pool holds a buffer of the size class.  take() carves 256-byte
+// aligned sub-buffers in order.
+struct ScratchEntry;
+class Scratch {
+ public:
+  Scratch(size_t bytes, hipStream_t st);
+  ~Scratch();
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  uint8_t* take(size_t bytes);
+  template <typename T>
+  T* take_n(size_t count) {
+    return reinterpret_cast<T*>(take(sizeof(T) * count));
+  }
+  uint8_t* p = nullptr;
+  size_t capacity = 0;
+
+ private:
+  ScratchEntry* e_ = nullptr;
+  hipStream_t st_ = nullptr;
+  size_t off_ = 0;
+};
+// bytes that take() calls of these sizes use together (with the alignment)
+inline size_t scratch_bytes(std::initializer_list<size_t> parts) {
+  size_t t = 0;
+  for (size_t b : parts) t = (t + 255) / 256 * 256 + b;
+  return t;
+}
+
 inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 // Checks that a kernel launch was accepted (async errors surface at the sync).
@@ -90,6 +127,20 @@ inline void check_launch(const char* what) {
 }
 
 void ensure_device();
+
+// p[0 .. n) = v.  Used instead of hipMemsetAsync so that a caller's captured
+// graph replays the fill.
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+template <typename T>
+inline void fill_device(T* p, int64_t n, T v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<T>, dim3(div_up(n, 256)), dim3(256), 0, st, p, n, v);
+  check_launch("k_fill");
+}
 
 // Brackets a kernel launch with HIP events when profiling is enabled
 // (gwn_profile_enable); the elapsed time is collected by gwn_profile_read.
@@ -162,7 +213,6 @@ void rope_first_slots(const Levels& L, const uint32_t* nodes, const uint32_t* ns
                       hipStream_t st);
 // soup queries (soup.hip)
 size_t soup_query_work_bytes(int64_t nq);
-void soup_ensure_fp64(gwn_soup* s, hipStream_t st);
 void soup_ensure_dist(gwn_soup* s, hipStream_t st);
 // .off / .obj file as a triangle mesh (polygons split into fans)
 void read_triangle_mesh(const char* path, std::vector<double>& V, std::vector<int32_t>& F);
@@ -183,9 +233,11 @@ void to_float_device(const double* x, int64_t n, float* y, hipStream_t st);
 void curve_build_device(gwn_curve* c, hipStream_t st);
 void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
                         hipStream_t st);
-// traversal kernel: WALK_WAVE (wave-coherent, batch queries) or WALK_LANE
-// (per-lane, used by the serving engine); bitwise the same results
-constexpr int WALK_WAVE = 0, WALK_LANE = 1;
+// traversal kernel: WALK_WAVE (wave-coherent, sorted batch queries),
+// WALK_LANE (per-lane over the sorted queries, used by the serving engine for
+// small chunks) or WALK_LANE_UNSORTED (per-lane in input order: no sort, a
+// workspace of 4 nq + 4 bytes); bitwise the same results
+constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2;
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk);
 
@@ -204,12 +256,14 @@ struct gwn_soup {
   gwn::DevBuf<float> boxdata;   // nnodes x 92, reference BoxData order
   gwn::DevBuf<float4> slots;    // nslots x 6 (rope layout, FP32 moments)
   int64_t nslots = 0;
-  gwn::DevBuf<double> slots64;  // nslots x 24 (FP64 moments), built on first FP64 use
+  gwn::DevBuf<double> slots64;  // nslots x 24 (FP64 moments), built with the tree
   gwn::DevBuf<double> V64;      // nv x 3, the vertices as given (closest-point queries)
   gwn::DevBuf<float4> dbox;     // nslots x 2 slot boxes for closest-point queries, built on first use
   gwn_mixed_params mixed{1e-3f, 0.05f};
-  // per-handle scratch for queries (grow-only)
-  gwn::DevBuf<uint8_t> work;
+  // After the build a handle is read-only to queries, except for dbox, which
+  // the first closest-point query builds under dist_mu (dist_ready set after).
+  std::mutex dist_mu;
+  std::atomic<bool> dist_ready{false};
 };
 
 // 2D curve (UT_SubtendedAngle<float,float>): segments, the 4-wide tree over
@@ -227,7 +281,6 @@ struct gwn_curve {
   gwn::DevBuf<float> boxdata;    // nnodes x 48, reference BoxData order
   gwn::DevBuf<float4> slots;     // nslots x 3 (rope layout)
   int64_t nslots = 0;
-  gwn::DevBuf<uint8_t> work;     // per-handle query scratch (grow-only)
 };
 
 // Antipodal-method scene: the soup's triangles, tree and slot boxes for the
@@ -256,5 +309,4 @@ struct gwn_cloud {
   gwn::DevBuf<float> CM, R, EC;        // ncells x 3, ncells, ncells x nterms
   gwn::DevBuf<float4> head;            // nrec: CM.xyz, R of each non-empty non-root cell (rope order)
   gwn::DevBuf<uint2> meta;             // nrec: (skip, cell) or (LEAF|first point, count)
-  gwn::DevBuf<uint8_t> work;
 };

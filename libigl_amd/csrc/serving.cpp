@@ -62,6 +62,9 @@ size_t class_bytes(int64_t nq) {
 struct Request {
   int64_t id = 0;
   int32_t mesh = 0;
+  // resolved under the server lock at submission: the worker never reads the
+  // mesh list, which gwn_server_add_mesh may grow concurrently
+  const gwn_soup* soup = nullptr;
   const double* Q = nullptr;
   int64_t nq = 0;
   float beta = 2.0f;
@@ -178,7 +181,7 @@ struct gwn_server {
 // need_bytes(r->nq); called without the server lock.
 void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
   using namespace gwn;
-  gwn_soup* m = meshes[r->mesh];
+  const gwn_soup* m = r->soup;
   const int64_t nchunk = (r->nq + kChunk - 1) / kChunk;
   const int64_t cn = std::min<int64_t>(r->nq, kChunk);
   const size_t reg = region_bytes(cn);
@@ -366,15 +369,8 @@ int gwn_server_add_mesh(gwn_server* s, const double* V, int64_t nv, const int32_
     return GWN_ERR_INVALID;
   }
   gwn_soup* m = nullptr;
-  int rc = gwn_soup_build(V, nv, F, nf, order, tree_kind, &m);
+  int rc = gwn_soup_build(V, nv, F, nf, order, tree_kind, &m);  // FP64 moments included
   if (rc != GWN_OK) return rc;
-  try {
-    soup_ensure_fp64(m, nullptr);
-  } catch (const Error& e) {
-    gwn_soup_free(m);
-    set_error(e.msg);
-    return e.code;
-  }
   gwn_soup_info info;
   gwn_soup_get_info(m, &info);
   std::lock_guard<std::mutex> l(s->mu);
@@ -417,6 +413,7 @@ int gwn_server_submit(gwn_server* s, int32_t mesh_id, const double* Q, int64_t n
   auto r = std::make_unique<Request>();
   r->id = (int64_t)s->reqs.size();
   r->mesh = mesh_id;
+  r->soup = s->meshes[mesh_id];
   r->Q = Q;
   r->nq = nq;
   r->beta = accuracy_scale;

@@ -494,7 +494,8 @@ struct D4 {
 };
 
 // ---------------------------------------------------------------------------
-// traversal: FP32 (reference arithmetic) and mixed (FP64 near triangles + FP64 sum)
+// traversal: FP32 (reference arithmetic) and mixed (FP32 far field and
+// running sum; FP64 solid angles and sum of the triangles near the query)
 //
 // Wave-coherent rope walk.  The slot records are in depth-first ("rope")
 // order: a far child adds its Taylor value and skips to `link` (the record
@@ -511,6 +512,28 @@ struct D4 {
 // through the scalar cache instead of 64 times through the vector memory path.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Mixed form: a triangle slot that may lie within near_d of the query
+// (|q - P| < sqrt(maxP2) + near_d, P the slot's centre and maxP2 its squared
+// radius) has its solid angle computed in FP64 (UTsignedSolidAngleTri,
+// 6070-6110, in double) and summed in an FP64 accumulator of its own; the far
+// field and every other exact term go to the FP32 running sum.
+__device__ __forceinline__ bool near_triangle(float maxp2, float l2, float near_d) {
+  const float rr = sqrtf(maxp2) + near_d;
+  return l2 < rr * rr;
+}
+
+// Exact term of a near triangle slot: sum (FP32) or, in the mixed form for a
+// triangle near the query, sum_near (FP64).
+template <bool MIXED>
+__device__ __forceinline__ void exact_term(const float* __restrict__ p, float maxp2, float l2, float qx, float qy,
+                                           float qz, float near_d, float& sum, double& sum_near) {
+  if (MIXED && near_triangle(maxp2, l2, near_d))
+    sum_near = __dadd_rn(sum_near,
+                         tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+  else
+    sum = __fadd_rn(sum, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+}
+
 template <int ORDER, bool MIXED>
 __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slots, uint32_t nslots,
                                                   const float* __restrict__ tri, const float* __restrict__ Q,
@@ -520,8 +543,8 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
-  float sum32 = 0.0f;
-  double sum64 = 0.0;
+  float sum = 0.0f;
+  double sum_near = 0.0;  // mixed form: FP64 terms of near triangles
   uint32_t skip = 0;
   uint32_t i = 0;
   while (i < nslots) {
@@ -553,43 +576,26 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
       if (!(l2 <= __fmul_rn(a.w, acc2))) {
         const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
         if (isfinite(o)) {  // otherwise the reference descends / evaluates the item
-          if (MIXED)
-            sum64 = __dadd_rn(sum64, (double)o);
-          else
-            sum32 = __fadd_rn(sum32, o);
+          sum = __fadd_rn(sum, o);
           exact = false;
           if (!leaf) skip = link;
         }
       }
-      if (exact) {
-        const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-        if (MIXED) {
-          const float rr = sqrtf(a.w) + near_d;
-          if (l2 < rr * rr) {
-            sum64 = __dadd_rn(sum64, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-          } else {
-            sum64 = __dadd_rn(sum64, (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
-                                                  qz));
-          }
-        } else {
-          sum32 = __fadd_rn(sum32, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-        }
-      }
+      if (exact) exact_term<MIXED>(tri + 9 * (size_t)(link & ~LINK_LEAF), a.w, l2, qx, qy, qz, near_d, sum, sum_near);
     }
     const bool step = leaf || __ballot(skip == i + 1) != 0;
     i = wave_uniform(step ? i + 1 : link);
   }
-  W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
+  W[qi] = (MIXED ? (double)sum + sum_near : (double)sum) / (4.0 * PI_D);
 }
 
 // Per-lane rope walk: each lane follows its own skip-pointer path and gathers
 // its own records.  It visits the same records in the same order and sums the
 // same terms as k_traverse; the running sums use non-contracting adds in both
 // kernels, so no multiply is fused into them differently and the results are
-// bitwise the same.  Used by the
-// serving engine, whose many small, concurrent batches over different resident
-// trees run faster this way (fewer iterations per wave; no scalar-cache
-// round trip on each step's critical path).
+// bitwise the same.  Used for small batches (whose queries are too sparse
+// for wavefront coherence), unsorted below a few thousand queries, and by
+// the serving engine for chunks under 64k queries.
 template <int ORDER, bool MIXED>
 __global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict__ slots, uint32_t nslots,
                                                        const float* __restrict__ tri, const float* __restrict__ Q,
@@ -599,8 +605,8 @@ __global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict_
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
-  float sum32 = 0.0f;
-  double sum64 = 0.0;
+  float sum = 0.0f;
+  double sum_near = 0.0;  // mixed form: FP64 terms of near triangles
   uint32_t i = 0;
   while (i < nslots) {
     const float4* r = slots + 6 * (size_t)i;
@@ -621,30 +627,15 @@ __global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict_
       }
       const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
       if (isfinite(o)) {
-        if (MIXED)
-          sum64 = __dadd_rn(sum64, (double)o);
-        else
-          sum32 = __fadd_rn(sum32, o);
+        sum = __fadd_rn(sum, o);
         i = leaf ? i + 1 : link;
         continue;
       }
     }
-    if (leaf) {
-      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-      if (MIXED) {
-        const float rr = sqrtf(a.w) + near_d;
-        if (l2 < rr * rr) {
-          sum64 = __dadd_rn(sum64, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-        } else {
-          sum64 = __dadd_rn(sum64, (double)tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-        }
-      } else {
-        sum32 = __fadd_rn(sum32, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-      }
-    }
+    if (leaf) exact_term<MIXED>(tri + 9 * (size_t)(link & ~LINK_LEAF), a.w, l2, qx, qy, qz, near_d, sum, sum_near);
     ++i;
   }
-  W[qi] = (MIXED ? sum64 : (double)sum32) / (4.0 * PI_D);
+  W[qi] = (MIXED ? (double)sum + sum_near : (double)sum) / (4.0 * PI_D);
 }
 
 // all-FP64 form: FP64 moments, far field, atan2 and sum
@@ -830,6 +821,24 @@ static void upward(gwn_soup* s, const Levels& L, T* boxdata, DevBuf<uint32_t>& n
   GWN_HIP(hipStreamSynchronize(st));
 }
 
+// FP64 moments of every slot (the reference's item()/post() in double), in
+// the field order of k_write_slots64: the all-FP64 form and the FP64
+// re-evaluation of the mixed form read them.
+static void build_fp64_moments(gwn_soup* s, const Levels& L, hipStream_t st) {
+  const int64_t nn = s->nnodes;
+  DevBuf<double> bd64(92 * nn);
+  DevBuf<uint32_t> nsub(nn), first(nn);
+  upward<double>(s, L, bd64.p, nsub, st);
+  rope_first_slots(L, s->nodes.p, nsub.p, first.p, st);
+  s->slots64.alloc(24 * (size_t)s->nslots);
+  hipLaunchKernelGGL(k_write_slots64, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, bd64.p, first.p,
+                     nsub.p, s->slots64.p);
+  check_launch("k_write_slots64");
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+// Tree levels, FP32 BoxData (the reference's), the rope-ordered slot records
+// and their FP64 moments.  After this the handle is read-only to queries.
 void soup_finish_build(gwn_soup* s, hipStream_t st) {
   const int64_t nn = s->nnodes;
   if (nn == 0) {
@@ -851,28 +860,18 @@ void soup_finish_build(gwn_soup* s, hipStream_t st) {
   hipLaunchKernelGGL(k_write_slots, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, s->boxdata.p, nsub.p,
                      first.p, s->slots.p);
   check_launch("k_write_slots");
-  GWN_HIP(hipStreamSynchronize(st));
+  build_fp64_moments(s, L, st);
 }
 
-void soup_ensure_fp64(gwn_soup* s, hipStream_t st) {
-  if (s->slots64.p || s->nnodes == 0) return;
-  const int64_t nn = s->nnodes;
-  Levels L;
-  compute_levels(s->nodes.p, nn, L, st);
-  DevBuf<double> bd64(92 * nn);
-  DevBuf<uint32_t> nsub(nn), first(nn);
-  upward<double>(s, L, bd64.p, nsub, st);
-  rope_first_slots(L, s->nodes.p, nsub.p, first.p, st);
-  DevBuf<double> s64(24 * (size_t)s->nslots);
-  hipLaunchKernelGGL(k_write_slots64, dim3(div_up(nn, 256)), dim3(256), 0, st, nn, s->nodes.p, bd64.p, first.p,
-                     nsub.p, s64.p);
-  check_launch("k_write_slots64");
-  GWN_HIP(hipStreamSynchronize(st));
-  s->slots64 = std::move(s64);
-}
-
+// Slot boxes for closest-point queries (and the antipodal rays), built by the
+// first call that needs them.  Concurrent first calls on one handle build
+// them once: the others wait on dist_mu, and the boxes are complete on the
+// device (the build ends with a stream synchronisation) before dist_ready is
+// set.
 void soup_ensure_dist(gwn_soup* s, hipStream_t st) {
-  if (s->dbox.p || s->nnodes == 0) return;
+  if (s->dist_ready.load(std::memory_order_acquire) || s->nnodes == 0) return;
+  std::lock_guard<std::mutex> lock(s->dist_mu);
+  if (s->dist_ready.load(std::memory_order_relaxed)) return;
   const int64_t nn = s->nnodes;
   Levels L;
   compute_levels(s->nodes.p, nn, L, st);
@@ -900,6 +899,7 @@ void soup_ensure_dist(gwn_soup* s, hipStream_t st) {
   check_launch("k_write_dbox");
   GWN_HIP(hipStreamSynchronize(st));
   s->dbox = std::move(db);
+  s->dist_ready.store(true, std::memory_order_release);
 }
 
 void soup_setup(gwn_soup* s, hipStream_t st) {
@@ -913,15 +913,15 @@ void soup_setup(gwn_soup* s, hipStream_t st) {
   }
 }
 
-// Workspace for one query call: Morton keys/permutation + sort temp + ambiguity list.
+// Workspace of one sorted query call: Hilbert keys, the query permutation,
+// sort temp, and the list and count of ambiguous queries (mixed form).
 size_t soup_query_work_bytes(int64_t nq) {
   const size_t a = 256;
   auto up = [&](size_t b) { return (b + a - 1) / a * a; };
   return up(4 * (size_t)nq) * 2 + up(radix_sort_temp_bytes(nq, false)) + up(4 * (size_t)nq) + up(4);
 }
-
-void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk);
+// Workspace of an unsorted (per-lane) call: the ambiguous list and count only.
+static size_t soup_small_work_bytes(int64_t nq) { return (4 * (size_t)nq + 255) / 256 * 256 + 256; }
 
 // 30-bit Morton keys of the queries over a box (queries outside are clamped
 // to it), and the identity permutation to be sorted along with them
@@ -943,51 +943,56 @@ void hilbert_keys(const float* Q, int64_t nq, const float* bb, uint32_t* keys, u
   check_launch("k_hilbert_q");
 }
 
-void soup_query_device(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
+// Batches below this size are walked per lane in input order: the Hilbert
+// sort (a key kernel and four radix passes of three kernels) would cost more
+// than the coherence it buys.
+constexpr int64_t kSortMin = 4096;
+
+void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                        double* W, hipStream_t st) {
-  gwn_soup* s = const_cast<gwn_soup*>(sc);
   if (nq <= 0) return;
-  const size_t need = soup_query_work_bytes(nq);
-  s->work.reserve(need);
-  soup_query_device_ws(sc, Q, nq, accuracy_scale, precision, W, st, s->work.p, s->work.bytes(), WALK_WAVE);
+  const bool sorted = nq >= kSortMin;
+  Scratch ws(sorted ? soup_query_work_bytes(nq) : soup_small_work_bytes(nq), st);
+  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws.p, ws.capacity,
+                       sorted ? WALK_WAVE : WALK_LANE_UNSORTED);
 }
 
 // Query with caller-provided device workspace (soup_query_work_bytes(nq)
 // bytes); calls with distinct workspaces may run concurrently on distinct
 // streams against the same tree once its FP64 moments exist.
-void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float accuracy_scale, int precision,
+void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk) {
-  gwn_soup* s = const_cast<gwn_soup*>(sc);
   if (nq <= 0) return;
   if (s->nslots == 0) {
-    GWN_HIP(hipMemsetAsync(W, 0, sizeof(double) * nq, st));
+    fill_device<double>(W, nq, 0.0, st);
     return;
   }
-  GWN_CHECK(ws_bytes >= soup_query_work_bytes(nq), GWN_ERR_INVALID, "query workspace too small");
+  const bool sorted = walk != WALK_LANE_UNSORTED;
+  GWN_CHECK(ws_bytes >= (sorted ? soup_query_work_bytes(nq) : soup_small_work_bytes(nq)), GWN_ERR_INVALID,
+            "query workspace too small");
   uint8_t* w = workspace;
   auto take = [&](size_t b) {
     uint8_t* p = w;
     w += (b + 255) / 256 * 256;
     return p;
   };
-  uint32_t* keys = (uint32_t*)take(4 * (size_t)nq);
-  uint32_t* perm = (uint32_t*)take(4 * (size_t)nq);
-  const size_t tmpb = radix_sort_temp_bytes(nq, false);
-  void* tmp = take(tmpb);
-  uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
-  uint32_t* ambn = (uint32_t*)take(4);
-
-  hilbert_keys(Q, nq, s->bbox, keys, perm, st);
-  {
+  uint32_t* perm = nullptr;
+  if (sorted) {
+    uint32_t* keys = (uint32_t*)take(4 * (size_t)nq);
+    perm = (uint32_t*)take(4 * (size_t)nq);
+    const size_t tmpb = radix_sort_temp_bytes(nq, false);
+    void* tmp = take(tmpb);
+    hilbert_keys(Q, nq, s->bbox, keys, perm, st);
     ProfileScope prof("radix_sort", st);
     radix_sort_u32(keys, perm, nq, 30, tmp, tmpb, st);
   }
+  uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
+  uint32_t* ambn = (uint32_t*)take(4);
 
   const float acc2 = accuracy_scale * accuracy_scale;
   const unsigned g = div_up(nq, 256);
   const uint32_t ns = (uint32_t)s->nslots;
   if (precision == GWN_PREC_FP64) {
-    soup_ensure_fp64(s, st);
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st);
     switch (s->order) {
@@ -1002,34 +1007,33 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
   const float nd = s->mixed.near_dist;
 #define GWN_LAUNCH_T(O, M)                                                                                      \
   do {                                                                                                          \
-    if (walk == WALK_LANE)                                                                                      \
-      hipLaunchKernelGGL((k_traverse_lane<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, \
+    if (walk == WALK_WAVE)                                                                                      \
+      hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq,  \
                          acc2, nd, W);                                                                          \
     else                                                                                                        \
-      hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq,  \
+      hipLaunchKernelGGL((k_traverse_lane<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, \
                          acc2, nd, W);                                                                          \
   } while (0)
   {
-  ProfileScope prof("soup_traverse", st);
-  if (mixed) {
-    switch (s->order) {
-      case 0: GWN_LAUNCH_T(0, true); break;
-      case 1: GWN_LAUNCH_T(1, true); break;
-      default: GWN_LAUNCH_T(2, true); break;
+    ProfileScope prof("soup_traverse", st);
+    if (mixed) {
+      switch (s->order) {
+        case 0: GWN_LAUNCH_T(0, true); break;
+        case 1: GWN_LAUNCH_T(1, true); break;
+        default: GWN_LAUNCH_T(2, true); break;
+      }
+    } else {
+      switch (s->order) {
+        case 0: GWN_LAUNCH_T(0, false); break;
+        case 1: GWN_LAUNCH_T(1, false); break;
+        default: GWN_LAUNCH_T(2, false); break;
+      }
     }
-  } else {
-    switch (s->order) {
-      case 0: GWN_LAUNCH_T(0, false); break;
-      case 1: GWN_LAUNCH_T(1, false); break;
-      default: GWN_LAUNCH_T(2, false); break;
-    }
-  }
   }
 #undef GWN_LAUNCH_T
   check_launch("k_traverse");
   if (mixed && s->mixed.ambiguous_band > 0.0f) {
-    soup_ensure_fp64(s, st);
-    GWN_HIP(hipMemsetAsync(ambn, 0, 4, st));
+    fill_device<uint32_t>(ambn, 1, 0u, st);
     hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
                        ambn);
     check_launch("k_collect_ambiguous");
@@ -1042,6 +1046,7 @@ void soup_query_device_ws(const gwn_soup* sc, const float* Q, int64_t nq, float 
     if (na > 0) {
       const double a2 = (double)accuracy_scale * (double)accuracy_scale;
       const unsigned ga = div_up(na, 256);
+      ProfileScope prof("soup_traverse64", st);
       switch (s->order) {
         case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
         case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;

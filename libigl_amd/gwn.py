@@ -133,7 +133,7 @@ class FastWindingNumberBVH:
         return W
 
     def query_device(self, Q_ptr, nq, W_ptr, accuracy_scale=2.0, precision=PREC_FP32, stream=None):
-        """Device pointers (e.g. torch tensor data_ptr()): Q float nq x 3, W double nq."""
+        """Device pointers (e.g. libigl_amd.devmem.DeviceArray.data_ptr): Q float nq x 3, W double nq."""
         check(lib().gwn_soup_query_device(self.handle, Q_ptr, nq, float(accuracy_scale), precision, W_ptr, stream))
 
     def signed_distance(self, P, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX):
@@ -391,7 +391,7 @@ class SubtendedAngle:
         return out
 
     def query_device(self, Q_ptr, nq, angle_ptr, accuracy_scale=2.0, stream=None):
-        """Device pointers (e.g. torch tensor data_ptr()): Q float nq x 2, angle float nq."""
+        """Device pointers (e.g. libigl_amd.devmem.DeviceArray.data_ptr): Q float nq x 2, angle float nq."""
         check(lib().gwn_curve_query_device(self._h, Q_ptr, nq, float(accuracy_scale), angle_ptr, stream))
 
 

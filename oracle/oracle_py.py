@@ -15,6 +15,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref_soup.so")
+# the same reference sources built with its Release flags (-O3 -DNDEBUG): the
+# CPU baseline that bench.py times
+REF_RELEASE_SO = os.path.join(HERE, "_ref", "libref_soup_release.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -23,7 +26,7 @@ _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 class OctreeT(C.Structure):
@@ -91,16 +94,18 @@ def lib():
     return _lib
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_SO)
+def ref_available(release=False) -> bool:
+    return os.path.exists(REF_RELEASE_SO if release else REF_SO)
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        if not os.path.exists(REF_SO):
-            raise RuntimeError(f"reference build missing: {REF_SO}")
-        R = C.CDLL(REF_SO)
+def ref(release=False):
+    """The compiled reference: the parity build (-O2 -ffp-contract=off) or,
+    with release=True, the Release-flag build."""
+    if release not in _ref:
+        path = REF_RELEASE_SO if release else REF_SO
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference build missing: {path}")
+        R = C.CDLL(path)
         R.ref_soup_build.restype = C.c_void_p
         R.ref_soup_build.argtypes = [_f32p, C.c_int, _i32p, C.c_int, C.c_int]
         R.ref_soup_free.argtypes = [C.c_void_p]
@@ -121,8 +126,9 @@ def ref():
         R.ref_curve_nodes.argtypes = [C.c_void_p, _u32p]
         R.ref_curve_boxdata.argtypes = [C.c_void_p, _f32p]
         R.ref_curve_angle.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_float, _f32p]
-        _ref = R
-    return _ref
+        R.ref_curve_angle_par.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_float, _f32p]
+        _ref[release] = R
+    return _ref[release]
 
 
 def _c(a, dt):
@@ -223,40 +229,42 @@ class Soup:
 
 
 class RefSoup:
-    """The reference's own UT_SolidAngle (compiled from the reference header)."""
+    """The reference's own UT_SolidAngle (compiled from the reference header);
+    release=True uses the Release-flag build (the timed CPU baseline)."""
 
-    def __init__(self, V, F, order=2):
+    def __init__(self, V, F, order=2, release=False):
+        self.R = ref(release)
         self.U = _c(V, np.float32)
         self.F = _c(F, np.int32)
-        self.h = ref().ref_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
+        self.h = self.R.ref_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
 
     def __del__(self):
         if getattr(self, "h", None):
-            ref().ref_soup_free(self.h)
+            self.R.ref_soup_free(self.h)
             self.h = None
 
     def nodes(self):
-        n = ref().ref_soup_num_nodes(self.h)
+        n = self.R.ref_soup_num_nodes(self.h)
         out = np.empty((n, 4), np.uint32)
-        ref().ref_soup_nodes(self.h, out)
+        self.R.ref_soup_nodes(self.h, out)
         return out
 
     def boxdata(self):
-        n = ref().ref_soup_num_nodes(self.h)
+        n = self.R.ref_soup_num_nodes(self.h)
         out = np.empty((n, 92), np.float32)
-        ref().ref_soup_boxdata(self.h, out)
+        self.R.ref_soup_boxdata(self.h, out)
         return out
 
     def omega(self, Q, accuracy_scale=2.0):
         Q = _c(Q, np.float32).reshape(-1, 3)
         out = np.empty(Q.shape[0], np.float32)
-        ref().ref_soup_solid_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
+        self.R.ref_soup_solid_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
         return out
 
     def query(self, Q, accuracy_scale=2.0):
         Q = _c(Q, np.float64).reshape(-1, 3)
         W = np.empty(Q.shape[0], np.float64)
-        ref().ref_soup_query(self.h, Q, Q.shape[0], accuracy_scale, W)
+        self.R.ref_soup_query(self.h, Q, Q.shape[0], accuracy_scale, W)
         return W
 
 
@@ -312,15 +320,28 @@ class Curve(_CurveBase):
 class RefCurve(_CurveBase):
     """The reference's own UT_SubtendedAngle (compiled from the reference header)."""
     _prefix = "ref_curve_"
+    _release = False
 
     def _api(self):
-        return ref()
+        return ref(self._release)
 
     def angle(self, Q, accuracy_scale=2.0):
         Q = _c(Q, np.float32).reshape(-1, 2)
         out = np.empty(Q.shape[0], np.float32)
-        ref().ref_curve_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
+        self._api().ref_curve_angle(self.h, Q, Q.shape[0], accuracy_scale, out)
         return out
+
+    def angle_parallel(self, Q, accuracy_scale=2.0):
+        """computeAngle over the host cores with the reference's igl::parallel_for."""
+        Q = _c(Q, np.float32).reshape(-1, 2)
+        out = np.empty(Q.shape[0], np.float32)
+        self._api().ref_curve_angle_par(self.h, Q, Q.shape[0], accuracy_scale, out)
+        return out
+
+
+class RefCurveRelease(RefCurve):
+    """RefCurve from the Release-flag build (the timed CPU baseline)."""
+    _release = True
 
 
 # ---------------- cloud ----------------

@@ -151,4 +151,19 @@ void ref_curve_angle(void* h, const float* Q, int64_t nq, float accuracy_scale, 
     angle[p] = r->sa.computeAngle(q, accuracy_scale);
   }
 }
+
+// computeAngle for each query over the host cores with the reference's own
+// igl::parallel_for (min_parallel 1000, as its batch soup query uses)
+void ref_curve_angle_par(void* h, const float* Q, int64_t nq, float accuracy_scale, float* angle) {
+  RefCurve* r = static_cast<RefCurve*>(h);
+  igl::parallel_for(
+      (int)nq,
+      [&](int p) {
+        UT_Vector2T<float> q;
+        q[0] = Q[2 * (int64_t)p];
+        q[1] = Q[2 * (int64_t)p + 1];
+        angle[p] = r->sa.computeAngle(q, accuracy_scale);
+      },
+      1000);
+}
 }

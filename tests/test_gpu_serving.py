@@ -137,3 +137,35 @@ def test_add_mesh_file(tmp_path):
     srv.drain()
     assert np.array_equal(srv.wait(ra), srv.wait(rb))
     srv.close()
+
+
+def test_add_mesh_while_requests_in_flight():
+    """gwn_server_add_mesh may grow the mesh list while slots serve requests
+    on earlier meshes (each request holds its own mesh, resolved at submit)."""
+    import threading
+    meshes = _meshes()
+    srv = Server(slots=4, device_budget_bytes=8 << 30)
+    first = srv.add_mesh(*meshes[0])
+    alone = gwn.FastWindingNumberBVH().build(*meshes[0])
+    rng = np.random.default_rng(12)
+    Qs = [D.uniform_queries(meshes[0][0].min(0), meshes[0][0].max(0), 150_000, rng) for _ in range(24)]
+    added = []
+
+    def add_more():
+        for _ in range(6):
+            for V, F in meshes[1:]:
+                added.append(srv.add_mesh(V, F))
+
+    th = threading.Thread(target=add_more)
+    th.start()
+    rids = [srv.submit(first, Q, 2.0) for Q in Qs]
+    th.join()
+    srv.drain()
+    for rid, Q in zip(rids, Qs):
+        assert np.array_equal(srv.wait(rid), alone.query(Q, 2.0))
+    assert len(added) == 18 and len(set(added)) == 18
+    Q = D.uniform_queries(meshes[3][0].min(0), meshes[3][0].max(0), 10_000, rng)
+    r = srv.submit(added[-1], Q)
+    assert np.array_equal(srv.wait(r), gwn.FastWindingNumberBVH().build(*meshes[3]).query(Q, 2.0))
+    srv.close()
+
