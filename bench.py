@@ -80,12 +80,13 @@ def main():
     if args.copy_calibration > 0:
         import numpy as np
 
+        from libigl_amd import _native as nat
         from libigl_amd import devmem as dm
         n = args.copy_calibration // 4
         src = dm.DeviceArray.from_numpy(np.ones(n, np.float32))
         dst = dm.DeviceArray((n,), np.float32)
-        for _ in range(3):
-            dst.copy_from(src)  # hipMemcpy device to device: the runtime's copy kernel
+        for _ in range(3):  # the library's plain copy kernel: a known number of bytes each way
+            nat.check(nat.lib().gwn_profile_copy(dst.data_ptr, src.data_ptr, 4 * n, None))
         dm.synchronize()
         print(json.dumps({"mode": "copy", "bytes": 4 * n, "copies": 3}), flush=True)
         return
@@ -148,8 +149,8 @@ def main():
             n = args.calibrate_copy // 4
             src = dm.DeviceArray.from_numpy(np.ones(n, np.float32))
             dst = dm.DeviceArray((n,), np.float32)
-            for _ in range(3):
-                dst.copy_from(src)
+            for _ in range(3):  # the library's plain copy kernel: a known number of bytes each way
+                nat.check(nat.lib().gwn_profile_copy(dst.data_ptr, src.data_ptr, 4 * n, stream.ptr))
             dm.synchronize()
         print(json.dumps({"mode": "profile", "beta": args.beta, "tree": args.tree, "ms_per_step": 1e3 * dt,
                           "queries_per_s": nq / dt}), flush=True)
@@ -381,15 +382,20 @@ def run_boundary(bvh, Q, beta):
     import threading
 
     import numpy as np
+    from libigl_amd import _native as nat
     out = {}
     for n, reps in ((1, 400), (1000, 100), (1_000_000, 5)):
         Qn = np.ascontiguousarray(Q[:n])
         bvh.query(Qn, beta)
+        nat.lib().gwn_profile_reset()
+        nat.lib().gwn_profile_enable(1)
         t = time.perf_counter()
         for _ in range(reps):
             bvh.query(Qn, beta)
         dt = (time.perf_counter() - t) / reps
-        out[f"nq_{n}"] = {"call_ms": round(1e3 * dt, 4), "queries_per_s": round(n / dt, 1)}
+        nat.lib().gwn_profile_enable(0)
+        out[f"nq_{n}"] = {"call_ms": round(1e3 * dt, 4), "queries_per_s": round(n / dt, 1),
+                          "traverse_kernel_ms": round(_kernel_ms("soup_traverse"), 4)}
     per, nth = 250, 8
     Wt = np.zeros(per * nth)
     errs = []
@@ -419,9 +425,9 @@ def measure_traffic(args, beta):
     """HBM bytes per k_traverse<2,false> launch of the headline run, measured
     now: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch) over a child run of this script at the same accuracy_scale and
-    tree, which also makes three 256 MiB device-to-device copies; each counter
-    is scaled by what it reads for the copy (256 MiB read and 256 MiB
-    written).  Returns (bytes, note) or (None, reason)."""
+    tree, which also makes three 256 MiB copies with the library's plain copy
+    kernel; each counter is scaled by what it reads for a copy (256 MiB read
+    and 256 MiB written).  Returns (bytes, note) or (None, reason)."""
     import csv
     import glob
     import re
@@ -456,7 +462,7 @@ def measure_traffic(args, beta):
                     continue
                 if kre.search(name):
                     kern.append(float(v))
-                elif "copyBuffer" in name:
+                elif "k_copy_calibration" in name:
                     copy.append(float(v))
         shutil.rmtree(d, ignore_errors=True)
         if not kern or not copy:

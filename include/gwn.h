@@ -96,6 +96,12 @@ const char* gwn_version(void);
 int gwn_profile_enable(int on);
 int gwn_profile_read(const char* kernel, double* total_ms, int64_t* launches);
 void gwn_profile_reset(void);
+/* One device-to-device copy of `bytes` (a positive multiple of 16) from src
+ * to dst by a plain copy kernel (k_copy_calibration), asynchronous on
+ * `stream`: a kernel of known traffic (bytes read, bytes written) against
+ * which hardware memory counters such as FETCH_SIZE and WRITE_SIZE are
+ * scaled. */
+int gwn_profile_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Exact FP64 generalized winding number.

@@ -53,6 +53,7 @@ SIGNATURES = {
     "gwn_profile_enable": (C.c_int, [C.c_int]),
     "gwn_profile_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "gwn_profile_reset": (None, []),
+    "gwn_profile_copy": (C.c_int, [vp, vp, C.c_int64, vp]),
     "gwn_exact_winding_number": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
     "gwn_exact_winding_number_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, vp, C.c_int64, vp, vp]),
     "gwn_soup_build": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),

@@ -146,7 +146,7 @@ inline void fill_device(T* p, int64_t n, T v, hipStream_t st) {
 // (gwn_profile_enable); the elapsed time is collected by gwn_profile_read.
 class ProfileScope {
  public:
-  ProfileScope(const char* name, hipStream_t st);
+  ProfileScope(const char* name, hipStream_t st, bool enabled = true);
   ~ProfileScope();
   ProfileScope(const ProfileScope&) = delete;
   ProfileScope& operator=(const ProfileScope&) = delete;

@@ -1,6 +1,7 @@
 // profile.cpp -- per-kernel timing with HIP events on the launching stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -43,8 +44,8 @@ bool profiling_on() {
   return g_on;
 }
 
-ProfileScope::ProfileScope(const char* name, hipStream_t st) : name_(name), st_(st) {
-  if (!profiling_on()) return;
+ProfileScope::ProfileScope(const char* name, hipStream_t st, bool enabled) : name_(name), st_(st) {
+  if (!enabled || !profiling_on()) return;
   if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
     (void)hipGetLastError();
     return;
@@ -66,9 +67,25 @@ ProfileScope::~ProfileScope() {
   g_pending.push_back({name_, a_, b_});
 }
 
+// A plain device-to-device copy of n 16-byte words: the known-traffic kernel
+// that hardware memory counters are calibrated against.
+__global__ void k_copy_calibration(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace gwn
 
 extern "C" {
+int gwn_profile_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  if (!dst || !src || bytes <= 0 || bytes % 16 != 0) return GWN_ERR_INVALID;
+  const int64_t n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(gwn::k_copy_calibration, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  return hipGetLastError() == hipSuccess ? GWN_OK : GWN_ERR_HIP;
+}
+
 int gwn_profile_enable(int on) {
   std::lock_guard<std::mutex> l(gwn::g_mu);
   gwn::g_on = on != 0;
