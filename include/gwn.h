@@ -140,6 +140,23 @@ int gwn_soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float a
                           double* W, void* stream);
 int gwn_soup_set_mixed_params(gwn_soup* s, const gwn_mixed_params* p);
 
+/* Scratch-free form of gwn_soup_query_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and bitwise the same results as gwn_soup_query_device; the
+ * caller passes device inputs and outputs, and the call's temporary memory is
+ * a static array of 22 MiB that the library holds on each device it runs on
+ * until the process ends.  It calls no hipMalloc, hipFree or allocating
+ * library routine, creates no timing events, and never waits on the host
+ * (the mixed form's FP64 re-evaluation is sized on the device).  At most
+ * gwn_soup_noalloc_max_queries() = 1,048,576 queries per call
+ * (GWN_ERR_INVALID beyond).  Every call on one device shares that device's
+ * array: calls that overlap on one device race, so serialise them (one
+ * stream, or events between streams). */
+int gwn_soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale,
+                                  int precision, double* W, void* stream);
+int64_t gwn_soup_noalloc_max_queries(void);
+
 /* Host-only (no GPU needed): the reference-topology tree of a mesh, i.e.
  * UT_SolidAngle::init's triangle boxes (FastWindingNumberForSoups.h:6566-6590)
  * followed by UT::BVH<4>::init<BOX_AREA> (4579-4634).  Writes up to

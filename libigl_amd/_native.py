@@ -63,6 +63,8 @@ SIGNATURES = {
     "gwn_soup_query": (C.c_int, [vp, f64p, C.c_int64, C.c_float, C.c_int, f64p]),
     "gwn_soup_query_device": (C.c_int, [vp, vp, C.c_int64, C.c_float, C.c_int, vp, vp]),
     "gwn_soup_set_mixed_params": (C.c_int, [vp, C.POINTER(MixedParams)]),
+    "gwn_soup_query_device_noalloc": (C.c_int, [vp, vp, C.c_int64, C.c_float, C.c_int, vp, vp]),
+    "gwn_soup_noalloc_max_queries": (C.c_int64, []),
     "gwn_sah4_tree": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64)]),
     "gwn_soup_copy_nodes": (C.c_int, [vp, u32p]),
     "gwn_soup_copy_boxdata": (C.c_int, [vp, f32p]),

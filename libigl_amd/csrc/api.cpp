@@ -279,6 +279,19 @@ int gwn_soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float a
   });
 }
 
+int gwn_soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                                  double* W, void* stream) {
+  return guard([&] {
+    GWN_CHECK(s != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(nq >= 0, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(nq == 0 || (Q && W), GWN_ERR_INVALID, "null query/output");
+    GWN_CHECK(precision >= GWN_PREC_FP32 && precision <= GWN_PREC_FP64, GWN_ERR_INVALID, "unknown precision");
+    soup_query_device_noalloc(s, Q, nq, accuracy_scale, precision, W, (hipStream_t)stream);
+  });
+}
+
+int64_t gwn_soup_noalloc_max_queries(void) { return soup_noalloc_max_queries(); }
+
 int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accuracy_scale, int precision, double* W) {
   return guard([&] {
     GWN_CHECK(s != nullptr, GWN_ERR_INVALID, "null handle");

@@ -238,8 +238,18 @@ void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float ac
 // small chunks) or WALK_LANE_UNSORTED (per-lane in input order: no sort, a
 // workspace of 4 nq + 4 bytes); bitwise the same results
 constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2;
+// flags of soup_query_device_ws: WS_DEVICE_COUNT sizes the mixed form's FP64
+// re-evaluation on the device (no host round trip); WS_NO_PROFILE creates no
+// kernel-timing events
+constexpr int WS_DEVICE_COUNT = 1, WS_NO_PROFILE = 2;
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk);
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags = 0);
+// workspace bytes of a WALK_LANE_UNSORTED call
+size_t soup_small_work_bytes(int64_t nq);
+// scratch-free soup query over the static device scratch (soup.hip)
+void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                               double* W, hipStream_t st);
+int64_t soup_noalloc_max_queries();
 
 }  // namespace gwn
 

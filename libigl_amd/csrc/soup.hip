@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -643,10 +644,12 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
                                                     uint32_t nslots, const float* __restrict__ tri,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ perm,
-                                                    int64_t nq, double acc2, double* __restrict__ W) {
-  // the same wave-coherent rope walk as k_traverse, over the FP64 records
+                                                    int64_t nq, const uint32_t* __restrict__ dcount, double acc2,
+                                                    double* __restrict__ W) {
+  // the same wave-coherent rope walk as k_traverse, over the FP64 records;
+  // dcount (when given) holds the number of queries, read on the device
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
+  if (t >= (dcount ? (int64_t)*dcount : nq)) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   double sum = 0.0;
@@ -921,7 +924,7 @@ size_t soup_query_work_bytes(int64_t nq) {
   return up(4 * (size_t)nq) * 2 + up(radix_sort_temp_bytes(nq, false)) + up(4 * (size_t)nq) + up(4);
 }
 // Workspace of an unsorted (per-lane) call: the ambiguous list and count only.
-static size_t soup_small_work_bytes(int64_t nq) { return (4 * (size_t)nq + 255) / 256 * 256 + 256; }
+size_t soup_small_work_bytes(int64_t nq) { return (4 * (size_t)nq + 255) / 256 * 256 + 256; }
 
 // 30-bit Morton keys of the queries over a box (queries outside are clamped
 // to it), and the identity permutation to be sorted along with them
@@ -943,6 +946,44 @@ void hilbert_keys(const float* Q, int64_t nq, const float* bb, uint32_t* keys, u
   check_launch("k_hilbert_q");
 }
 
+// ---------------------------------------------------------------------------
+// Scratch-free query (gwn_soup_query_device_noalloc).  Its workspace is this
+// file-scope device array, not pooled memory: each device that runs the
+// library holds one copy for the life of the process.  Host code never uses
+// the array's name as an address; it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device (with that device
+// current) on the first call there.  Every call on a device shares that
+// device's copy, so calls that overlap on one device race; the contents a
+// call needs zeroed (the ambiguous-query count) are zeroed by a kernel on the
+// call's stream.
+constexpr int64_t kNoAllocMaxQueries = (int64_t)1 << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)22 << 20;  // >= soup_query_work_bytes(kNoAllocMaxQueries)
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+static uint8_t* noalloc_scratch() {
+  static std::mutex mu;
+  static uint8_t* ptr[kNoAllocMaxDevices] = {};
+  int dev = 0;
+  GWN_HIP(hipGetDevice(&dev));
+  GWN_CHECK(dev >= 0 && dev < kNoAllocMaxDevices, GWN_ERR_NODEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      throw Error{GWN_ERR_HIP, std::string("hipGetSymbolAddress of the query scratch failed: ") + hipGetErrorString(e)};
+    }
+    GWN_CHECK(soup_query_work_bytes(kNoAllocMaxQueries) <= kNoAllocScratchBytes, GWN_ERR_INVALID,
+              "scratch-free query: workspace layout exceeds the static scratch");
+    ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  return ptr[dev];
+}
+
+int64_t soup_noalloc_max_queries() { return kNoAllocMaxQueries; }
+
 // Batches below this size are walked per lane in input order: the Hilbert
 // sort (a key kernel and four radix passes of three kernels) would cost more
 // than the coherence it buys.
@@ -957,12 +998,30 @@ void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accu
                        sorted ? WALK_WAVE : WALK_LANE_UNSORTED);
 }
 
-// Query with caller-provided device workspace (soup_query_work_bytes(nq)
-// bytes); calls with distinct workspaces may run concurrently on distinct
-// streams against the same tree once its FP64 moments exist.
-void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
-                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk) {
+// The scratch-free form of soup_query_device: same kernels, same results;
+// the workspace is the static scratch, the mixed form's FP64 re-evaluation is
+// sized on the device (no host round trip) and no timing events are made.
+void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                               double* W, hipStream_t st) {
+  GWN_CHECK(nq <= kNoAllocMaxQueries, GWN_ERR_INVALID,
+            "scratch-free query: at most 1,048,576 queries per call (got " + std::to_string(nq) + ")");
   if (nq <= 0) return;
+  uint8_t* ws = noalloc_scratch();
+  const bool sorted = nq >= kSortMin;
+  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws, kNoAllocScratchBytes,
+                       sorted ? WALK_WAVE : WALK_LANE_UNSORTED, WS_DEVICE_COUNT | WS_NO_PROFILE);
+}
+
+// Query with caller-provided device workspace (soup_query_work_bytes(nq)
+// bytes, or soup_small_work_bytes(nq) for WALK_LANE_UNSORTED); calls with
+// distinct workspaces may run concurrently on distinct streams against the
+// same tree.  flags: WS_DEVICE_COUNT launches the mixed form's FP64
+// re-evaluation over every query with the count read on the device (no host
+// round trip); WS_NO_PROFILE creates no kernel-timing events.
+void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                          double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags) {
+  if (nq <= 0) return;
+  const bool prof_on = (flags & WS_NO_PROFILE) == 0;
   if (s->nslots == 0) {
     fill_device<double>(W, nq, 0.0, st);
     return;
@@ -983,7 +1042,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
     const size_t tmpb = radix_sort_temp_bytes(nq, false);
     void* tmp = take(tmpb);
     hilbert_keys(Q, nq, s->bbox, keys, perm, st);
-    ProfileScope prof("radix_sort", st);
+    ProfileScope prof("radix_sort", st, prof_on);
     radix_sort_u32(keys, perm, nq, 30, tmp, tmpb, st);
   }
   uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
@@ -994,11 +1053,11 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   const uint32_t ns = (uint32_t)s->nslots;
   if (precision == GWN_PREC_FP64) {
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
-    ProfileScope prof("soup_traverse64", st);
+    ProfileScope prof("soup_traverse64", st, prof_on);
     switch (s->order) {
-      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
-      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
-      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, a2, W); break;
+      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
     }
     check_launch("k_traverse64");
     return;
@@ -1015,7 +1074,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
                          acc2, nd, W);                                                                          \
   } while (0)
   {
-    ProfileScope prof("soup_traverse", st);
+    ProfileScope prof("soup_traverse", st, prof_on);
     if (mixed) {
       switch (s->order) {
         case 0: GWN_LAUNCH_T(0, true); break;
@@ -1037,20 +1096,31 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
     hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
                        ambn);
     check_launch("k_collect_ambiguous");
-    // the count comes back to the host so that the FP64 pass is launched at its
-    // size (a launch sized for every query, reading the count on the device,
-    // measured slower under concurrent serving load)
-    uint32_t na = 0;
-    GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
-    GWN_HIP(hipStreamSynchronize(st));
-    if (na > 0) {
-      const double a2 = (double)accuracy_scale * (double)accuracy_scale;
-      const unsigned ga = div_up(na, 256);
-      ProfileScope prof("soup_traverse64", st);
+    const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+    int64_t na64 = 0;
+    unsigned ga = 0;
+    const uint32_t* dcount = nullptr;
+    if (flags & WS_DEVICE_COUNT) {
+      // sized for every query; threads beyond the device-side count exit
+      na64 = nq;
+      ga = g;
+      dcount = ambn;
+    } else {
+      // the count comes back to the host so that the FP64 pass is launched at
+      // its size (a launch sized for every query, reading the count on the
+      // device, measured slower under concurrent serving load)
+      uint32_t na = 0;
+      GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
+      GWN_HIP(hipStreamSynchronize(st));
+      na64 = na;
+      ga = div_up(na, 256);
+    }
+    if (na64 > 0) {
+      ProfileScope prof("soup_traverse64", st, prof_on);
       switch (s->order) {
-        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
-        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
-        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, (int64_t)na, a2, W); break;
+        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
+        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
+        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
       }
       check_launch("k_traverse64 (ambiguous)");
     }
