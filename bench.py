@@ -631,43 +631,112 @@ def run_cloud(no_cpu):
     return out
 
 
+# FP64 FLOPs of one solid angle (solid_angle.cpp:10-55): 9 subtractions, 3
+# norms (6 each), the determinant (17), 3 dot products (15), the denominator
+# (8), the division by 2 pi and the add into the sum (2); the FP64 atan2
+# counted as 20 (argument reduction and a rational approximation).
+FLOP_SOLID_ANGLE = 9 + 18 + 17 + 15 + 8 + 2 + 20
+
+
 def run_exact(no_cpu):
-    """Exact FP64 winding number: level-5 noisy icosphere (20,480 triangles),
-    100k queries uniform in [-1.5,1.5]^3."""
+    """Exact FP64 winding number (igl::winding_number), level-5 noisy
+    icosphere (20,480 triangles), 100k queries uniform in [-1.5,1.5]^3: the
+    reference's hierarchy (built per call, as the reference does; and
+    prebuilt, queries only) and the brute-force sum over every face; plus
+    the hierarchy on the 1,310,720-triangle mesh (100k queries, 80% uniform,
+    20% within 1e-4 of the surface)."""
     import numpy as np
 
     from libigl_amd import _native as nat
-    from libigl_amd import devmem as dm
     from libigl_amd import datagen as D
+    from libigl_amd import devmem as dm
+    from libigl_amd import gwn
 
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
     V, F, Q = D.exact_workload()
     st = _stream().ptr
+    nq = Q.shape[0]
     dV = dm.DeviceArray.from_numpy(np.ascontiguousarray(V))
     dF = dm.DeviceArray.from_numpy(F)
     dQ = dm.DeviceArray.from_numpy(np.ascontiguousarray(Q))
-    dW = dm.DeviceArray((Q.shape[0],), np.float64)
+    dW = dm.DeviceArray((nq,), np.float64)
+    # brute force over every face
     dt, kms = _timed_device(lambda: nat.check(nat.lib().gwn_exact_winding_number_device(
-        dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0], dQ.data_ptr, Q.shape[0], dW.data_ptr, st)),
-        5, "exact")
-    W = dW.numpy()
-    pairs = Q.shape[0] * F.shape[0]
-    out = {"workload": "20,480 tris, 100k queries, FP64", "queries_per_s": round(Q.shape[0] / dt, 1),
-           "kernel_ms": round(kms, 4), "pairs_per_s": round(pairs / (kms * 1e-3), 1)}
-    if not no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as O
-        sub = np.arange(0, Q.shape[0], 50)
-        threads = max(1, min(16, host_info()['nproc'] or 1))
+        dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0], dQ.data_ptr, nq, dW.data_ptr, st)), 5, "exact")
+    Wb = dW.numpy()
+    pairs = nq * F.shape[0]
+    brute = {"queries_per_s": round(nq / dt, 1), "kernel_ms": round(kms, 4),
+             "pairs_per_s": round(pairs / (kms * 1e-3), 1)}
+    brute["roofline"] = _fp64_roofline(pairs, kms, "k_exact")
+    # the hierarchy: one call as igl::winding_number makes it (build + queries, host arrays)
+    gwn.winding_number(V, F, Q[:10])
+    reps = []
+    for _ in range(3):
         t = time.perf_counter()
-        Wo = O.winding_number_bf(V, F, Q[sub], threads)
-        cdt = time.perf_counter() - t
-        out["max_abs_dw_vs_oracle"] = float(np.abs(W[sub] - Wo).max())
+        Wh = gwn.winding_number(V, F, Q)
+        reps.append(time.perf_counter() - t)
+    call_s = float(np.median(reps))
+    tb = time.perf_counter()
+    tree = gwn.WindingNumberAABB(V, F)
+    build_s = time.perf_counter() - tb
+    dt2, kms2 = _timed_device(lambda: nat.check(nat.lib().gwn_exact_tree_query_device(
+        tree._h, dQ.data_ptr, nq, dW.data_ptr, st)), 5, "exact_tree")
+    Wt = dW.numpy()
+    otree = O.WNTree(V, F)
+    samp = np.arange(0, nq, 50)
+    tris_per_q = float(np.mean([otree.count(Q[i]) for i in samp]))
+    hier = {"call_s": round(call_s, 4), "call_queries_per_s": round(nq / call_s, 1), "build_s": round(build_s, 4),
+            "queries_per_s": round(nq / dt2, 1), "kernel_ms": round(kms2, 4),
+            "nodes": int(tree.nodes().shape[0]), "solid_angles_per_query": round(tris_per_q, 1),
+            "max_abs_dw_vs_bruteforce": float(np.abs(Wh - Wb).max()),
+            "prebuilt_equals_one_shot": bool(np.array_equal(Wt, Wh))}
+    hier["roofline"] = _fp64_roofline(tris_per_q * nq, kms2, "k_wn_tree")
+    out = {"workload": "20,480 tris, 100k queries, FP64", "hierarchy": hier, "bruteforce": brute,
+           "queries_per_s": hier["queries_per_s"], "kernel_ms": hier["kernel_ms"]}
+    sub = np.arange(0, nq, 50)
+    threads = max(1, min(16, host_info()['nproc'] or 1))
+    t = time.perf_counter()
+    Wo = otree.query(Q[sub], threads)
+    cdt = time.perf_counter() - t
+    out["max_abs_dw_vs_oracle"] = float(np.abs(Wh[sub] - Wo).max())
+    if not no_cpu:
         out["cpu_baseline"] = {"value": round(sub.size / cdt, 1), "unit": "queries/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"{sub.size} queries; brute-force FP64 port (every triangle per query), "
-                                         "not the reference's WindingNumberAABB hierarchy"}
+                               "sample": f"{sub.size} queries through the oracle's restatement of the reference's "
+                                         "WindingNumberAABB hierarchy (prebuilt), igl::parallel_for"}
+    # 1,310,720 triangles
+    Vb, Fb = D.bumpy_sphere(8, 64, 44)
+    rng = np.random.default_rng(46)
+    Qb = np.concatenate([D.uniform_queries(Vb.min(0), Vb.max(0), 80_000, rng),
+                         D.near_surface_queries(Vb, Fb, 20_000, 1e-4, rng)])
+    tb = time.perf_counter()
+    tree_b = gwn.WindingNumberAABB(Vb, Fb)
+    build_b = time.perf_counter() - tb
+    dQb = dm.DeviceArray.from_numpy(Qb)
+    dWb = dm.DeviceArray((Qb.shape[0],), np.float64)
+    dt3, kms3 = _timed_device(lambda: nat.check(nat.lib().gwn_exact_tree_query_device(
+        tree_b._h, dQb.data_ptr, Qb.shape[0], dWb.data_ptr, st)), 3, "exact_tree")
+    Wtb = dWb.numpy()
+    chk = np.arange(0, Qb.shape[0], 50)
+    Wbb = gwn.winding_number_bruteforce(Vb, Fb, Qb[chk])
+    big = {"workload": "1,310,720 tris, 100k queries (80% uniform, 20% within 1e-4), FP64",
+           "build_s": round(build_b, 3), "nodes": int(tree_b.nodes().shape[0]),
+           "queries_per_s": round(Qb.shape[0] / dt3, 1), "kernel_ms": round(kms3, 4),
+           "max_abs_dw_vs_bruteforce": float(np.abs(Wtb[chk] - Wbb).max()),
+           "bruteforce_pairs_per_s_equivalent": round(Qb.shape[0] * Fb.shape[0] / (kms3 * 1e-3), 1)}
+    out["hierarchy_1310720_tris"] = big
     log(f"[bench] exact: {out}")
     return out
+
+
+def _fp64_roofline(evals, kernel_ms, kernel):
+    if not kernel_ms or kernel_ms <= 0:
+        return None
+    achieved = evals * FLOP_SOLID_ANGLE / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": 78.6, "unit": "TFLOP/s",
+            "frac": round(achieved / 78.6, 4), "kernel": kernel, "flop_per_solid_angle": FLOP_SOLID_ANGLE,
+            "traffic": None, "note": "FP64 VALU (78.6 TFLOP/s vector FP64 peak); atan2 counted as 20 FLOP"}
 
 
 def run_signed_distance(no_cpu):

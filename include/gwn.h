@@ -89,8 +89,9 @@ const char* gwn_version(void);
 
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
- * "soup_traverse64", "exact", "cloud_traverse", "radix_sort", "closest_point",
- * "knn", "point_areas", "cloud_brute", "exact2d", "curve_traverse") with HIP events
+ * "soup_traverse64", "exact", "exact_tree", "cloud_traverse", "radix_sort",
+ * "closest_point", "knn", "point_areas", "cloud_brute", "exact2d",
+ * "curve_traverse") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises
  * on those events and returns the accumulated time and launch count. */
 int gwn_profile_enable(int on);
@@ -105,14 +106,50 @@ int gwn_profile_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Exact FP64 generalized winding number.
- * Replaces igl::winding_number(V,F,O,W), winding_number.h:41-48
- * (winding_number.cpp:22-56, 3D branch): W(o) = sum_f solid_angle(f, O(o)),
- * solid_angle.cpp:10-55.  V nv x 3, F nf x 3, O no x 3, W no.
+ * gwn_exact_winding_number replaces igl::winding_number(V,F,O,W),
+ * winding_number.h:41-48 (winding_number.cpp:22-56, 3D branch) and follows
+ * it: the reference's hierarchy (igl::WindingNumberAABB, built for the call:
+ * duplicate vertices merged, median splits down to 100 faces, every node's
+ * open boundary closed by a triangle fan whose apex rand() picks, so the
+ * call consumes rand() values as the reference's does), then, per origin,
+ * solid_angle.cpp:10-55 summed over the faces of the leaves whose boxes
+ * contain it and over the fans of the other nodes it meets.  Equal to the sum
+ * over every face up to rounding.  V nv x 3, F nf x 3, O no x 3, W no; host
+ * arrays.
+ * gwn_exact_winding_number_device takes device arrays and sums over every
+ * face (the brute-force form of the same quantity; no hierarchy, no rand()).
  */
 int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
                              int64_t no, double* W);
 int gwn_exact_winding_number_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
                                     int64_t no, double* W, void* stream);
+/* Replaces igl::winding_number(V,F,p), winding_number.h:49-60
+ * (winding_number.cpp:62-89): solid_angle summed over every face, for each
+ * of the no points; host arrays (the same sum as the _device form). */
+int gwn_exact_winding_number_bruteforce(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                        int64_t no, double* W);
+
+/* The hierarchy of gwn_exact_winding_number, built once and queried many
+ * times (igl::WindingNumberAABB<double,int> hier(V,F); hier.grow(); then
+ * hier.winding_number(p) per origin, WindingNumberAABB.h / WindingNumberTree.h).
+ * The build draws rand() as the reference's constructors do.
+ * gwn_exact_tree_nodes writes 5 int64 per node in the reference's
+ * construction (depth-first pre-) order: #faces, #cap triangles, the fan's
+ * apex (an index into the vertices de-duplicated and sorted
+ * lexicographically; -1 when the node has no open boundary), left child,
+ * right child (-1 for a leaf).  Queries: O no x 3, W no (FP64). */
+typedef struct gwn_exact_tree gwn_exact_tree;
+int gwn_exact_tree_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, gwn_exact_tree** out);
+void gwn_exact_tree_free(gwn_exact_tree* t);
+int gwn_exact_tree_info(const gwn_exact_tree* t, int64_t* num_nodes, int64_t* num_triangles);
+int gwn_exact_tree_nodes(const gwn_exact_tree* t, int64_t* out);
+int gwn_exact_tree_query(const gwn_exact_tree* t, const double* O, int64_t no, double* W);
+int gwn_exact_tree_query_device(const gwn_exact_tree* t, const double* O, int64_t no, double* W, void* stream);
+/* Host-only (no GPU needed): the same hierarchy built on the host (drawing
+ * rand() as gwn_exact_tree_build does); writes up to `capacity` nodes in the
+ * gwn_exact_tree_nodes layout and the node count. */
+int gwn_exact_tree_host_nodes(const double* V, int64_t nv, const int32_t* F, int64_t nf, int64_t* nodes,
+                              int64_t capacity, int64_t* num_nodes);
 
 /* ------------------------------------------------------------------------
  * Triangle soup (UT_SolidAngle).

@@ -3,6 +3,7 @@
 // Replaces, with the same names, argument meaning and output resizing:
 //   igl::winding_number(V,F,O,W)                      winding_number.h:41-48 (triangles, or 2D edges)
 //   igl::winding_number(V,F,p)                        winding_number.h:49-60
+//   igl::WindingNumberAABB<Scalar,Index>              WindingNumberAABB.h (the exact hierarchy)
 //   igl::octree(P,point_indices,CH,CN,W)              octree.h:36-41
 //   igl::fast_winding_number(P,N,A,point_indices,CH,expansion_order,CM,R,EC)
 //                                                     fast_winding_number.h:44-53
@@ -75,6 +76,11 @@ struct FastWindingNumberBVH {
   gwn_soup* handle = nullptr;
   int tree_kind = GWN_TREE_SAH4;   // reference topology by default
   int precision = GWN_PREC_FP32;   // reference arithmetic by default
+  // fingerprint of the vertices the tree was built from (as doubles), so that
+  // signed_distance_fast_winding_number can tell whether its closest points
+  // may come from this tree's vertices
+  uint64_t vertex_hash = 0;
+  int64_t num_vertices = -1;
   FastWindingNumberBVH() = default;
   FastWindingNumberBVH(const FastWindingNumberBVH&) = delete;
   FastWindingNumberBVH& operator=(const FastWindingNumberBVH&) = delete;
@@ -87,6 +93,14 @@ namespace gwn_detail {
 
 inline void check(int rc) {
   if (rc != GWN_OK) throw std::runtime_error(std::string("gwn: ") + gwn_last_error());
+}
+
+// FNV-1a over the bytes of a double array
+inline uint64_t hash_doubles(const std::vector<double>& v) {
+  uint64_t h = 1469598103934665603ull;
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(v.data());
+  for (size_t i = 0; i < v.size() * sizeof(double); ++i) h = (h ^ b[i]) * 1099511628211ull;
+  return h;
 }
 
 template <typename T, typename M>
@@ -106,6 +120,70 @@ std::vector<int32_t> faces(const M& F) {
     for (int j = 0; j < 3; ++j) out[3 * i + j] = static_cast<int32_t>(F(i, j));
   return out;
 }
+
+}  // namespace gwn_detail
+
+// igl::WindingNumberAABB<Scalar, Index> (WindingNumberAABB.h): the hierarchy
+// igl::winding_number(V,F,O,W) builds, kept on the GPU.  Construct it over
+// (V, F), call grow() (which builds it, drawing rand() as the reference's
+// constructors and grow() do), then winding_number(p).  Without grow() the
+// reference keeps only the root; here the first query builds the tree (call
+// grow() before querying from several threads).  Not copyable; freed with
+// the object.
+template <typename Scalar, typename Index>
+class WindingNumberAABB;
+template <typename T>
+struct is_winding_number_aabb : std::false_type {};
+template <typename S, typename I>
+struct is_winding_number_aabb<WindingNumberAABB<S, I>> : std::true_type {};
+
+template <typename Scalar, typename Index>
+class WindingNumberAABB {
+ public:
+  WindingNumberAABB() = default;
+  template <typename MV, typename MF>
+  WindingNumberAABB(const MV& V, const MF& F) {
+    set_mesh(V, F);
+  }
+  WindingNumberAABB(const WindingNumberAABB&) = delete;
+  WindingNumberAABB& operator=(const WindingNumberAABB&) = delete;
+  ~WindingNumberAABB() {
+    if (handle) gwn_exact_tree_free(handle);
+  }
+  template <typename MV, typename MF>
+  void set_mesh(const MV& V, const MF& F) {
+    if (handle) gwn_exact_tree_free(handle);
+    handle = nullptr;
+    V_ = gwn_detail::rows3<double>(V, "V");
+    F_ = gwn_detail::faces(F);
+  }
+  void grow() {
+    if (handle) gwn_exact_tree_free(handle);
+    handle = nullptr;
+    gwn_detail::check(gwn_exact_tree_build(V_.data(), (int64_t)V_.size() / 3, F_.data(), (int64_t)F_.size() / 3,
+                                           &handle));
+  }
+  // WindingNumberTree::winding_number(p) (WindingNumberTree.h:262-336)
+  template <typename Mp>
+  Scalar winding_number(const Mp& p) const {
+    const double o[3] = {static_cast<double>(p(0, 0)), static_cast<double>(p(0, 1)), static_cast<double>(p(0, 2))};
+    double w = 0.0;
+    gwn_detail::check(gwn_exact_tree_query(tree(), o, 1, &w));
+    return static_cast<Scalar>(w);
+  }
+  // the built tree (built here if grow() was not called)
+  const gwn_exact_tree* tree() const {
+    if (!handle) const_cast<WindingNumberAABB*>(this)->grow();
+    return handle;
+  }
+  gwn_exact_tree* handle = nullptr;
+
+ private:
+  std::vector<double> V_;
+  std::vector<int32_t> F_;
+};
+
+namespace gwn_detail {
 
 template <typename MV, typename MF, typename MO, typename MW>
 void winding_number(const MV& V, const MF& F, const MO& O, MW& W) {
@@ -140,8 +218,8 @@ double winding_number_point(const MV& V, const MF& F, const Mp& p) {
   const auto v = rows3<double>(V, "V");
   const auto f = faces(F);
   const double o[3] = {static_cast<double>(p(0, 0)), static_cast<double>(p(0, 1)), static_cast<double>(p(0, 2))};
-  double w = 0;
-  check(gwn_exact_winding_number(v.data(), V.rows(), f.data(), F.rows(), o, 1, &w));
+  double w = 0;  // winding_number.cpp:62-89: the sum over every face
+  check(gwn_exact_winding_number_bruteforce(v.data(), V.rows(), f.data(), F.rows(), o, 1, &w));
   return w;
 }
 
@@ -154,6 +232,8 @@ void build(const MV& V, const MF& F, int order, FastWindingNumberBVH& bvh) {
     bvh.handle = nullptr;
   }
   check(gwn_soup_build(v.data(), V.rows(), f.data(), F.rows(), order, bvh.tree_kind, &bvh.handle));
+  bvh.vertex_hash = hash_doubles(v);
+  bvh.num_vertices = V.rows();
 }
 
 template <typename MQ, typename MW>
@@ -439,10 +519,12 @@ void cloud_estimated_areas(const MP& P, const MN& N, const MQ& Q, int order, dou
 
 // signed_distance.cpp:611-629 for one point q: closest point on the soup `h`
 // (squared distance recomputed from it, summed as the device sums it) and
-// s = 1 - 2 w with the exact winding number
+// s = 1 - 2 w with the exact winding number, from the hierarchy `hier` when
+// the caller passed one (hier.winding_number(q), as the reference), else
+// summed over every face
 template <typename MV, typename MF, typename Mq>
-void signed_distance_wn_point(gwn_soup* h, const MV& V, const MF& F, const Mq& q, double& s, double& sqrd, int& i,
-                              double c[3]) {
+void signed_distance_wn_point(gwn_soup* h, const gwn_exact_tree* hier, const MV& V, const MF& F, const Mq& q,
+                              double& s, double& sqrd, int& i, double c[3]) {
   const double p[3] = {static_cast<double>(q(0, 0)), static_cast<double>(q(0, 1)), static_cast<double>(q(0, 2))};
   double S = 0.0;
   int32_t I = -1;
@@ -451,10 +533,14 @@ void signed_distance_wn_point(gwn_soup* h, const MV& V, const MF& F, const Mq& q
   const double e0 = p[0] - c[0], e1 = p[1] - c[1], e2 = p[2] - c[2];
   sqrd = (e0 * e0 + e1 * e1) + e2 * e2;
   i = I;
-  const auto v = rows3<double>(V, "V");
-  const auto f = faces(F);
   double w = 0.0;
-  check(gwn_exact_winding_number(v.data(), V.rows(), f.data(), F.rows(), p, 1, &w));
+  if (hier) {
+    check(gwn_exact_tree_query(hier, p, 1, &w));
+  } else {  // the sum over every face (no hierarchy built for one point)
+    const auto v = rows3<double>(V, "V");
+    const auto f = faces(F);
+    check(gwn_exact_winding_number_bruteforce(v.data(), V.rows(), f.data(), F.rows(), p, 1, &w));
+  }
   s = 1. - 2. * w;
 }
 
@@ -657,62 +743,99 @@ void signed_distance(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBa
                   C, N);
 }
 
-// signed_distance.cpp:665-687: the caller's tree is not needed (the closest
-// points use fwn_bvh's own tree); S = sqrt(d2) (1 - 2 |w|)
+namespace gwn_detail {
+// The soup whose vertices equal V (as doubles): `tree` when it is a built
+// FastWindingNumberBVH, else fwn_bvh when it was built from the same
+// vertices, else a tree built here from V (owned by `tmp`).
+template <typename Tree, typename MV, typename MF>
+gwn_soup* closest_point_soup(const Tree& tree, const FastWindingNumberBVH& fwn_bvh, const MV& V, const MF& F,
+                             FastWindingNumberBVH& tmp) {
+  if constexpr (std::is_same<Tree, FastWindingNumberBVH>::value) {
+    if (tree.handle) return tree.handle;
+  }
+  const auto v = rows3<double>(V, "V");
+  if (fwn_bvh.handle && fwn_bvh.num_vertices == (int64_t)V.rows() && fwn_bvh.vertex_hash == hash_doubles(v))
+    return fwn_bvh.handle;
+  build(V, F, 0, tmp);
+  return tmp.handle;
+}
+
+// signed_distance.cpp:665-711 for the points of P: unsigned distance and
+// closest point from `closest`, sign 1 - 2|w| with w = fast_winding_number(
+// fwn_bvh, 2, p.cast<float>()) as a float
+template <typename MP, typename MS>
+void signed_distance_fwn(gwn_soup* closest, const FastWindingNumberBVH& fwn_bvh, const MP& P, MS& S) {
+  const auto p = rows3<double>(P, "P");
+  const int64_t n = P.rows();
+  std::vector<double> d((size_t)n), c(3 * (size_t)n), w((size_t)n);
+  std::vector<int32_t> i((size_t)n);
+  if (n > 0) {
+    check(gwn_signed_distance(closest, p.data(), n, GWN_SIGN_UNSIGNED, std::numeric_limits<double>::min(),
+                              std::numeric_limits<double>::max(), d.data(), i.data(), c.data()));
+    check(gwn_soup_query(fwn_bvh.handle, p.data(), n, 2.0f, GWN_PREC_FP32, w.data()));
+  }
+  S.resize(n, 1);
+  for (int64_t k = 0; k < n; ++k) {
+    const double wf = (double)(float)w[k];
+    S(k, 0) = static_cast<typename std::decay<decltype(S(0, 0))>::type>((1. - 2. * std::fabs(wf)) * d[k]);
+  }
+}
+}  // namespace gwn_detail
+
+// signed_distance.cpp:665-687: closest points from `tree` when it is a
+// FastWindingNumberBVH over (V, F), else from fwn_bvh when that was built
+// from V itself, else from a tree built here from V (the reference's AABB is
+// over the caller's V; fwn_bvh may hold other, e.g. float-rounded, vertices);
+// the sign from fwn_bvh at accuracy_scale 2; S = sqrt(d2) (1 - 2 |w|)
 template <typename DerivedP, typename DerivedV, typename DerivedF, typename Tree, typename DerivedS>
 void signed_distance_fast_winding_number(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBase<DerivedV>& V,
                                          const Eigen::MatrixBase<DerivedF>& F, const Tree& tree,
                                          const FastWindingNumberBVH& fwn_bvh, Eigen::PlainObjectBase<DerivedS>& S) {
-  (void)V;
-  (void)F;
-  (void)tree;
   if (!fwn_bvh.handle) throw std::runtime_error("FastWindingNumberBVH not built");
-  Eigen::Matrix<int, Eigen::Dynamic, 1> I;
-  Eigen::Matrix<double, Eigen::Dynamic, 3> C;
-  gwn_detail::signed_distance_bvh(fwn_bvh.handle, P.derived(), GWN_SIGN_FAST_WINDING_NUMBER,
-                                  std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), S.derived(),
-                                  I, C);
+  FastWindingNumberBVH tmp;
+  gwn_soup* closest = gwn_detail::closest_point_soup(tree, fwn_bvh, V.derived(), F.derived(), tmp);
+  gwn_detail::signed_distance_fwn(closest, fwn_bvh, P.derived(), S.derived());
 }
 
-// signed_distance.cpp:688-711: one point, sign from fwn_bvh (accuracy_scale 2)
+// signed_distance.cpp:688-711: one point, the same closest-point tree choice
 template <typename Derivedq, typename DerivedV, typename DerivedF, typename Tree>
 typename DerivedV::Scalar signed_distance_fast_winding_number(const Eigen::MatrixBase<Derivedq>& q,
                                                               const Eigen::MatrixBase<DerivedV>& V,
                                                               const Eigen::MatrixBase<DerivedF>& F, const Tree& tree,
                                                               const FastWindingNumberBVH& fwn_bvh) {
-  (void)V;
-  (void)F;
-  (void)tree;
   if (!fwn_bvh.handle) throw std::runtime_error("FastWindingNumberBVH not built");
-  const double p[3] = {static_cast<double>(q.derived()(0, 0)), static_cast<double>(q.derived()(0, 1)),
-                       static_cast<double>(q.derived()(0, 2))};
-  double S = 0.0, c[3];
-  int32_t I = -1;
-  gwn_detail::check(gwn_signed_distance(fwn_bvh.handle, p, 1, GWN_SIGN_FAST_WINDING_NUMBER,
-                                        std::numeric_limits<double>::min(), std::numeric_limits<double>::max(), &S,
-                                        &I, c));
-  return static_cast<typename DerivedV::Scalar>(S);
+  FastWindingNumberBVH tmp;
+  gwn_soup* closest = gwn_detail::closest_point_soup(tree, fwn_bvh, V.derived(), F.derived(), tmp);
+  Eigen::Matrix<double, Eigen::Dynamic, 1> S1;
+  Eigen::Matrix<double, 1, 3> q1;
+  for (int j = 0; j < 3; ++j) q1(0, j) = static_cast<double>(q.derived()(0, j));
+  gwn_detail::signed_distance_fwn(closest, fwn_bvh, q1, S1);
+  return static_cast<typename DerivedV::Scalar>(S1(0, 0));
 }
 
 // signed_distance.cpp:579-629 (3D): closest point and s = 1 - 2 w with the
 // exact winding number.  `tree` and `hier` stand for the reference's AABB and
-// WindingNumberAABB: a FastWindingNumberBVH built over (V, F) is used as the
-// closest-point tree; anything else (e.g. 0) makes the call build one.
+// WindingNumberAABB: a FastWindingNumberBVH built over (V, F) passed as `tree`
+// serves the closest point (anything else, e.g. 0, makes the call build one);
+// an igl::WindingNumberAABB passed as `hier` gives w (hier.winding_number(q),
+// as the reference); anything else sums over every face.
 template <typename Tree, typename DerivedV, typename DerivedF, typename Hier, typename Derivedq, typename Scalar,
           typename Derivedc>
 void signed_distance_winding_number(const Tree& tree, const Eigen::MatrixBase<DerivedV>& V,
                                     const Eigen::MatrixBase<DerivedF>& F, const Hier& hier,
                                     const Eigen::MatrixBase<Derivedq>& q, Scalar& s, Scalar& sqrd, int& i,
                                     Eigen::PlainObjectBase<Derivedc>& c) {
-  (void)hier;
+  const gwn_exact_tree* h = nullptr;
+  if constexpr (is_winding_number_aabb<Hier>::value) h = hier.tree();
+  else (void)hier;
   double sd = 0.0, d2 = 0.0, cc[3];
   if constexpr (std::is_same<Tree, FastWindingNumberBVH>::value) {
     if (!tree.handle) throw std::runtime_error("FastWindingNumberBVH not built");
-    gwn_detail::signed_distance_wn_point(tree.handle, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
+    gwn_detail::signed_distance_wn_point(tree.handle, h, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
   } else {
     FastWindingNumberBVH bvh;
     gwn_detail::build(V.derived(), F.derived(), 0, bvh);
-    gwn_detail::signed_distance_wn_point(bvh.handle, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
+    gwn_detail::signed_distance_wn_point(bvh.handle, h, V.derived(), F.derived(), q.derived(), sd, d2, i, cc);
   }
   s = static_cast<Scalar>(sd);
   sqrd = static_cast<Scalar>(d2);

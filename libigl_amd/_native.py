@@ -56,6 +56,15 @@ SIGNATURES = {
     "gwn_profile_copy": (C.c_int, [vp, vp, C.c_int64, vp]),
     "gwn_exact_winding_number": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
     "gwn_exact_winding_number_device": (C.c_int, [vp, C.c_int64, vp, C.c_int64, vp, C.c_int64, vp, vp]),
+    "gwn_exact_winding_number_bruteforce": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
+    "gwn_exact_tree_build": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, C.POINTER(vp)]),
+    "gwn_exact_tree_free": (None, [vp]),
+    "gwn_exact_tree_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "gwn_exact_tree_nodes": (C.c_int, [vp, i64p]),
+    "gwn_exact_tree_query": (C.c_int, [vp, f64p, C.c_int64, f64p]),
+    "gwn_exact_tree_query_device": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
+    "gwn_exact_tree_host_nodes": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, vp, C.c_int64,
+                                            C.POINTER(C.c_int64)]),
     "gwn_soup_build": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
     "gwn_soup_build_f32": (C.c_int, [f32p, C.c_int64, i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
     "gwn_soup_free": (None, [vp]),

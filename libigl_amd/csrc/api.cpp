@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "gwn_internal.h"
+#include "wntree.h"
 
 namespace gwn {
 
@@ -32,6 +33,7 @@ void exact2d_device(const double* V, const int32_t* E, int64_t ne, const double*
                     hipStream_t st);
 void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
                   hipStream_t st);
+void wntree_upload(const WNTreeHost& T, DevBuf<uint8_t>& rec, DevBuf<double>& tri, hipStream_t st);
 void soup_setup(gwn_soup* s, hipStream_t st);
 void soup_finish_build(gwn_soup* s, hipStream_t st);
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
@@ -148,6 +150,33 @@ void antipodal_build(gwn_antipodal* a, const double* V, int64_t nv, const int32_
                      hipStream_t st);
 void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no, double* W, hipStream_t st);
 
+// igl::WindingNumberAABB<double,int> hier(V,F); hier.grow()
+// (winding_number.cpp:43-47), built on the host (wntree.cpp) and uploaded
+static void exact_tree_build(gwn_exact_tree* t, const double* V, int64_t nv, const int32_t* F, int64_t nf) {
+  t->nv = nv;
+  t->nf = nf;
+  WNTreeHost T;
+  wntree_build_host(V, nv, F, nf, T);
+  t->nrec = (int64_t)T.rec.size();
+  t->ntri = (int64_t)(T.tri.size() / 9);
+  if (t->nrec) {
+    for (int d = 0; d < 3; ++d) {
+      const double lo = T.rec[0].lo[d], hi = T.rec[0].hi[d];
+      t->bbox[d] = std::isfinite(lo) ? (float)lo : 0.0f;
+      t->bbox[3 + d] = std::isfinite(hi) ? (float)hi : 0.0f;
+    }
+  }
+  t->nodes.resize(5 * (size_t)t->nrec);
+  for (int64_t i = 0; i < t->nrec; ++i) {
+    t->nodes[5 * i] = T.nfaces[i];
+    t->nodes[5 * i + 1] = T.ncap[i];
+    t->nodes[5 * i + 2] = T.apex[i];
+    t->nodes[5 * i + 3] = T.left[i];
+    t->nodes[5 * i + 4] = T.right[i];
+  }
+  wntree_upload(T, t->rec, t->tri, nullptr);
+}
+
 }  // namespace gwn
 
 using namespace gwn;
@@ -183,6 +212,31 @@ int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int6
   return guard([&] {
     GWN_CHECK(nv >= 0 && nf >= 0 && no >= 0, GWN_ERR_INVALID, "negative size");
     GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    if (nf > 0) {
+      GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
+      check_faces(F, nf, nv);
+    }
+    ensure_device();
+    // winding_number.cpp:39-53: the hierarchy is built for the call (drawing
+    // rand() as the reference's does), then every origin is walked through it
+    gwn_exact_tree t;
+    exact_tree_build(&t, V, nv, F, nf);
+    if (no == 0) return;
+    hipStream_t st = nullptr;
+    Scratch io(scratch_bytes({24 * (size_t)no, 8 * (size_t)no}), st);
+    const double* dO = stage_in(io, O, 3 * (size_t)no, st);
+    double* dW = io.take_n<double>((size_t)no);
+    wntree_query_device(t.rec.p, t.nrec, t.tri.p, t.bbox, dO, no, dW, st);
+    stage_out(W, dW, (size_t)no, st);
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_exact_winding_number_bruteforce(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O,
+                                        int64_t no, double* W) {
+  return guard([&] {
+    GWN_CHECK(nv >= 0 && nf >= 0 && no >= 0, GWN_ERR_INVALID, "negative size");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
     if (no == 0) return;
     if (nf > 0) {
       GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
@@ -197,6 +251,93 @@ int gwn_exact_winding_number(const double* V, int64_t nv, const int32_t* F, int6
     const double* dO = stage_in(io, O, o, st);
     double* dW = io.take_n<double>((size_t)no);
     exact_device(dV, nv, dF, nf, dO, no, dW, st);
+    stage_out(W, dW, (size_t)no, st);
+    GWN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int gwn_exact_tree_build(const double* V, int64_t nv, const int32_t* F, int64_t nf, gwn_exact_tree** out) {
+  return guard([&] {
+    GWN_CHECK(out != nullptr, GWN_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    GWN_CHECK(nv >= 0 && nf >= 0, GWN_ERR_INVALID, "negative size");
+    if (nf > 0) {
+      GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
+      check_faces(F, nf, nv);
+    }
+    ensure_device();
+    gwn_exact_tree* t = new gwn_exact_tree();
+    try {
+      exact_tree_build(t, V, nv, F, nf);
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+void gwn_exact_tree_free(gwn_exact_tree* t) { delete t; }
+
+int gwn_exact_tree_host_nodes(const double* V, int64_t nv, const int32_t* F, int64_t nf, int64_t* nodes,
+                              int64_t capacity, int64_t* num_nodes) {
+  return guard([&] {
+    GWN_CHECK(num_nodes != nullptr, GWN_ERR_INVALID, "null node count");
+    GWN_CHECK(nv >= 0 && nf >= 0, GWN_ERR_INVALID, "negative size");
+    if (nf > 0) {
+      GWN_CHECK(V && F, GWN_ERR_INVALID, "null mesh");
+      check_faces(F, nf, nv);
+    }
+    WNTreeHost T;
+    wntree_build_host(V, nv, F, nf, T);
+    *num_nodes = (int64_t)T.rec.size();
+    GWN_CHECK(nodes == nullptr || capacity >= *num_nodes, GWN_ERR_INVALID, "node capacity too small");
+    if (nodes)
+      for (int64_t i = 0; i < *num_nodes; ++i) {
+        nodes[5 * i] = T.nfaces[i];
+        nodes[5 * i + 1] = T.ncap[i];
+        nodes[5 * i + 2] = T.apex[i];
+        nodes[5 * i + 3] = T.left[i];
+        nodes[5 * i + 4] = T.right[i];
+      }
+  });
+}
+
+int gwn_exact_tree_info(const gwn_exact_tree* t, int64_t* num_nodes, int64_t* num_triangles) {
+  return guard([&] {
+    GWN_CHECK(t != nullptr, GWN_ERR_INVALID, "null handle");
+    if (num_nodes) *num_nodes = t->nrec;
+    if (num_triangles) *num_triangles = t->ntri;
+  });
+}
+
+int gwn_exact_tree_nodes(const gwn_exact_tree* t, int64_t* out) {
+  return guard([&] {
+    GWN_CHECK(t && out, GWN_ERR_INVALID, "null argument");
+    if (!t->nodes.empty()) std::memcpy(out, t->nodes.data(), sizeof(int64_t) * t->nodes.size());
+  });
+}
+
+int gwn_exact_tree_query_device(const gwn_exact_tree* t, const double* O, int64_t no, double* W, void* stream) {
+  return guard([&] {
+    GWN_CHECK(t != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(no >= 0 && no < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    wntree_query_device(t->rec.p, t->nrec, t->tri.p, t->bbox, O, no, W, (hipStream_t)stream);
+  });
+}
+
+int gwn_exact_tree_query(const gwn_exact_tree* t, const double* O, int64_t no, double* W) {
+  return guard([&] {
+    GWN_CHECK(t != nullptr, GWN_ERR_INVALID, "null handle");
+    GWN_CHECK(no >= 0 && no < (int64_t)0xFFFFFFFF, GWN_ERR_INVALID, "bad query count");
+    GWN_CHECK(no == 0 || (O && W), GWN_ERR_INVALID, "null query/output");
+    if (no == 0) return;
+    hipStream_t st = nullptr;
+    Scratch io(scratch_bytes({24 * (size_t)no, 8 * (size_t)no}), st);
+    const double* dO = stage_in(io, O, 3 * (size_t)no, st);
+    double* dW = io.take_n<double>((size_t)no);
+    wntree_query_device(t->rec.p, t->nrec, t->tri.p, t->bbox, dO, no, dW, st);
     stage_out(W, dW, (size_t)no, st);
     GWN_HIP(hipStreamSynchronize(st));
   });

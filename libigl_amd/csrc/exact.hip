@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gwn_internal.h"
+#include "wntree.h"
 
 namespace gwn {
 
@@ -222,6 +223,90 @@ void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, con
     hipLaunchKernelGGL(k_exact_sum, dim3(div_up(no, 256)), dim3(256), 0, st, part, nch, no, true, W);
     check_launch("k_exact_sum");
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// The reference's hierarchy (wntree.cpp): per query, a walk over the
+// pre-order records.  Inside an internal node's closed box the walk steps to
+// its first child; otherwise it sums the node's triangle list (inside a leaf:
+// its faces; outside the box: its cap, or its faces when the cap is not
+// smaller) and jumps past the subtree.  Each term is solid_angle(...)
+// (solid_angle.cpp: atan2 over 2 pi) added to one running FP64 sum in walk
+// order (the reference adds a node's children's sums; the two orders agree to
+// rounding).  One query per lane, queries in Hilbert order so that a
+// wavefront's lanes mostly read the same records and triangles.
+__global__ __launch_bounds__(256) void k_wn_tree(const WNRecord* __restrict__ rec, uint32_t nrec,
+                                                  const double* __restrict__ tri, const double* __restrict__ O,
+                                                  const uint32_t* __restrict__ perm, int64_t no,
+                                                  double* __restrict__ W) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= no) return;
+  const int64_t q = perm ? (int64_t)perm[t] : t;
+  const double px = O[3 * q], py = O[3 * q + 1], pz = O[3 * q + 2];
+  double sum = 0.0;
+  uint32_t i = 0;
+  while (i < nrec) {
+    const WNRecord& r = rec[i];
+    // WindingNumberAABB::inside (WindingNumberAABB.h:238-253): closed box
+    const bool inside = !(px < r.lo[0] || px > r.hi[0] || py < r.lo[1] || py > r.hi[1] || pz < r.lo[2] ||
+                          pz > r.hi[2]);
+    if (inside && !r.leaf) {
+      ++i;
+      continue;
+    }
+    const uint32_t start = inside ? r.in_start : r.out_start;
+    const uint32_t count = inside ? r.in_count : r.out_count;
+    for (uint32_t k = 0; k < count; ++k)
+      sum += vos_atan2(tri + 9 * (size_t)(start + k), px, py, pz) / (2.0 * EX_PI);
+    i = r.link;
+  }
+  W[q] = sum;
+}
+
+__global__ void k_wn_narrow(const double* __restrict__ O, int64_t n3, float* __restrict__ Of) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) Of[i] = (float)O[i];
+}
+
+void wntree_upload(const WNTreeHost& T, DevBuf<uint8_t>& rec, DevBuf<double>& tri, hipStream_t st) {
+  rec.alloc(sizeof(WNRecord) * T.rec.size());
+  if (!T.rec.empty())
+    GWN_HIP(hipMemcpyAsync(rec.p, T.rec.data(), sizeof(WNRecord) * T.rec.size(), hipMemcpyHostToDevice, st));
+  tri.alloc(T.tri.size());
+  if (!T.tri.empty())
+    GWN_HIP(hipMemcpyAsync(tri.p, T.tri.data(), sizeof(double) * T.tri.size(), hipMemcpyHostToDevice, st));
+  GWN_HIP(hipStreamSynchronize(st));
+}
+
+// O no x 3 and W no, device pointers; bbox: the root's box (query order)
+void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, const float* bbox, const double* O,
+                         int64_t no, double* W, hipStream_t st) {
+  if (no <= 0) return;
+  if (nrec == 0) {
+    fill_device<double>(W, no, 0.0, st);
+    return;
+  }
+  const unsigned g = div_up(no, 256);
+  uint32_t* perm = nullptr;
+  // small batches walk in input order; larger ones along the Hilbert curve
+  const bool sorted = no >= 4096;
+  const size_t tb = sorted ? radix_sort_temp_bytes(no, false) : 0;
+  Scratch ws(sorted ? scratch_bytes({12 * (size_t)no, 4 * (size_t)no, 4 * (size_t)no, tb}) : 1, st);
+  if (sorted) {
+    float* Of = ws.take_n<float>(3 * (size_t)no);
+    uint32_t* keys = ws.take_n<uint32_t>((size_t)no);
+    perm = ws.take_n<uint32_t>((size_t)no);
+    void* tmp = ws.take(tb);
+    hipLaunchKernelGGL(k_wn_narrow, dim3(div_up(3 * no, 256)), dim3(256), 0, st, O, 3 * no, Of);
+    check_launch("k_wn_narrow");
+    hilbert_keys(Of, no, bbox, keys, perm, st);
+    radix_sort_u32(keys, perm, no, 30, tmp, tb, st);
+  }
+  ProfileScope prof("exact_tree", st);
+  hipLaunchKernelGGL(k_wn_tree, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec), (uint32_t)nrec,
+                     tri, O, perm, no, W);
+  check_launch("k_wn_tree");
 }
 
 }  // namespace gwn

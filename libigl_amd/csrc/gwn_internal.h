@@ -301,6 +301,22 @@ struct gwn_antipodal {
   int64_t nseg = 0;
   double x0[3] = {0, 0, 0};
 };
+// Exact winding-number hierarchy (igl::WindingNumberAABB): pre-order records
+// and triangle corners on the device; the per-node counts kept for parity
+// checks.
+struct gwn_exact_tree {
+  int64_t nv = 0, nf = 0, nrec = 0, ntri = 0;
+  float bbox[6] = {0, 0, 0, 0, 0, 0};  // root box (query order)
+  gwn::DevBuf<uint8_t> rec;             // nrec WNRecord (wntree.h)
+  gwn::DevBuf<double> tri;              // ntri x 9
+  std::vector<int64_t> nodes;           // nrec x 5: #faces, #cap, apex, left, right
+};
+namespace gwn {
+// exact.hip: the hierarchy's records on the device, and its queries (O, W device)
+void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, const float* bbox, const double* O,
+                         int64_t no, double* W, hipStream_t st);
+}  // namespace gwn
+
 struct gwn_cloud {
   int64_t n = 0, ncells = 0, nrec = 0;
   int order = 2, nterms = 39;

@@ -523,16 +523,51 @@ __device__ __forceinline__ bool near_triangle(float maxp2, float l2, float near_
   return l2 < rr * rr;
 }
 
-// Exact term of a near triangle slot: sum (FP32) or, in the mixed form for a
-// triangle near the query, sum_near (FP64).
+// Near-triangle FP64 terms of the mixed form are deferred: a lane notes each
+// such triangle in its own column of an LDS list and evaluates the list, in
+// the order noted, after the walk.  In the wave-coherent walk this turns FP64
+// work that one lane of a wavefront needs at a record (which every lane would
+// sit through) into work all lanes do together at the end, and keeps the FP64
+// code and its registers out of the walk.  The terms and the order of their
+// FP64 additions are those of evaluating each where it is met.  A query that
+// meets more than kDefer near triangles gets W = NaN, which sends it to the
+// FP64 re-evaluation pass with the ambiguous queries.
+constexpr int kDefer = 16;
+constexpr int kTravThreads = 256;
+
+struct NearList {
+  uint32_t* lds;  // column threadIdx.x of a kDefer x kTravThreads array
+  uint32_t n;     // kDefer + 1 after an overflow
+};
+
+// Exact term of a near triangle slot: added to sum (FP32) or, in the mixed
+// form for a triangle near the query, noted for the FP64 sum.
 template <bool MIXED>
-__device__ __forceinline__ void exact_term(const float* __restrict__ p, float maxp2, float l2, float qx, float qy,
-                                           float qz, float near_d, float& sum, double& sum_near) {
-  if (MIXED && near_triangle(maxp2, l2, near_d))
+__device__ __forceinline__ void exact_term(const float* __restrict__ tri, uint32_t t, float maxp2, float l2, float qx,
+                                           float qy, float qz, float near_d, float& sum, NearList& L) {
+  if (MIXED && near_triangle(maxp2, l2, near_d)) {
+    if (L.n < (uint32_t)kDefer) L.lds[L.n * kTravThreads] = t;
+    L.n = L.n < (uint32_t)kDefer ? L.n + 1 : (uint32_t)kDefer + 1;
+  } else {
+    const float* p = tri + 9 * (size_t)t;
+    sum = __fadd_rn(sum, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+  }
+}
+
+// W of a walk: the FP32 sum, plus (mixed form) the noted near triangles'
+// FP64 solid angles summed in order; NaN after an overflow.
+template <bool MIXED>
+__device__ __forceinline__ double finish_walk(float sum, const NearList& L, const float* __restrict__ tri, float qx,
+                                              float qy, float qz) {
+  if (!MIXED) return (double)sum / (4.0 * PI_D);
+  if (L.n > (uint32_t)kDefer) return __longlong_as_double(0x7FF8000000000000ll);
+  double sum_near = 0.0;
+  for (uint32_t k = 0; k < L.n; ++k) {
+    const float* p = tri + 9 * (size_t)L.lds[k * kTravThreads];
     sum_near = __dadd_rn(sum_near,
                          tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-  else
-    sum = __fadd_rn(sum, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+  }
+  return ((double)sum + sum_near) / (4.0 * PI_D);
 }
 
 template <int ORDER, bool MIXED>
@@ -545,7 +580,8 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   float sum = 0.0f;
-  double sum_near = 0.0;  // mixed form: FP64 terms of near triangles
+  __shared__ uint32_t near_lds[MIXED ? kDefer * kTravThreads : 1];  // mixed form: noted near triangles
+  NearList L{near_lds + threadIdx.x, 0};
   uint32_t skip = 0;
   uint32_t i = 0;
   while (i < nslots) {
@@ -582,12 +618,12 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
           if (!leaf) skip = link;
         }
       }
-      if (exact) exact_term<MIXED>(tri + 9 * (size_t)(link & ~LINK_LEAF), a.w, l2, qx, qy, qz, near_d, sum, sum_near);
+      if (exact) exact_term<MIXED>(tri, link & ~LINK_LEAF, a.w, l2, qx, qy, qz, near_d, sum, L);
     }
     const bool step = leaf || __ballot(skip == i + 1) != 0;
     i = wave_uniform(step ? i + 1 : link);
   }
-  W[qi] = (MIXED ? (double)sum + sum_near : (double)sum) / (4.0 * PI_D);
+  W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
 }
 
 // Per-lane rope walk: each lane follows its own skip-pointer path and gathers
@@ -607,7 +643,8 @@ __global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict_
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   float sum = 0.0f;
-  double sum_near = 0.0;  // mixed form: FP64 terms of near triangles
+  __shared__ uint32_t near_lds[MIXED ? kDefer * kTravThreads : 1];  // mixed form: noted near triangles
+  NearList L{near_lds + threadIdx.x, 0};
   uint32_t i = 0;
   while (i < nslots) {
     const float4* r = slots + 6 * (size_t)i;
@@ -633,10 +670,10 @@ __global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict_
         continue;
       }
     }
-    if (leaf) exact_term<MIXED>(tri + 9 * (size_t)(link & ~LINK_LEAF), a.w, l2, qx, qy, qz, near_d, sum, sum_near);
+    if (leaf) exact_term<MIXED>(tri, link & ~LINK_LEAF, a.w, l2, qx, qy, qz, near_d, sum, L);
     ++i;
   }
-  W[qi] = (MIXED ? (double)sum + sum_near : (double)sum) / (4.0 * PI_D);
+  W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
 }
 
 // all-FP64 form: FP64 moments, far field, atan2 and sum
@@ -689,12 +726,14 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
   W[qi] = sum / (4.0 * PI_D);
 }
 
-// queries with |w - 0.5| < band are collected for the FP64 re-evaluation
+// queries with |w - 0.5| < band, and queries whose walk gave NaN (more near
+// triangles than the walk notes), are collected for the FP64 re-evaluation
 __global__ void k_collect_ambiguous(const double* __restrict__ W, int64_t nq, double band, uint32_t* __restrict__ list,
                                     uint32_t* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
-  if (fabs(W[i] - 0.5) < band) {
+  const double w = W[i];
+  if (isnan(w) || fabs(w - 0.5) < band) {
     const uint32_t p = atomicAdd(count, 1u);
     list[p] = (uint32_t)i;
   }
@@ -1091,7 +1130,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   }
 #undef GWN_LAUNCH_T
   check_launch("k_traverse");
-  if (mixed && s->mixed.ambiguous_band > 0.0f) {
+  if (mixed) {
     fill_device<uint32_t>(ambn, 1, 0u, st);
     hipLaunchKernelGGL(k_collect_ambiguous, dim3(g), dim3(256), 0, st, W, nq, (double)s->mixed.ambiguous_band, amb,
                        ambn);

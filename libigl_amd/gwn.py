@@ -28,7 +28,8 @@ from ._native import (PREC_FP32, PREC_FP64, PREC_MIXED, SIGN_DEFAULT, SIGN_FAST_
 
 DBL_MIN, DBL_MAX = 2.2250738585072014e-308, 1.7976931348623157e308
 
-__all__ = ["FastWindingNumberBVH", "PointCloudFWN", "SubtendedAngle", "fast_winding_number", "winding_number", "octree",
+__all__ = ["FastWindingNumberBVH", "PointCloudFWN", "SubtendedAngle", "WindingNumberAABB", "fast_winding_number",
+           "winding_number", "winding_number_bruteforce", "octree",
            "PREC_FP32", "PREC_MIXED", "PREC_FP64", "TREE_SAH4", "TREE_LBVH", "GWNError"]
 
 
@@ -58,6 +59,55 @@ def winding_number(V, F, O):
     W = np.empty(O.shape[0], np.float64)
     check(lib().gwn_exact_winding_number(V, V.shape[0], F, F.shape[0], O, O.shape[0], W))
     return W
+
+
+def winding_number_bruteforce(V, F, O):
+    """igl::winding_number(V, F, p) for each point (winding_number.cpp:62-89):
+    the exact FP64 solid angles summed over every face on the GPU (no
+    hierarchy, no rand()); equal to winding_number(V, F, O) up to rounding."""
+    V = _rows3(V, "V", np.float64)
+    F = _rows3(F, "F", np.int32)
+    O = _rows3(O, "O", np.float64)
+    W = np.empty(O.shape[0], np.float64)
+    if O.shape[0]:
+        check(lib().gwn_exact_winding_number_bruteforce(V, V.shape[0], F, F.shape[0], O, O.shape[0], W))
+    return W
+
+
+class WindingNumberAABB:
+    """igl::WindingNumberAABB<double,int> (WindingNumberAABB.h) with grow():
+    the hierarchy winding_number(V,F,O,W) builds (the build draws rand() as
+    the reference's does), kept on the GPU for repeated exact queries."""
+
+    def __init__(self, V, F):
+        V = _rows3(V, "V", np.float64)
+        F = _rows3(F, "F", np.int32)
+        h = nat.vp()
+        check(lib().gwn_exact_tree_build(V, V.shape[0], F, F.shape[0], C_byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            lib().gwn_exact_tree_free(self._h)
+            self._h = None
+
+    def winding_number(self, O):
+        O = _rows3(O, "O", np.float64)
+        W = np.empty(O.shape[0], np.float64)
+        if O.shape[0]:
+            check(lib().gwn_exact_tree_query(self._h, O, O.shape[0], W))
+        return W
+
+    def nodes(self):
+        """Per node in construction order: #faces, #cap triangles, fan apex
+        (into the de-duplicated vertices), left child, right child."""
+        import ctypes
+        n, t = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gwn_exact_tree_info(self._h, ctypes.byref(n), ctypes.byref(t)))
+        out = np.empty((n.value, 5), np.int64)
+        if n.value:
+            check(lib().gwn_exact_tree_nodes(self._h, out))
+        return out
 
 
 class FastWindingNumberBVH:

@@ -194,6 +194,20 @@ int oracle_knn(const double* P, int64_t np, const double* V, int64_t nv, int64_t
 int oracle_knn_bf(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
                   int64_t ncand, int32_t* I, int nthreads);
 
+/* ---- exact winding number through the reference's hierarchy
+ * (winding_number.cpp:39-53, WindingNumberAABB.h, WindingNumberTree.h) ----
+ * oracle_wntree_build calls rand() in the reference's order.
+ * oracle_wntree_nodes: 5 int64 per node in construction order: #faces,
+ * #cap triangles, fan apex (-1 without exterior edges), left, right (-1). */
+typedef struct oracle_wntree oracle_wntree;
+oracle_wntree* oracle_wntree_build(const double* V, int64_t nv, const int32_t* F, int64_t nf);
+void oracle_wntree_free(oracle_wntree* t);
+int64_t oracle_wntree_num_nodes(const oracle_wntree* t);
+void oracle_wntree_nodes(const oracle_wntree* t, int64_t* out);
+void oracle_wntree_query(const oracle_wntree* t, const double* O, int64_t no, double* W, int nthreads);
+/* the number of solid angles the query of p evaluates */
+int64_t oracle_wntree_count(const oracle_wntree* t, const double* p);
+
 #ifdef __cplusplus
 }
 #endif

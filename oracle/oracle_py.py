@@ -90,8 +90,56 @@ def lib():
         L.oracle_signed_distance.restype = C.c_int
         L.oracle_signed_distance.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, C.c_int,
                                              C.c_double, C.c_double, _f64p, _i32p, _f64p, C.c_int]
+        L.oracle_wntree_build.restype = C.c_void_p
+        L.oracle_wntree_build.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64]
+        L.oracle_wntree_free.argtypes = [C.c_void_p]
+        L.oracle_wntree_num_nodes.restype = C.c_int64
+        L.oracle_wntree_num_nodes.argtypes = [C.c_void_p]
+        L.oracle_wntree_nodes.argtypes = [C.c_void_p, _i64p]
+        L.oracle_wntree_query.argtypes = [C.c_void_p, _f64p, C.c_int64, _f64p, C.c_int]
+        L.oracle_wntree_count.restype = C.c_int64
+        L.oracle_wntree_count.argtypes = [C.c_void_p, _f64p]
         _lib = L
     return _lib
+
+
+def libc_srand(seed):
+    """Seeds the C library's rand(), which the hierarchy build draws its fan
+    apexes from (triangle_fan.cpp:31)."""
+    C.CDLL(None).srand(C.c_uint(seed))
+
+
+class WNTree:
+    """The reference's exact winding-number hierarchy (WindingNumberAABB),
+    restated; building it calls rand() as the reference does."""
+
+    def __init__(self, V, F):
+        self.V = _c(V, np.float64).reshape(-1, 3)
+        self.F = _c(F, np.int32).reshape(-1, 3)
+        self.h = lib().oracle_wntree_build(self.V, self.V.shape[0], self.F, self.F.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_wntree_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        """Per node in construction order: #faces, #cap triangles, fan apex,
+        left, right."""
+        n = lib().oracle_wntree_num_nodes(self.h)
+        out = np.empty((n, 5), np.int64)
+        lib().oracle_wntree_nodes(self.h, out)
+        return out
+
+    def query(self, O, nthreads=0):
+        O = _c(O, np.float64).reshape(-1, 3)
+        W = np.empty(O.shape[0], np.float64)
+        lib().oracle_wntree_query(self.h, O, O.shape[0], W, nthreads)
+        return W
+
+    def count(self, p):
+        """Solid angles the query of p evaluates."""
+        return lib().oracle_wntree_count(self.h, _c(p, np.float64).reshape(3))
 
 
 def ref_available(release=False) -> bool:

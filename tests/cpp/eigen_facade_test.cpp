@@ -247,6 +247,53 @@ int main() {
         std::fabs(Wt(1, 0) - Wx(1, 0)) > 1e-12)
       bad |= 65536;
   }
+  // a bvh built from float-rounded vertices (as igl::fast_winding_number(
+  // V.cast<float>(), F, 2, bvh) makes it): signed_distance_fast_winding_number
+  // takes its closest points from the caller's double V, not the bvh's floats
+  {
+    Eigen::MatrixXd Vp(4, 3);
+    const double vp[4][3] = {{0.1, 0.2, 0.3}, {1.1, 0.2, 0.3}, {0.1, 1.3, 0.3}, {0.1, 0.2, 1.7}};
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 3; ++j) Vp(i, j) = vp[i][j];
+    Eigen::Matrix<float, Eigen::Dynamic, Eigen::Dynamic> Vf(4, 3);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 3; ++j) Vf(i, j) = (float)vp[i][j];
+    igl::FastWindingNumberBVH bvh_f;
+    igl::fast_winding_number(Vf, F, 2, bvh_f);
+    Eigen::MatrixXd Pp(3, 3);
+    const double pp[3][3] = {{0.9, 0.9, 0.9}, {0.2, 0.3, 0.35}, {-0.4, 0.5, 0.6}};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Pp(i, j) = pp[i][j];
+    Eigen::VectorXd Sf, Sd;
+    Eigen::VectorXi Id;
+    Eigen::MatrixXd Cd, Nd;
+    igl::signed_distance_fast_winding_number(Pp, Vp, F, 0, bvh_f, Sf);
+    igl::signed_distance(Pp, Vp, F, igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER, Sd, Id, Cd, Nd);
+    for (int k = 0; k < 3; ++k)
+      if (Sf.rows() != 3 || std::fabs(std::fabs(Sf(k, 0)) - std::fabs(Sd(k, 0))) > 1e-15 ||
+          (Sf(k, 0) < 0) != (Sd(k, 0) < 0)) {
+        std::printf("float-built bvh, point %d: %.17g vs %.17g\n", k, Sf(k, 0), Sd(k, 0));
+        bad |= 1 << 20;
+      }
+  }
+  // igl::WindingNumberAABB as `hier` of signed_distance_winding_number, and on its own
+  {
+    igl::WindingNumberAABB<double, int> hier(V, F);
+    hier.grow();
+    Eigen::Matrix<double, 1, 3> qi;
+    qi(0, 0) = 0.1, qi(0, 1) = 0.1, qi(0, 2) = 0.2;
+    if (std::fabs(hier.winding_number(qi) - 1.0) > 1e-14) bad |= 1 << 21;
+    igl::FastWindingNumberBVH bvh;
+    igl::fast_winding_number(V, F, 2, bvh);
+    double s = 0, sqrd = 0;
+    int i = -1;
+    Eigen::MatrixXd c;
+    igl::signed_distance_winding_number(bvh, V, F, hier, qi, s, sqrd, i, c);
+    if (std::fabs(s + 1.0) > 1e-14 || std::fabs(std::sqrt(sqrd) - 0.1) > 1e-15) {
+      std::printf("signed_distance_winding_number with hier: s %.17g sqrd %.17g\n", s, sqrd);
+      bad |= 1 << 22;
+    }
+  }
   std::printf("eigen_facade_test: %s (flags %d)\n", bad ? "FAIL" : "OK", bad);
   return bad;
 }

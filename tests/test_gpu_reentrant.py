@@ -116,7 +116,7 @@ def test_exact_device_on_two_streams():
     V, F = D.noisy_sphere(5, 0.01, 35)
     rng = np.random.default_rng(35)
     Os = [rng.uniform(-1.5, 1.5, (300, 3)) for _ in range(8)]
-    serial = [gwn.winding_number(V, F, O) for O in Os]
+    serial = [gwn.winding_number_bruteforce(V, F, O) for O in Os]
     dV, dF = DeviceArray.from_numpy(np.ascontiguousarray(V)), DeviceArray.from_numpy(F.astype(np.int32))
     dO = [DeviceArray.from_numpy(O) for O in Os]
     dW = [DeviceArray((O.shape[0],), np.float64) for O in Os]
@@ -126,6 +126,28 @@ def test_exact_device_on_two_streams():
         nat.check(nat.lib().gwn_exact_winding_number_device(dV.data_ptr, V.shape[0], dF.data_ptr, F.shape[0],
                                                            dO[c].data_ptr, O.shape[0], dW[c].data_ptr,
                                                            streams[c % 2].ptr))
+    synchronize()
+    for c in range(len(Os)):
+        assert np.array_equal(dW[c].numpy(), serial[c]), c
+    for s in streams:
+        s.close()
+
+
+def test_exact_tree_device_on_two_streams():
+    """One hierarchy queried on two streams at once: each call sorts its
+    points in its own scratch."""
+    V, F = D.noisy_sphere(5, 0.01, 36)
+    rng = np.random.default_rng(36)
+    Os = [rng.uniform(-1.5, 1.5, (5000 + 700 * c, 3)) for c in range(6)]
+    T = gwn.WindingNumberAABB(V, F)
+    serial = [T.winding_number(O) for O in Os]
+    dO = [DeviceArray.from_numpy(O) for O in Os]
+    dW = [DeviceArray((O.shape[0],), np.float64) for O in Os]
+    streams = [Stream(), Stream()]
+    from libigl_amd import _native as nat
+    for c, O in enumerate(Os):
+        nat.check(nat.lib().gwn_exact_tree_query_device(T._h, dO[c].data_ptr, O.shape[0], dW[c].data_ptr,
+                                                        streams[c % 2].ptr))
     synchronize()
     for c in range(len(Os)):
         assert np.array_equal(dW[c].numpy(), serial[c]), c
