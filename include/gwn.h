@@ -89,7 +89,7 @@ const char* gwn_version(void);
 
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
- * "soup_traverse64", "exact", "exact_tree", "cloud_traverse", "radix_sort",
+ * "soup_traverse64", "soup_reeval64", "exact", "exact_tree", "cloud_traverse", "radix_sort",
  * "closest_point", "knn", "point_areas", "cloud_brute", "exact2d",
  * "curve_traverse") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises

@@ -570,11 +570,13 @@ __device__ __forceinline__ double finish_walk(float sum, const NearList& L, cons
   return ((double)sum + sum_near) / (4.0 * PI_D);
 }
 
+// Both walks are held to 8 wavefronts per SIMD (64 VGPRs): the walk loop fits
+// in that, and only the mixed form's end-of-walk FP64 code would take more
+// (94 VGPRs, 5 wavefronts); it keeps some of its constants in scratch instead.
 template <int ORDER, bool MIXED>
-__global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slots, uint32_t nslots,
-                                                  const float* __restrict__ tri, const float* __restrict__ Q,
-                                                  const uint32_t* __restrict__ perm, int64_t nq, float acc2,
-                                                  float near_d, double* __restrict__ W) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse(
+    const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
+    const uint32_t* __restrict__ perm, int64_t nq, float acc2, float near_d, double* __restrict__ W) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
@@ -634,10 +636,9 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ slo
 // for wavefront coherence), unsorted below a few thousand queries, and by
 // the serving engine for chunks under 64k queries.
 template <int ORDER, bool MIXED>
-__global__ __launch_bounds__(256) void k_traverse_lane(const float4* __restrict__ slots, uint32_t nslots,
-                                                       const float* __restrict__ tri, const float* __restrict__ Q,
-                                                       const uint32_t* __restrict__ perm, int64_t nq, float acc2,
-                                                       float near_d, double* __restrict__ W) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse_lane(
+    const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
+    const uint32_t* __restrict__ perm, int64_t nq, float acc2, float near_d, double* __restrict__ W) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
@@ -724,6 +725,168 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
     i = wave_uniform(step ? i + 1 : link);
   }
   W[qi] = sum / (4.0 * PI_D);
+}
+
+// Per-lane form of k_traverse64: each lane follows its own skip pointers and
+// gathers its own FP64 records; same records, terms and order as the
+// wave-coherent walk.  A wavefront of this walk costs its longest path, where
+// the wave-coherent walk costs the union of its 64 paths: the mixed form's
+// re-evaluation list (a few queries scattered over the mesh) and unsorted small
+// batches take this one.  dcount as in k_traverse64.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_traverse64_lane(const float4* __restrict__ slots,
+                                                         const double* __restrict__ s64, uint32_t nslots,
+                                                         const float* __restrict__ tri, const float* __restrict__ Q,
+                                                         const uint32_t* __restrict__ perm, int64_t nq,
+                                                         const uint32_t* __restrict__ dcount, double acc2,
+                                                         double* __restrict__ W) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= (dcount ? (int64_t)*dcount : nq)) return;
+  const int64_t qi = perm ? (int64_t)perm[t] : t;
+  const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+  double sum = 0.0;
+  uint32_t i = 0;
+  while (i < nslots) {
+    const double* r = s64 + 24 * (size_t)i;
+    const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+    const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const bool leaf = (link & LINK_LEAF) != 0;
+    const double x = qx - r0, y = qy - r1, z = qz - r2;
+    const double l2 = x * x + y * y + z * z;
+    if (!(l2 <= r3 * acc2)) {
+      const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
+          e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
+      const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
+      if (isfinite(o)) {
+        sum = __dadd_rn(sum, o);
+        i = leaf ? i + 1 : link;
+        continue;
+      }
+    }
+    if (leaf) {
+      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+      sum = __dadd_rn(sum, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+    }
+    ++i;
+  }
+  W[qi] = sum / (4.0 * PI_D);
+}
+
+// ---- FP64 re-evaluation of a few scattered queries: one wavefront per query ----
+// A sequential walk is a chain of dependent record fetches as long as the
+// query's path (thousands on a large mesh): a pass over a few hundred queries
+// runs a handful of wavefronts for milliseconds.  Here the 64 lanes of a
+// wavefront share one query's walk.  The wavefront keeps a ring of pending
+// (first slot, end) ranges of sibling slots in LDS; each round every lane takes
+// one range, evaluates its first record, and pushes the record's child range
+// [i+1, link) if the walk descends there and the rest of the range
+// (first = i+1 after an item, link after an internal slot) if not empty, so a
+// round needs no fetch beyond the record itself.  When the ring is full a lane
+// walks its range sequentially instead.  The records visited and the terms are
+// those of the sequential walk (each record's decision depends on the record
+// and the query only); their FP64 sum is taken per lane and then across lanes
+// in a fixed order, so the result is deterministic and differs from the
+// sequential sum only by rounding.
+constexpr int kReevalRing = 512;  // ranges per wavefront (4 KiB of LDS)
+
+// FP64 term of record i for query q: adds a far-field or exact term to part
+// and returns true, or returns false when the walk descends into the record.
+template <int ORDER>
+__device__ __forceinline__ bool record_term64(const double* __restrict__ s64, const float* __restrict__ tri,
+                                              uint32_t i, uint32_t link, double qx, double qy, double qz, double acc2,
+                                              double& part) {
+  const double* r = s64 + 24 * (size_t)i;
+  const bool leaf = (link & LINK_LEAF) != 0;
+  const double x = qx - r[0], y = qy - r[1], z = qz - r[2];
+  const double l2 = x * x + y * y + z * z;
+  if (!(l2 <= r[3] * acc2)) {
+    const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
+        e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
+    const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
+    if (isfinite(o)) {
+      part = __dadd_rn(part, o);
+      return true;
+    }
+  }
+  if (!leaf) return false;
+  const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
+  part = __dadd_rn(part, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
+  return true;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// list[k] for k < count (count = *dcount when given, else n): 4 queries per
+// 256-thread block, one per wavefront, grid-strided.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_reeval64(const float4* __restrict__ slots, const double* __restrict__ s64,
+                                                  uint32_t nslots, const float* __restrict__ tri,
+                                                  const float* __restrict__ Q, const uint32_t* __restrict__ list,
+                                                  int64_t n, const uint32_t* __restrict__ dcount, double acc2,
+                                                  double* __restrict__ W) {
+  __shared__ uint2 ring_all[4][kReevalRing];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint2* ring = ring_all[wv];
+  const int64_t count = dcount ? (int64_t)*dcount : n;
+  for (int64_t k = (int64_t)blockIdx.x * 4 + wv; k < count; k += (int64_t)gridDim.x * 4) {
+    const int64_t qi = list[k];
+    const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+    double part = 0.0;
+    if (lane == 0) ring[0] = make_uint2(0u, nslots);
+    uint32_t head = 0, tail = 1;  // wave-uniform ring positions
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    while (head != tail) {
+      const uint32_t take = min(tail - head, 64u);
+      const bool have = (uint32_t)lane < take;
+      uint2 it = make_uint2(0u, 0u);
+      if (have) it = ring[(head + lane) % kReevalRing];
+      head += take;
+      uint2 push0 = make_uint2(0u, 0u), push1 = make_uint2(0u, 0u);
+      uint32_t np = 0;
+      if (have) {
+        const uint32_t i = it.x;
+        const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+        const bool done = record_term64<ORDER>(s64, tri, i, link, qx, qy, qz, acc2, part);
+        const uint32_t next = (link & LINK_LEAF) ? i + 1 : link;
+        if (!done) push0 = make_uint2(i + 1, link), np = 1;  // child range
+        if (next < it.y) {
+          if (np) push1 = make_uint2(next, it.y);
+          else push0 = make_uint2(next, it.y);
+          ++np;
+        }
+      }
+      const uint64_t b0 = __ballot(np & 1u), b1 = __ballot(np >> 1);
+      const uint32_t below = lanes_below(b0) + 2u * lanes_below(b1);
+      const uint32_t room = kReevalRing - (tail - head);
+      const bool fits = below + np <= room;
+      if (fits) {
+        if (np > 0) ring[(tail + below) % kReevalRing] = push0;
+        if (np > 1) ring[(tail + below + 1) % kReevalRing] = push1;
+      } else {
+        // ring full: walk the ranges sequentially (rope order)
+        for (uint32_t m = 0; m < np; ++m) {
+          const uint2 rg = m ? push1 : push0;
+          uint32_t i = rg.x;
+          while (i < rg.y) {
+            const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+            const bool done = record_term64<ORDER>(s64, tri, i, link, qx, qy, qz, acc2, part);
+            i = (done && !(link & LINK_LEAF)) ? link : i + 1;
+          }
+        }
+      }
+      // lanes that fit are a prefix in lane order: their ranges are contiguous
+      tail += (uint32_t)__popcll(__ballot(fits && (np & 1u))) + 2u * (uint32_t)__popcll(__ballot(fits && (np >> 1)));
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for (int off = 32; off >= 1; off >>= 1) part = __dadd_rn(part, __shfl_xor(part, off, 64));
+    if (lane == 0) W[qi] = part / (4.0 * PI_D);
+  }
 }
 
 // queries with |w - 0.5| < band, and queries whose walk gave NaN (more near
@@ -1057,6 +1220,29 @@ void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, fl
 // same tree.  flags: WS_DEVICE_COUNT launches the mixed form's FP64
 // re-evaluation over every query with the count read on the device (no host
 // round trip); WS_NO_PROFILE creates no kernel-timing events.
+// FP64 walk over nq queries (perm: their order, or null): wave-coherent for
+// sorted batches, per-lane otherwise.
+static void launch_traverse64(const gwn_soup* s, bool wave, unsigned g, hipStream_t st, const float* Q,
+                              const uint32_t* perm, int64_t nq, const uint32_t* dcount, double a2, double* W) {
+  const uint32_t ns = (uint32_t)s->nslots;
+#define GWN_LAUNCH_64(O)                                                                                         \
+  do {                                                                                                           \
+    if (wave)                                                                                                    \
+      hipLaunchKernelGGL(k_traverse64<O>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, \
+                         perm, nq, dcount, a2, W);                                                               \
+    else                                                                                                         \
+      hipLaunchKernelGGL(k_traverse64_lane<O>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, \
+                         Q, perm, nq, dcount, a2, W);                                                            \
+  } while (0)
+  switch (s->order) {
+    case 0: GWN_LAUNCH_64(0); break;
+    case 1: GWN_LAUNCH_64(1); break;
+    default: GWN_LAUNCH_64(2); break;
+  }
+#undef GWN_LAUNCH_64
+  check_launch(wave ? "k_traverse64" : "k_traverse64_lane");
+}
+
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags) {
   if (nq <= 0) return;
@@ -1093,12 +1279,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   if (precision == GWN_PREC_FP64) {
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st, prof_on);
-    switch (s->order) {
-      case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-      case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-      default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-    }
-    check_launch("k_traverse64");
+    launch_traverse64(s, walk == WALK_WAVE, g, st, Q, perm, nq, nullptr, a2, W);
     return;
   }
   const bool mixed = precision == GWN_PREC_MIXED;
@@ -1137,31 +1318,30 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
     check_launch("k_collect_ambiguous");
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     int64_t na64 = 0;
-    unsigned ga = 0;
     const uint32_t* dcount = nullptr;
     if (flags & WS_DEVICE_COUNT) {
-      // sized for every query; threads beyond the device-side count exit
+      // bounded by every query; the kernel reads the count on the device
       na64 = nq;
-      ga = g;
       dcount = ambn;
     } else {
       // the count comes back to the host so that the FP64 pass is launched at
-      // its size (a launch sized for every query, reading the count on the
-      // device, measured slower under concurrent serving load)
+      // its size, or not at all
       uint32_t na = 0;
       GWN_HIP(hipMemcpyAsync(&na, ambn, 4, hipMemcpyDeviceToHost, st));
       GWN_HIP(hipStreamSynchronize(st));
       na64 = na;
-      ga = div_up(na, 256);
     }
     if (na64 > 0) {
-      ProfileScope prof("soup_traverse64", st, prof_on);
+      // one wavefront per query, four per block; with the count on the
+      // device the grid is capped and strides over the list
+      const unsigned gr = dcount ? (unsigned)std::min<int64_t>(div_up(na64, 4), 2048) : (unsigned)div_up(na64, 4);
+      ProfileScope prof("soup_reeval64", st, prof_on);
       switch (s->order) {
-        case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
-        case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
-        default: hipLaunchKernelGGL(k_traverse64<2>, dim3(ga), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
+        case 0: hipLaunchKernelGGL(k_reeval64<0>, dim3(gr), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
+        case 1: hipLaunchKernelGGL(k_reeval64<1>, dim3(gr), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
+        default: hipLaunchKernelGGL(k_reeval64<2>, dim3(gr), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, amb, na64, dcount, a2, W); break;
       }
-      check_launch("k_traverse64 (ambiguous)");
+      check_launch("k_reeval64");
     }
   }
 }

@@ -94,6 +94,10 @@ void oracle_soup_query(const oracle_soup* s, const double* Q, int64_t nq, float 
  * calls), accepted far-field child slots and exact triangles. */
 void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale,
                        int64_t* visits, int64_t* far_accepted, int64_t* exact_tris);
+/* The same, plus the exact triangles within near_d of the query's slot sphere
+ * (|q - P| < sqrt(maxP2) + near_d): the mixed form's FP64 near terms. */
+void oracle_soup_count_near(const oracle_soup* s, const float* q, float accuracy_scale, float near_d,
+                            int64_t* visits, int64_t* far_accepted, int64_t* exact_tris, int64_t* near_tris);
 
 /* BVH::init<BOX_AREA> (FastWindingNumberForSoups.h:4579-5944) over nboxes
  * boxes of 6 floats (x0 x1 y0 y1 z0 z1; 2D boxes carry z0 = z1 = 0 with

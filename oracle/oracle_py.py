@@ -59,6 +59,7 @@ def lib():
         L.oracle_soup_query.argtypes = [C.c_void_p, _f64p, C.c_int64, C.c_float, _f64p, C.c_int]
         L.oracle_soup_count.argtypes = [C.c_void_p, _f32p, C.c_float, C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.oracle_soup_count_near.argtypes = [C.c_void_p, _f32p, C.c_float, C.c_float] + [C.POINTER(C.c_int64)] * 4
         for sfx, fp in (("f64", _f64p), ("f32", _f32p)):
             f = getattr(L, f"oracle_octree_{sfx}")
             f.restype = C.c_int
@@ -274,6 +275,14 @@ class Soup:
         v, a, e = C.c_int64(), C.c_int64(), C.c_int64()
         lib().oracle_soup_count(self.h, _c(q, np.float32), accuracy_scale, C.byref(v), C.byref(a), C.byref(e))
         return v.value, a.value, e.value
+
+    def count_near(self, q, accuracy_scale, near_dist):
+        """Exact triangles of the query's walk within near_dist of their
+        slot's bounding sphere (the mixed form's FP64 near terms)."""
+        v, a, e, n = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        lib().oracle_soup_count_near(self.h, _c(q, np.float32), accuracy_scale, near_dist, C.byref(v),
+                                     C.byref(a), C.byref(e), C.byref(n))
+        return n.value
 
 
 class RefSoup:

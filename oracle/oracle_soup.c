@@ -10,7 +10,8 @@
  *
  * The restatement follows the reference's operation order so that, built with
  * -ffp-contract=off, it reproduces the compiled reference bit for bit (node
- * arrays, BoxData and per-query Omega); tests/test_oracle_ref.py checks this.
+ * arrays, BoxData and per-query Omega); tests/test_oracle.py checks this
+ * (test_soup_oracle_bit_exact_vs_reference and the two tests after it).
  * The 4-wide SSE lanes of the reference (v4uf) are restated as scalar loops
  * over the 4 child slots: each SSE lane operation is an independent IEEE
  * single-precision operation, so the result is identical.
@@ -860,6 +861,8 @@ typedef struct {
   float q[3];
   float acc2;
   int64_t visits, far_accepted, exact;
+  float near_d;  /* < 0: near triangles not counted */
+  int64_t near;  /* exact triangles with |q - P| < sqrt(maxP2) + near_d */
 } sa_ctx;
 
 /* SolidAngleFunctors::pre (7176-7258): far-field Taylor terms of the child
@@ -943,6 +946,12 @@ static void sa_traverse(sa_ctx* c, uint32_t nodei, float* data_for_parent) {
       } else {
         const int32_t* tp = c->s->F + 3 * (int64_t)ni;
         c->exact++;
+        if (c->near_d >= 0.0f) {
+          const boxdata_t* d = &c->s->data[nodei];
+          const float x = c->q[0] - d->P[0][sidx], y = c->q[1] - d->P[1][sidx], z = c->q[2] - d->P[2][sidx];
+          const float rr = sqrtf(d->maxP2[sidx]) + c->near_d;
+          if (x * x + y * y + z * z < rr * rr) c->near++;
+        }
         local[sidx] = solid_angle_tri(c->s->U + 3 * (int64_t)tp[0], c->s->U + 3 * (int64_t)tp[1],
                                       c->s->U + 3 * (int64_t)tp[2], c->q);
       }
@@ -963,15 +972,16 @@ float oracle_soup_solid_angle(const oracle_soup* s, const float* q, float accura
   c.q[1] = q[1];
   c.q[2] = q[2];
   c.acc2 = accuracy_scale * accuracy_scale;
-  c.visits = c.far_accepted = c.exact = 0;
+  c.visits = c.far_accepted = c.exact = c.near = 0;
+  c.near_d = -1.0f;
   float sum;
   sa_traverse(&c, 0, &sum);
   return sum;
 }
 
-void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale, int64_t* visits,
-                       int64_t* far_accepted, int64_t* exact_tris) {
-  *visits = *far_accepted = *exact_tris = 0;
+void oracle_soup_count_near(const oracle_soup* s, const float* q, float accuracy_scale, float near_d,
+                            int64_t* visits, int64_t* far_accepted, int64_t* exact_tris, int64_t* near_tris) {
+  *visits = *far_accepted = *exact_tris = *near_tris = 0;
   if (s->nnodes == 0) return;
   sa_ctx c;
   c.s = s;
@@ -979,12 +989,20 @@ void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scal
   c.q[1] = q[1];
   c.q[2] = q[2];
   c.acc2 = accuracy_scale * accuracy_scale;
-  c.visits = c.far_accepted = c.exact = 0;
+  c.visits = c.far_accepted = c.exact = c.near = 0;
+  c.near_d = near_d;
   float sum;
   sa_traverse(&c, 0, &sum);
   *visits = c.visits;
   *far_accepted = c.far_accepted;
   *exact_tris = c.exact;
+  *near_tris = c.near;
+}
+
+void oracle_soup_count(const oracle_soup* s, const float* q, float accuracy_scale, int64_t* visits,
+                       int64_t* far_accepted, int64_t* exact_tris) {
+  int64_t near;
+  oracle_soup_count_near(s, q, accuracy_scale, -1.0f, visits, far_accepted, exact_tris, &near);
 }
 
 typedef struct {

@@ -122,6 +122,31 @@ def test_soup_oracle_closed_mesh_and_large_beta_is_exact():
     assert np.abs(s.query(Q, 2.0) - w_exact).max() < 1e-2
 
 
+def test_soup_oracle_near_count():
+    """The near-triangle count of the instrumented walk: every exact triangle
+    at a huge distance, none at a point far outside, monotone in the distance,
+    and by brute force (the sphere test over the walk's exact triangles equals
+    the sphere test over all triangles once accuracy_scale makes every
+    triangle exact)."""
+    V, F = D.bumpy_sphere(3, 8, 12)
+    s = O.Soup(V, F)
+    rng = np.random.default_rng(5)
+    Q = np.concatenate([rng.uniform(-1.2, 1.2, (60, 3)), D.near_surface_queries(V, F, 60, 1e-3, rng)])
+    T = V[F].astype(np.float32)
+    P = T.mean(1)
+    half = np.maximum(np.abs(T.min(1) - P), np.abs(T.max(1) - P))
+    r = np.sqrt((half.astype(np.float64) ** 2).sum(1))
+    for q in Q.astype(np.float32):
+        _, _, exact = s.count(q, 2.0)
+        assert s.count_near(q, 2.0, 1e30) == exact
+        n0, n1 = s.count_near(q, 2.0, 0.0), s.count_near(q, 2.0, 0.05)
+        assert n0 <= n1 <= exact
+        d = np.sqrt(((q.astype(np.float64) - P) ** 2).sum(1))
+        bf = int((d < r + 0.05).sum())
+        assert abs(s.count_near(q, 1e6, 0.05) - bf) <= 1  # one float rounding at the sphere's edge at most
+    assert s.count_near(np.array([50.0, 0, 0], np.float32), 2.0, 0.05) == 0
+
+
 # ---------------- cloud path ----------------
 def test_cloud_one_point_golden():
     # tests/include/igl/fast_winding_number.cpp:17-53
