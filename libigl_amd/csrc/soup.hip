@@ -503,10 +503,12 @@ struct D4 {
 // after its subtree), a near internal child is entered (next record), an item
 // is evaluated exactly when near.  Each lane keeps `skip`, the next record it
 // needs; the wavefront walks one shared index i and a lane takes part in a
-// record exactly when i == skip.  The wave steps to i+1 when some lane needs
-// it (a near child, or an item) and otherwise to link(i): every lane that is
-// not taking part needs a record at or beyond link(i), because the subtrees
-// of the rope are nested intervals.  So each lane visits exactly the records
+// record exactly when i == skip.  (k_traverse tests skip <= i: a lane that
+// enters a child or evaluates an item leaves skip where it was, and only a
+// lane that takes a far field moves it past the subtree.)  The wave steps to
+// i+1 when some lane needs it (a near child, or an item) and otherwise to
+// link(i): every lane that is not taking part needs a record at or beyond
+// link(i), because the subtrees of the rope are nested intervals.  So each lane visits exactly the records
 // of its own walk, in the same order, and sums the same terms in the same
 // order as an independent per-lane walk; but every record (and triangle) a
 // wave reads is the same for all its lanes, so it is fetched once per wave
@@ -573,12 +575,22 @@ __device__ __forceinline__ double finish_walk(float sum, const NearList& L, cons
 // Both walks are held to 8 wavefronts per SIMD (64 VGPRs): the walk loop fits
 // in that, and only the mixed form's end-of-walk FP64 code would take more
 // (94 VGPRs, 5 wavefronts); it keeps some of its constants in scratch instead.
+// A kernel argument copied into a VGPR: a VALU instruction reads at most one
+// SGPR, so the product of the slot radius (an SGPR in the wave-coherent walk)
+// and accuracy_scale^2 would otherwise need a move at every step.
+__device__ __forceinline__ float in_vgpr(float x) {
+  float v;
+  asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
+
 template <int ORDER, bool MIXED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse(
     const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
-    const uint32_t* __restrict__ perm, int64_t nq, float acc2, float near_d, double* __restrict__ W) {
+    const uint32_t* __restrict__ perm, int64_t nq, float acc2_arg, float near_d, double* __restrict__ W) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nq) return;
+  const float acc2 = in_vgpr(acc2_arg);
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
   float sum = 0.0f;
@@ -605,24 +617,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     else asm volatile("" ::"s"(a.x), "s"(b.w));
     const uint32_t link = __float_as_uint(b.w);
     const bool leaf = (link & LINK_LEAF) != 0;
-    if (skip == i) {
+    // the record after this one's subtree, picked in scalar code so that a
+    // lane taking the far field only copies it
+    const uint32_t past = leaf ? i + 1 : link;
+    // skip >= i holds for every lane (nested intervals), so skip <= i means
+    // skip == i; a lane that descends or evaluates an item leaves skip at i,
+    // which is <= i + 1, the record it needs next
+    if (skip <= i) {
       const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
       // |q-P|^2 rounded as the reference rounds it (no fused multiply-add),
       // so every near/far decision is the reference's
       const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
       bool exact = leaf;
-      skip = i + 1;
       if (!(l2 <= __fmul_rn(a.w, acc2))) {
         const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
         if (isfinite(o)) {  // otherwise the reference descends / evaluates the item
           sum = __fadd_rn(sum, o);
           exact = false;
-          if (!leaf) skip = link;
+          skip = past;
         }
       }
       if (exact) exact_term<MIXED>(tri, link & ~LINK_LEAF, a.w, l2, qx, qy, qz, near_d, sum, L);
     }
-    const bool step = leaf || __ballot(skip == i + 1) != 0;
+    const bool step = leaf || __ballot(skip <= i + 1) != 0;
     i = wave_uniform(step ? i + 1 : link);
   }
   W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
