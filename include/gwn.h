@@ -324,7 +324,7 @@ int gwn_cloud_build_estimated_areas(const double* P, const double* N, int64_t n,
  * from the winding number.  Sign types carry the reference's enum values.
  */
 typedef enum {
-  GWN_SIGN_PSEUDONORMAL = 0,        /* not supported here: GWN_ERR_INVALID */
+  GWN_SIGN_PSEUDONORMAL = 0,        /* s = sign((q - c) . n), n the angle-weighted pseudonormal at c */
   GWN_SIGN_WINDING_NUMBER = 1,      /* s = 1 - 2 w, w the exact winding number */
   GWN_SIGN_DEFAULT = 2,             /* as WINDING_NUMBER */
   GWN_SIGN_UNSIGNED = 3,            /* s = 1 */
@@ -332,13 +332,25 @@ typedef enum {
 } gwn_sign_type;
 
 /* igl::signed_distance(P,V,F,sign_type,lower_bound,upper_bound,S,I,C,N) over
- * the soup's mesh (the FWN sign uses the soup's tree and expansion order; the
- * reference builds an order-2 tree).  P np x 3; S np (s * distance), I np
- * (closest face), C np x 3 (closest point).  A point whose squared distance
- * lies outside [low^2, up^2) derived from the bounds as signed_distance.cpp:79-84
- * does gets S = NaN, I = nf + 1, C = 0; the reference's unbounded overload
- * (signed_distance.h:81-89) is lower = DBL_MIN, upper = DBL_MAX.  An empty
- * mesh gives S = NaN, I = -1, C = NaN.  N (pseudonormals) is not produced. */
+ * the soup's mesh (signed_distance.cpp:28-140; the FWN sign uses the soup's
+ * tree and expansion order; the reference builds an order-2 tree).  P np x 3;
+ * S np (s * distance), I np (closest face), C np x 3 (closest point).  A point
+ * whose squared distance lies outside [low^2, up^2) derived from the bounds as
+ * signed_distance.cpp:79-84 gets S = NaN, I = nf + 1, C = 0; the reference's
+ * unbounded overload (signed_distance.h:81-89) is lower = DBL_MIN,
+ * upper = DBL_MAX.  An empty mesh gives S = NaN, I = -1, C = NaN.
+ * WINDING_NUMBER / DEFAULT take w through the exact hierarchy built for the
+ * call, as the reference's does (and synchronise the stream).  PSEUDONORMAL
+ * builds the face, angle-weighted vertex and edge normals of the mesh on the
+ * first such call (pseudonormal_test.cpp:24-141 picks among them); N np x 3,
+ * when not NULL, receives the normal used (0 for points out of bounds, whose
+ * rows the reference leaves unset); for other sign types N is not written. */
+int gwn_signed_distance_with_normals(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                                     double upper_bound, double* S, int32_t* I, double* C, double* N);
+int gwn_signed_distance_with_normals_device(gwn_soup* s, const double* P, int64_t np, int sign_type,
+                                            double lower_bound, double upper_bound, double* S, int32_t* I, double* C,
+                                            double* N, void* stream);
+/* The same without N. */
 int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
                         double upper_bound, double* S, int32_t* I, double* C);
 int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,

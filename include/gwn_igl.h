@@ -401,15 +401,19 @@ void cloud_eval(const MP& P, const MN& N, const MA& A, const std::vector<std::ve
     WN(i, 0) = static_cast<typename std::decay<decltype(WN(0, 0))>::type>(w[i]);
 }
 
-// signed distance over a built soup; S, I, C resized as the reference does
-// (N is only written by the pseudonormal type, which is not provided here)
-template <typename MP, typename MS, typename MI, typename MC>
-void signed_distance_bvh(gwn_soup* h, const MP& P, int type, double lower, double upper, MS& S, MI& I, MC& C) {
+// signed distance over a built soup; S, I, C resized as the reference does,
+// and N (np x 3) for the pseudonormal type only (signed_distance.cpp:68-76)
+template <typename MP, typename MS, typename MI, typename MC, typename MN>
+void signed_distance_bvh(gwn_soup* h, const MP& P, int type, double lower, double upper, MS& S, MI& I, MC& C,
+                         MN* N) {
   const auto p = rows3<double>(P, "P");
   const int64_t n = P.rows();
-  std::vector<double> s((size_t)n), c(3 * (size_t)n);
+  const bool want_n = N != nullptr && type == GWN_SIGN_PSEUDONORMAL;
+  std::vector<double> s((size_t)n), c(3 * (size_t)n), nn(want_n ? 3 * (size_t)n : 0);
   std::vector<int32_t> i((size_t)n);
-  if (n > 0) check(gwn_signed_distance(h, p.data(), n, type, lower, upper, s.data(), i.data(), c.data()));
+  if (n > 0)
+    check(gwn_signed_distance_with_normals(h, p.data(), n, type, lower, upper, s.data(), i.data(), c.data(),
+                                           want_n ? nn.data() : nullptr));
   S.resize(n, 1);
   I.resize(n, 1);
   C.resize(n, 3);
@@ -418,15 +422,26 @@ void signed_distance_bvh(gwn_soup* h, const MP& P, int type, double lower, doubl
     I(k, 0) = static_cast<typename std::decay<decltype(I(0, 0))>::type>(i[k]);
     for (int j = 0; j < 3; ++j) C(k, j) = static_cast<typename std::decay<decltype(C(0, 0))>::type>(c[3 * k + j]);
   }
+  if (want_n) {
+    N->resize(n, 3);
+    for (int64_t k = 0; k < n; ++k)
+      for (int j = 0; j < 3; ++j)
+        (*N)(k, j) = static_cast<typename std::decay<decltype((*N)(0, 0))>::type>(nn[3 * k + j]);
+  }
+}
+
+template <typename MP, typename MV, typename MF, typename MS, typename MI, typename MC, typename MN>
+void signed_distance(const MP& P, const MV& V, const MF& F, int type, double lower, double upper, MS& S, MI& I,
+                     MC& C, MN* N) {
+  FastWindingNumberBVH bvh;  // signed_distance.cpp:65-67 builds an order-2 tree for the FWN sign
+  build(V, F, 2, bvh);
+  signed_distance_bvh(bvh.handle, P, type, lower, upper, S, I, C, N);
 }
 
 template <typename MP, typename MV, typename MF, typename MS, typename MI, typename MC>
 void signed_distance(const MP& P, const MV& V, const MF& F, int type, double lower, double upper, MS& S, MI& I,
                      MC& C) {
-  if (type == GWN_SIGN_PSEUDONORMAL) throw std::runtime_error("gwn: pseudonormal signed distance is not provided");
-  FastWindingNumberBVH bvh;  // signed_distance.cpp:65-67 builds an order-2 tree for the FWN sign
-  build(V, F, 2, bvh);
-  signed_distance_bvh(bvh.handle, P, type, lower, upper, S, I, C);
+  signed_distance(P, V, F, type, lower, upper, S, I, C, static_cast<MC*>(nullptr));
 }
 
 // The points igl::knn can return over an octree (knn.cpp:101-123): the first
@@ -727,9 +742,8 @@ void signed_distance(const Eigen::MatrixBase<DerivedP>& P, const Eigen::MatrixBa
                      const typename DerivedV::Scalar lower_bound, const typename DerivedV::Scalar upper_bound,
                      Eigen::PlainObjectBase<DerivedS>& S, Eigen::PlainObjectBase<DerivedI>& I,
                      Eigen::PlainObjectBase<DerivedC>& C, Eigen::PlainObjectBase<DerivedN>& N) {
-  (void)N;
   gwn_detail::signed_distance(P.derived(), V.derived(), F.derived(), (int)sign_type, (double)lower_bound,
-                              (double)upper_bound, S.derived(), I.derived(), C.derived());
+                              (double)upper_bound, S.derived(), I.derived(), C.derived(), &N.derived());
 }
 
 template <typename DerivedP, typename DerivedV, typename DerivedF, typename DerivedS, typename DerivedI,

@@ -108,6 +108,10 @@ SIGNATURES = {
     "gwn_cloud_build_estimated_areas": (C.c_int, [f64p, f64p, C.c_int64, C.c_int, vp, C.POINTER(vp)]),
     "gwn_signed_distance": (C.c_int, [vp, f64p, C.c_int64, C.c_int, C.c_double, C.c_double, f64p, i32p, f64p]),
     "gwn_signed_distance_device": (C.c_int, [vp, vp, C.c_int64, C.c_int, C.c_double, C.c_double, vp, vp, vp, vp]),
+    "gwn_signed_distance_with_normals": (C.c_int, [vp, f64p, C.c_int64, C.c_int, C.c_double, C.c_double, f64p, i32p,
+                                                   f64p, vp]),
+    "gwn_signed_distance_with_normals_device": (C.c_int, [vp, vp, C.c_int64, C.c_int, C.c_double, C.c_double, vp, vp,
+                                                          vp, vp, vp]),
     "gwn_exact_winding_number_2d": (C.c_int, [f64p, C.c_int64, i32p, C.c_int64, f64p, C.c_int64, f64p]),
     "gwn_read_triangle_mesh": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
                                          C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.c_int64)]),

@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libgwn.so")
 
 SOURCES = ["api.cpp", "sah4_build.cpp", "soup.hip", "radix.hip", "exact.hip", "lbvh.hip", "cloud.hip",
            "distance.hip", "knn.hip", "antipodal.hip", "curve.hip", "meshio.cpp", "serving.cpp", "profile.cpp",
-           "scratch.cpp", "wntree.cpp"]
+           "scratch.cpp", "wntree.cpp", "normals.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=off",

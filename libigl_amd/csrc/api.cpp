@@ -152,7 +152,7 @@ void antipodal_query_device(const gwn_antipodal* a, const double* O, int64_t no,
 
 // igl::WindingNumberAABB<double,int> hier(V,F); hier.grow()
 // (winding_number.cpp:43-47), built on the host (wntree.cpp) and uploaded
-static void exact_tree_build(gwn_exact_tree* t, const double* V, int64_t nv, const int32_t* F, int64_t nf) {
+void exact_tree_build(gwn_exact_tree* t, const double* V, int64_t nv, const int32_t* F, int64_t nf) {
   t->nv = nv;
   t->nf = nf;
   WNTreeHost T;
@@ -705,36 +705,51 @@ static void sd_check(gwn_soup* s, int64_t np, int sign_type, const double* P, do
   GWN_CHECK(np >= 0 && np < (int64_t)0x7FFFFFFF, GWN_ERR_INVALID, "bad point count");
   GWN_CHECK(np == 0 || (P && S && I && C), GWN_ERR_INVALID, "null input/output");
   GWN_CHECK(sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT || sign_type == GWN_SIGN_UNSIGNED ||
-                sign_type == GWN_SIGN_FAST_WINDING_NUMBER,
-            GWN_ERR_INVALID, "sign type not supported (pseudonormal signs are not implemented)");
+                sign_type == GWN_SIGN_FAST_WINDING_NUMBER || sign_type == GWN_SIGN_PSEUDONORMAL,
+            GWN_ERR_INVALID, "unknown sign type");
+}
+
+int gwn_signed_distance_with_normals_device(gwn_soup* s, const double* P, int64_t np, int sign_type,
+                                            double lower_bound, double upper_bound, double* S, int32_t* I, double* C,
+                                            double* N, void* stream) {
+  return guard([&] {
+    sd_check(s, np, sign_type, P, S, I, C);
+    signed_distance_device(s, P, np, sign_type, lower_bound, upper_bound, S, I, C, N, (hipStream_t)stream);
+  });
 }
 
 int gwn_signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
                                double upper_bound, double* S, int32_t* I, double* C, void* stream) {
-  return guard([&] {
-    sd_check(s, np, sign_type, P, S, I, C);
-    signed_distance_device(s, P, np, sign_type, lower_bound, upper_bound, S, I, C, (hipStream_t)stream);
-  });
+  return gwn_signed_distance_with_normals_device(s, P, np, sign_type, lower_bound, upper_bound, S, I, C, nullptr,
+                                                 stream);
 }
 
-int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
-                        double upper_bound, double* S, int32_t* I, double* C) {
+int gwn_signed_distance_with_normals(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                                     double upper_bound, double* S, int32_t* I, double* C, double* N) {
   return guard([&] {
     sd_check(s, np, sign_type, P, S, I, C);
     if (np == 0) return;
     hipStream_t st = nullptr;
     const size_t n = (size_t)np;
-    Scratch io(scratch_bytes({24 * n, 8 * n, 4 * n, 24 * n}), st);
+    const bool want_n = N != nullptr && sign_type == GWN_SIGN_PSEUDONORMAL;
+    Scratch io(scratch_bytes({24 * n, 8 * n, 4 * n, 24 * n, want_n ? 24 * n : 8}), st);
     const double* dP = stage_in(io, P, 3 * n, st);
     double* dS = io.take_n<double>(n);
     int32_t* dI = io.take_n<int32_t>(n);
     double* dC = io.take_n<double>(3 * n);
-    signed_distance_device(s, dP, np, sign_type, lower_bound, upper_bound, dS, dI, dC, st);
+    double* dN = want_n ? io.take_n<double>(3 * n) : nullptr;
+    signed_distance_device(s, dP, np, sign_type, lower_bound, upper_bound, dS, dI, dC, dN, st);
     stage_out(S, dS, n, st);
     stage_out(I, dI, n, st);
     stage_out(C, dC, 3 * n, st);
+    if (want_n) stage_out(N, dN, 3 * n, st);
     GWN_HIP(hipStreamSynchronize(st));
   });
+}
+
+int gwn_signed_distance(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower_bound,
+                        double upper_bound, double* S, int32_t* I, double* C) {
+  return gwn_signed_distance_with_normals(s, P, np, sign_type, lower_bound, upper_bound, S, I, C, nullptr);
 }
 
 // ---------------- antipodal method ----------------

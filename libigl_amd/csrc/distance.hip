@@ -207,12 +207,113 @@ __global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox
   for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = bc[k];
 }
 
+// pseudonormal_test.cpp:24-141: the normal at the closest point c of face f
+// -- the face's, an edge's or a vertex's, by where c lies on the face.
+// Faces of double area > 1e-4 (Kahan's Heron formula over the sorted edge
+// lengths, doublearea.cpp:144-197) classify c by barycentric coordinates
+// (barycentric_coordinates.cpp:88-100) <= 1e-12; smaller faces by distances
+// to the corners and to the edges (project_to_line_segment.cpp) < 1e-12.
+struct Pseudonormals {
+  const double* V;   // nv x 3
+  const int32_t* F;  // nf x 3
+  const double* FN;  // nf x 3
+  const double* VN;  // nv x 3
+  const double* EN;  // nf x 3 x 3: edge opposite corner c of face f
+};
+
+__device__ __forceinline__ double len3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+__device__ void pseudonormal_at(const Pseudonormals& pn, int64_t f, const double* c, double* n) {
+  constexpr double kMinDoubleArea = 1e-4, kEps = 1e-12;
+  const int32_t* t = pn.F + 3 * f;
+  const double* A = pn.V + 3 * (int64_t)t[0];
+  const double* B = pn.V + 3 * (int64_t)t[1];
+  const double* C = pn.V + 3 * (int64_t)t[2];
+  // edge lengths |B-C|, |C-A|, |A-B| sorted descending
+  double l0 = len3(B[0] - C[0], B[1] - C[1], B[2] - C[2]);
+  double l1 = len3(C[0] - A[0], C[1] - A[1], C[2] - A[2]);
+  double l2 = len3(A[0] - B[0], A[1] - B[1], A[2] - B[2]);
+  double tmp;
+  if (l0 < l1) tmp = l0, l0 = l1, l1 = tmp;
+  if (l1 < l2) tmp = l1, l1 = l2, l2 = tmp;
+  if (l0 < l1) tmp = l0, l0 = l1, l1 = tmp;
+  const double arg = (l0 + (l1 + l2)) * (l2 - (l0 - l1)) * (l2 + (l0 - l1)) * (l0 + (l1 - l2));
+  const double area = 2.0 * 0.25 * sqrt(arg);
+  const double* src = pn.FN + 3 * f;
+  if (area > kMinDoubleArea) {
+    const double v0[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    const double v1[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    const double v2[3] = {c[0] - A[0], c[1] - A[1], c[2] - A[2]};
+    const double d00 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+    const double d01 = v0[0] * v1[0] + v0[1] * v1[1] + v0[2] * v1[2];
+    const double d11 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+    const double d20 = v2[0] * v0[0] + v2[1] * v0[1] + v2[2] * v0[2];
+    const double d21 = v2[0] * v1[0] + v2[1] * v1[1] + v2[2] * v1[2];
+    const double denom = d00 * d11 - d01 * d01;
+    double b[3];
+    b[1] = (d11 * d20 - d01 * d21) / denom;
+    b[2] = (d00 * d21 - d01 * d20) / denom;
+    b[0] = 1.0 - (b[1] + b[2]);
+    const int type = (b[0] <= kEps) + (b[1] <= kEps) + (b[2] <= kEps);
+    if (type == 2) {  // a vertex
+      for (int x = 0; x < 3; ++x)
+        if (b[x] > kEps) {
+          src = pn.VN + 3 * (int64_t)t[x];
+          break;
+        }
+    } else if (type == 1) {  // an edge
+      for (int x = 0; x < 3; ++x)
+        if (b[x] <= kEps) {
+          src = pn.EN + 9 * f + 3 * x;
+          break;
+        }
+    }
+  } else {
+    bool found = false;
+    for (int v = 0; v < 3 && !found; ++v) {
+      const double* P = pn.V + 3 * (int64_t)t[v];
+      if (len3(c[0] - P[0], c[1] - P[1], c[2] - P[2]) < kEps) {
+        found = true;
+        src = pn.VN + 3 * (int64_t)t[v];
+      }
+    }
+    for (int e = 0; e < 3 && !found; ++e) {
+      const double* S = pn.V + 3 * (int64_t)t[(e + 1) % 3];
+      const double* D = pn.V + 3 * (int64_t)t[(e + 2) % 3];
+      const double dms[3] = {D[0] - S[0], D[1] - S[1], D[2] - S[2]};
+      const double vl = dms[0] * dms[0] + dms[1] * dms[1] + dms[2] * dms[2];
+      const double smp[3] = {S[0] - c[0], S[1] - c[1], S[2] - c[2]};
+      const double tt = -(dms[0] * smp[0] + dms[1] * smp[1] + dms[2] * smp[2]) / vl;
+      double sq;
+      if (tt < 0.0) {
+        sq = (c[0] - S[0]) * (c[0] - S[0]) + (c[1] - S[1]) * (c[1] - S[1]) + (c[2] - S[2]) * (c[2] - S[2]);
+      } else if (tt > 1.0) {
+        sq = (c[0] - D[0]) * (c[0] - D[0]) + (c[1] - D[1]) * (c[1] - D[1]) + (c[2] - D[2]) * (c[2] - D[2]);
+      } else {
+        const double pp[3] = {(1.0 - tt) * S[0] + tt * D[0], (1.0 - tt) * S[1] + tt * D[1],
+                              (1.0 - tt) * S[2] + tt * D[2]};
+        sq = (c[0] - pp[0]) * (c[0] - pp[0]) + (c[1] - pp[1]) * (c[1] - pp[1]) + (c[2] - pp[2]) * (c[2] - pp[2]);
+      }
+      if (sqrt(sq) < kEps) {
+        src = pn.EN + 9 * f + 3 * e;
+        found = true;
+      }
+    }
+  }
+  n[0] = src[0];
+  n[1] = src[1];
+  n[2] = src[2];
+}
+
 // signed_distance.cpp:96-135: out of [low2, up2) -> NaN, F.rows()+1, 0;
-// otherwise S = s * sqrt(d2) with s from the sign type.
-__global__ void k_sd_combine(int64_t np, const double* __restrict__ best, const int32_t* __restrict__ face,
-                             const double* __restrict__ cl, const double* __restrict__ W, int sign_type,
-                             double low2, double up2, int64_t nf, double* __restrict__ S, int32_t* __restrict__ I,
-                             double* __restrict__ C) {
+// otherwise S = s * sqrt(d2) with s from the sign type.  N (pseudonormal sign
+// only, when given) receives the normal the test used; 0 for points out of
+// bounds (the reference leaves those rows unset).
+__global__ void k_sd_combine(int64_t np, const double* __restrict__ P, const double* __restrict__ best,
+                             const int32_t* __restrict__ face, const double* __restrict__ cl,
+                             const double* __restrict__ W, int sign_type, double low2, double up2, int64_t nf,
+                             Pseudonormals pn, double* __restrict__ S, int32_t* __restrict__ I,
+                             double* __restrict__ C, double* __restrict__ N) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double d2 = best[p];
@@ -220,6 +321,8 @@ __global__ void k_sd_combine(int64_t np, const double* __restrict__ best, const 
     S[p] = __longlong_as_double(0x7FF8000000000000ll);
     I[p] = (int32_t)(nf + 1);
     for (int k = 0; k < 3; ++k) C[3 * p + k] = 0.0;
+    if (N)
+      for (int k = 0; k < 3; ++k) N[3 * p + k] = 0.0;
     return;
   }
   double s = 1.0;
@@ -228,6 +331,14 @@ __global__ void k_sd_combine(int64_t np, const double* __restrict__ best, const 
     s = 1.0 - 2.0 * fabs(w);
   } else if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT) {
     s = 1.0 - 2.0 * W[p];
+  } else if (sign_type == GWN_SIGN_PSEUDONORMAL) {
+    const double c[3] = {cl[3 * p], cl[3 * p + 1], cl[3 * p + 2]};
+    double n[3];
+    pseudonormal_at(pn, face[p], c, n);
+    const double qc[3] = {P[3 * p] - c[0], P[3 * p + 1] - c[1], P[3 * p + 2] - c[2]};
+    s = (qc[0] * n[0] + qc[1] * n[1] + qc[2] * n[2] >= 0.0) ? 1.0 : -1.0;
+    if (N)
+      for (int k = 0; k < 3; ++k) N[3 * p + k] = n[k];
   }
   S[p] = s * sqrt(d2);
   I[p] = face[p];
@@ -245,8 +356,6 @@ __global__ void k_sd_empty(int64_t np, double* __restrict__ S, int32_t* __restri
 
 }  // namespace
 
-void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* O, int64_t no, double* W,
-                  hipStream_t st);
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision, double* W,
                        hipStream_t st);
 
@@ -258,8 +367,31 @@ static void sqr_bounds(double lower, double upper, double& low2, double& up2) {
   low2 = std::pow(mx(max_abs - (upper - lower), 0.0), 2.0);
 }
 
+// The pseudonormals of the soup's mesh (normals.cpp), built on first use from
+// the vertices and faces as given.
+static void soup_ensure_pseudonormals(gwn_soup* s, hipStream_t st) {
+  if (s->pn_ready.load(std::memory_order_acquire)) return;
+  std::lock_guard<std::mutex> lock(s->pn_mu);
+  if (s->pn_ready.load(std::memory_order_relaxed)) return;
+  std::vector<double> V(3 * (size_t)s->nv), FN, VN, EN;
+  std::vector<int32_t> F(3 * (size_t)s->nf);
+  GWN_HIP(hipMemcpyAsync(V.data(), s->V64.p, 8 * V.size(), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipMemcpyAsync(F.data(), s->F.p, 4 * F.size(), hipMemcpyDeviceToHost, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  pseudonormals_host(V.data(), s->nv, F.data(), s->nf, FN, VN, EN);
+  DevBuf<double> fn(FN.size()), vn(VN.size()), en(EN.size());
+  GWN_HIP(hipMemcpyAsync(fn.p, FN.data(), 8 * FN.size(), hipMemcpyHostToDevice, st));
+  GWN_HIP(hipMemcpyAsync(vn.p, VN.data(), 8 * VN.size(), hipMemcpyHostToDevice, st));
+  GWN_HIP(hipMemcpyAsync(en.p, EN.data(), 8 * EN.size(), hipMemcpyHostToDevice, st));
+  GWN_HIP(hipStreamSynchronize(st));
+  s->pn_face = std::move(fn);
+  s->pn_vertex = std::move(vn);
+  s->pn_edge = std::move(en);
+  s->pn_ready.store(true, std::memory_order_release);
+}
+
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
-                            double* S, int32_t* I, double* C, hipStream_t st) {
+                            double* S, int32_t* I, double* C, double* N, hipStream_t st) {
   if (np <= 0) return;
   const unsigned g = div_up(np, 256);
   if (s->nf == 0) {  // signed_distance.cpp:397-403
@@ -268,6 +400,8 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
     return;
   }
   soup_ensure_dist(s, st);
+  const bool pseudo = sign_type == GWN_SIGN_PSEUDONORMAL;
+  if (pseudo) soup_ensure_pseudonormals(s, st);
   double low2, up2;
   sqr_bounds(lower, upper, low2, up2);
   const bool want_w = sign_type == GWN_SIGN_FAST_WINDING_NUMBER || sign_type == GWN_SIGN_WINDING_NUMBER ||
@@ -293,16 +427,27 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
                        perm, np, up2, best, face, cl);
     check_launch("k_closest");
   }
+  gwn_exact_tree hier;
   if (sign_type == GWN_SIGN_FAST_WINDING_NUMBER) {
     // signed_distance.cpp:125-130: fast_winding_number(fwn_bvh, 2, q.cast<float>())
     soup_query_device(s, Pf, np, 2.0f, GWN_PREC_FP32, W, st);
   } else if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT) {
-    // signed_distance.cpp:121-124: the exact winding number
-    exact_device(s->V64.p, s->nv, s->F.p, s->nf, P, np, W, st);
+    // signed_distance.cpp:60-64, 121-124: the exact winding number through a
+    // WindingNumberAABB built for the call (hier3.set_mesh; hier3.grow())
+    std::vector<double> V(3 * (size_t)s->nv);
+    std::vector<int32_t> F(3 * (size_t)s->nf);
+    GWN_HIP(hipMemcpyAsync(V.data(), s->V64.p, 8 * V.size(), hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipMemcpyAsync(F.data(), s->F.p, 4 * F.size(), hipMemcpyDeviceToHost, st));
+    GWN_HIP(hipStreamSynchronize(st));
+    exact_tree_build(&hier, V.data(), s->nv, F.data(), s->nf);
+    wntree_query_device(hier.rec.p, hier.nrec, hier.tri.p, hier.bbox, P, np, W, st);
   }
-  hipLaunchKernelGGL(k_sd_combine, dim3(g), dim3(256), 0, st, np, best, face, cl, W, sign_type, low2, up2,
-                     s->nf, S, I, C);
+  const Pseudonormals pn{s->V64.p, s->F.p, s->pn_face.p, s->pn_vertex.p, s->pn_edge.p};
+  hipLaunchKernelGGL(k_sd_combine, dim3(g), dim3(256), 0, st, np, P, best, face, cl, W, sign_type, low2, up2, s->nf,
+                     pn, S, I, C, pseudo ? N : nullptr);
   check_launch("k_sd_combine");
+  if (sign_type == GWN_SIGN_WINDING_NUMBER || sign_type == GWN_SIGN_DEFAULT)
+    GWN_HIP(hipStreamSynchronize(st));  // the hierarchy's records are freed on return
 }
 
 }  // namespace gwn

@@ -220,7 +220,11 @@ void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, 
 // 30-bit Hilbert keys (10 bits per axis over bbox) + identity values: query order for the tree walks
 void hilbert_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
-                            double* S, int32_t* I, double* C, hipStream_t st);
+                            double* S, int32_t* I, double* C, double* N, hipStream_t st);
+// the normals of the pseudonormal sign test, in the reference's summation
+// order (normals.cpp)
+void pseudonormals_host(const double* V, int64_t nv, const int32_t* F, int64_t nf, std::vector<double>& FN,
+                        std::vector<double>& VN, std::vector<double>& ENf);
 // k nearest neighbours and point areas (knn.hip)
 int knn_columns(int64_t nv, int64_t k);
 void knn_device(const double* P, int64_t np, const double* V, int64_t nv, int64_t k, const int32_t* cand,
@@ -269,11 +273,15 @@ struct gwn_soup {
   gwn::DevBuf<double> slots64;  // nslots x 24 (FP64 moments), built with the tree
   gwn::DevBuf<double> V64;      // nv x 3, the vertices as given (closest-point queries)
   gwn::DevBuf<float4> dbox;     // nslots x 2 slot boxes for closest-point queries, built on first use
+  // pseudonormal sign: face normals nf x 3, angle-weighted vertex normals
+  // nv x 3, edge normals per face and corner nf x 3 x 3; built on first use
+  gwn::DevBuf<double> pn_face, pn_vertex, pn_edge;
   gwn_mixed_params mixed{1e-3f, 0.05f};
-  // After the build a handle is read-only to queries, except for dbox, which
-  // the first closest-point query builds under dist_mu (dist_ready set after).
-  std::mutex dist_mu;
-  std::atomic<bool> dist_ready{false};
+  // After the build a handle is read-only to queries, except for dbox and the
+  // pseudonormals, which the first query that needs them builds under
+  // dist_mu / pn_mu (dist_ready / pn_ready set after).
+  std::mutex dist_mu, pn_mu;
+  std::atomic<bool> dist_ready{false}, pn_ready{false};
 };
 
 // 2D curve (UT_SubtendedAngle<float,float>): segments, the 4-wide tree over
@@ -315,6 +323,9 @@ namespace gwn {
 // exact.hip: the hierarchy's records on the device, and its queries (O, W device)
 void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, const float* bbox, const double* O,
                          int64_t no, double* W, hipStream_t st);
+// igl::WindingNumberAABB<double,int> of host arrays V, F (built on the host,
+// drawing rand() as the reference does; records uploaded to the device)
+void exact_tree_build(gwn_exact_tree* t, const double* V, int64_t nv, const int32_t* F, int64_t nf);
 }  // namespace gwn
 
 struct gwn_cloud {

@@ -186,15 +186,19 @@ class FastWindingNumberBVH:
         """Device pointers (e.g. libigl_amd.devmem.DeviceArray.data_ptr): Q float nq x 3, W double nq."""
         check(lib().gwn_soup_query_device(self.handle, Q_ptr, nq, float(accuracy_scale), precision, W_ptr, stream))
 
-    def signed_distance(self, P, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX):
-        """igl::signed_distance over this mesh: (S, I, C)."""
+    def signed_distance(self, P, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX,
+                        with_normals=False):
+        """igl::signed_distance over this mesh: (S, I, C), and N (np x 3, the
+        pseudonormal sign's normals) with with_normals."""
         P = _rows3(P, "P", np.float64)
         S = np.empty(P.shape[0])
         I = np.empty(P.shape[0], np.int32)
         C = np.empty((P.shape[0], 3))
-        check(lib().gwn_signed_distance(self.handle, P, P.shape[0], int(sign_type), float(lower_bound),
-                                        float(upper_bound), S, I, C))
-        return S, I, C
+        N = np.zeros((P.shape[0], 3)) if with_normals else None
+        check(lib().gwn_signed_distance_with_normals(self.handle, P, P.shape[0], int(sign_type), float(lower_bound),
+                                                     float(upper_bound), S, I, C,
+                                                     N.ctypes.data if with_normals else None))
+        return (S, I, C, N) if with_normals else (S, I, C)
 
 
 def C_byref(x):
@@ -563,11 +567,11 @@ def fast_winding_number(*args, **kw):
 
 
 def signed_distance(P, V, F, sign_type=SIGN_FAST_WINDING_NUMBER, lower_bound=DBL_MIN, upper_bound=DBL_MAX):
-    """igl::signed_distance(P,V,F,sign_type[,lower,upper],S,I,C,N) for the
-    unsigned, winding-number and fast-winding-number sign types (the FWN sign
-    uses an order-2 tree, as signed_distance.cpp:65-67 builds it).  Returns
-    (S, I, C)."""
-    return FastWindingNumberBVH().build(V, F, 2).signed_distance(P, sign_type, lower_bound, upper_bound)
+    """igl::signed_distance(P,V,F,sign_type[,lower,upper],S,I,C,N) for every
+    sign type (the FWN sign uses an order-2 tree, as signed_distance.cpp:65-67
+    builds it).  Returns (S, I, C), and N as well for SIGN_PSEUDONORMAL."""
+    bvh = FastWindingNumberBVH().build(V, F, 2)
+    return bvh.signed_distance(P, sign_type, lower_bound, upper_bound, with_normals=sign_type == SIGN_PSEUDONORMAL)
 
 
 def read_triangle_mesh(path):

@@ -129,11 +129,25 @@ void oracle_curve_count(const oracle_curve* c, const float* q, float accuracy_sc
  * writes it to out (may be NULL) and returns the squared distance */
 double oracle_point_tri_sqrdist(const double* p, const double* a, const double* b, const double* c, double* out);
 /* igl::signed_distance(P,V,F,sign_type,lower,upper,S,I,C,N) for sign types
- * 1/2 (winding number), 3 (unsigned), 4 (fast winding number), by brute force
- * over all faces; returns 0, or -1 for an unsupported sign type */
+ * 0 (pseudonormal), 1/2 (winding number), 3 (unsigned), 4 (fast winding
+ * number), by brute force over all faces; returns 0, or -1 for an unknown
+ * sign type */
 int oracle_signed_distance(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* P, int64_t np,
                            int sign_type, double lower, double upper, double* S, int32_t* I, double* C,
                            int nthreads);
+/* The same with N np x 3 (may be NULL): the pseudonormal sign type (0) writes
+ * the normal it used (0 for points out of bounds). */
+int oracle_signed_distance_n(const double* V, int64_t nv, const int32_t* F, int64_t nf, const double* P, int64_t np,
+                             int sign_type, double lower, double upper, double* S, int32_t* I, double* C, double* N,
+                             int nthreads);
+/* per_face_normals, per_vertex_normals (angle weights) and per_edge_normals
+ * (uniform weights) of (V,F): FN nf x 3, VN nv x 3, EN nf x 3 x 3 (the edge
+ * opposite corner c of face f).  0 on success. */
+int oracle_pseudonormals(const double* V, int64_t nv, const int32_t* F, int64_t nf, double* FN, double* VN,
+                         double* EN);
+/* pseudonormal_test: the normal for closest point c on face f. */
+void oracle_pseudonormal_at(const double* V, const int32_t* F, const double* FN, const double* VN, const double* EN,
+                            int64_t f, const double* c, double* n);
 
 /* ---- 2D curves (winding_number.cpp:62-89 with F.cols() == 2) ----
  * signed_angle.cpp:8-41; V nv x 2, E ne x 2, O no x 2 */

@@ -91,6 +91,11 @@ def lib():
         L.oracle_signed_distance.restype = C.c_int
         L.oracle_signed_distance.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, C.c_int,
                                              C.c_double, C.c_double, _f64p, _i32p, _f64p, C.c_int]
+        L.oracle_signed_distance_n.restype = C.c_int
+        L.oracle_signed_distance_n.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, C.c_int,
+                                               C.c_double, C.c_double, _f64p, _i32p, _f64p, _f64p, C.c_int]
+        L.oracle_pseudonormals.restype = C.c_int
+        L.oracle_pseudonormals.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, _f64p, _f64p]
         L.oracle_wntree_build.restype = C.c_void_p
         L.oracle_wntree_build.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64]
         L.oracle_wntree_free.argtypes = [C.c_void_p]
@@ -203,19 +208,34 @@ def point_tri_sqrdist(p, a, b, c):
     return d2, out
 
 
-def signed_distance(V, F, P, sign_type, lower=2.2250738585072014e-308, upper=1.7976931348623157e308, nthreads=0):
-    """igl::signed_distance restated (brute force): returns S, I, C."""
+def signed_distance(V, F, P, sign_type, lower=2.2250738585072014e-308, upper=1.7976931348623157e308, nthreads=0,
+                    with_normals=False):
+    """igl::signed_distance restated (brute force): returns S, I, C (and N
+    with with_normals, written by the pseudonormal sign type)."""
     V = _c(V, np.float64).reshape(-1, 3)
     F = _c(F, np.int32).reshape(-1, 3)
     P = _c(P, np.float64).reshape(-1, 3)
     S = np.empty(P.shape[0])
     I = np.empty(P.shape[0], np.int32)
     C_ = np.empty((P.shape[0], 3))
-    rc = lib().oracle_signed_distance(V, V.shape[0], F, F.shape[0], P, P.shape[0], int(sign_type), float(lower),
-                                      float(upper), S, I, C_, nthreads)
+    N = np.zeros((P.shape[0], 3))
+    rc = lib().oracle_signed_distance_n(V, V.shape[0], F, F.shape[0], P, P.shape[0], int(sign_type), float(lower),
+                                        float(upper), S, I, C_, N, nthreads)
     if rc != 0:
         raise ValueError("oracle_signed_distance failed")
-    return S, I, C_
+    return (S, I, C_, N) if with_normals else (S, I, C_)
+
+
+def pseudonormals(V, F):
+    """(FN nf x 3, VN nv x 3, EN nf x 3 x 3) as signed_distance.cpp:68-76 builds them."""
+    V = _c(V, np.float64).reshape(-1, 3)
+    F = _c(F, np.int32).reshape(-1, 3)
+    FN = np.empty((F.shape[0], 3))
+    VN = np.empty((V.shape[0], 3))
+    EN = np.empty((F.shape[0], 3, 3))
+    if lib().oracle_pseudonormals(V, V.shape[0], F, F.shape[0], FN, VN, EN) != 0:
+        raise MemoryError("oracle_pseudonormals")
+    return FN, VN, EN
 
 
 # ---------------- exact ----------------

@@ -22,8 +22,8 @@ int main() {
   P(0, 0) = 0.5, P(0, 1) = 0.5, P(0, 2) = 0.5;
   const double Sexact = std::sqrt(1.0 / 12.0);
   // igl::signed_distance, both overloads, every provided sign type
-  for (igl::SignedDistanceType t : {igl::SIGNED_DISTANCE_TYPE_WINDING_NUMBER, igl::SIGNED_DISTANCE_TYPE_DEFAULT,
-                                    igl::SIGNED_DISTANCE_TYPE_UNSIGNED,
+  for (igl::SignedDistanceType t : {igl::SIGNED_DISTANCE_TYPE_PSEUDONORMAL, igl::SIGNED_DISTANCE_TYPE_WINDING_NUMBER,
+                                    igl::SIGNED_DISTANCE_TYPE_DEFAULT, igl::SIGNED_DISTANCE_TYPE_UNSIGNED,
                                     igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER}) {
     Eigen::VectorXd S;
     Eigen::VectorXi I;
@@ -31,6 +31,11 @@ int main() {
     igl::signed_distance(P, V, F, t, S, I, C, N);
     const double tol = t == igl::SIGNED_DISTANCE_TYPE_FAST_WINDING_NUMBER ? 1e-7 : 1e-15;
     if (S.rows() != 1 || std::fabs(S(0, 0) - Sexact) > tol || I(0, 0) != 3 || C.cols() != 3) bad |= 1;
+    // the pseudonormal type resizes and fills N: the outward normal of face 3
+    if (t == igl::SIGNED_DISTANCE_TYPE_PSEUDONORMAL &&
+        (N.rows() != 1 || N.cols() != 3 || std::fabs(N(0, 0) - std::sqrt(1.0 / 3.0)) > 1e-15 ||
+         std::fabs(N(0, 2) - std::sqrt(1.0 / 3.0)) > 1e-15))
+      bad |= 1 << 23;
     // bounds (0.05, 0.1): squared distances kept in [0.05^2, 0.1^2) (signed_distance.cpp:79-84);
     // d^2 = 1/12 is outside: NaN, I = F.rows() + 1
     igl::signed_distance(P, V, F, t, 0.05, 0.1, S, I, C, N);

@@ -1,12 +1,16 @@
-"""GPU igl::signed_distance (unsigned, winding-number and fast-winding-number
-sign types) through the C ABI, against the oracle's brute-force restatement
-(itself pinned by closed forms and the single-tet golden value).
+"""GPU igl::signed_distance (all five sign types) through the C ABI, against
+the oracle's brute-force restatement (itself pinned by closed forms, the
+single-tet golden value and, for the pseudonormals, by
+tests/test_oracle_pseudonormal.py).
 
 Tolerances: squared distances and closest points are FP64 on both sides and
 agree to 1e-12 (relative); the closest face agrees except where several faces
 are equally close (then the GPU's face must be at the same distance); the
 fast-winding-number sign differs only through the FWN value (FP32, order 2,
-accuracy_scale 2), within 1e-4 in w, so |dS| <= 2e-4 |S|."""
+accuracy_scale 2), within 1e-4 in w, so |dS| <= 2e-4 |S|; the winding-number
+sign comes from the exact hierarchy on the GPU and from the brute-force sum in
+the oracle, which agree to ~1e-12 in w, so |dS| <= 1e-11 max(1, |S|); the
+pseudonormal sign is +-1 exactly and its normal agrees to 1e-12."""
 import numpy as np
 import pytest
 
@@ -16,7 +20,8 @@ from libigl_amd import gwn
 
 pytestmark = pytest.mark.gpu
 
-SIGNS = (gwn.SIGN_UNSIGNED, gwn.SIGN_WINDING_NUMBER, gwn.SIGN_DEFAULT, gwn.SIGN_FAST_WINDING_NUMBER)
+SIGNS = (gwn.SIGN_UNSIGNED, gwn.SIGN_WINDING_NUMBER, gwn.SIGN_DEFAULT, gwn.SIGN_FAST_WINDING_NUMBER,
+         gwn.SIGN_PSEUDONORMAL)
 
 
 def _check(V, F, P, S, I, C, So, Io, Co, sign):
@@ -26,6 +31,8 @@ def _check(V, F, P, S, I, C, So, Io, Co, sign):
     m = ~nan
     if sign == gwn.SIGN_FAST_WINDING_NUMBER:
         assert np.all(np.abs(S[m] - So[m]) <= 2e-4 * np.abs(So[m]) + 1e-12)
+    elif sign in (gwn.SIGN_WINDING_NUMBER, gwn.SIGN_DEFAULT):
+        assert np.all(np.abs(S[m] - So[m]) <= 1e-11 * np.maximum(1.0, np.abs(So[m])))
     else:
         assert np.all(np.abs(S[m] - So[m]) <= 1e-12 * np.maximum(1.0, np.abs(So[m])))
     same = I[m] == Io[m]
@@ -43,13 +50,15 @@ def test_single_tet_golden():
     # tests/include/igl/signed_distance.cpp:4-44
     V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
     F = np.array([[0, 1, 3], [0, 2, 1], [0, 3, 2], [1, 2, 3]], np.int32)
-    for sign, tol in ((gwn.SIGN_WINDING_NUMBER, 1e-15), (gwn.SIGN_DEFAULT, 1e-15), (gwn.SIGN_UNSIGNED, 1e-15),
-                      (gwn.SIGN_FAST_WINDING_NUMBER, 1e-7)):
-        S, I, C = gwn.signed_distance(np.array([[0.5, 0.5, 0.5]]), V, F, sign)
+    for sign, tol in ((gwn.SIGN_PSEUDONORMAL, 1e-15), (gwn.SIGN_WINDING_NUMBER, 1e-15), (gwn.SIGN_DEFAULT, 1e-15),
+                      (gwn.SIGN_UNSIGNED, 1e-15), (gwn.SIGN_FAST_WINDING_NUMBER, 1e-7)):
+        S, I, C = gwn.signed_distance(np.array([[0.5, 0.5, 0.5]]), V, F, sign)[:3]
         assert abs(S[0] - np.sqrt(1 / 12)) < tol, (sign, S)
         assert I[0] == 3 and np.allclose(C[0], [1 / 3, 1 / 3, 1 / 3], atol=1e-15)
+    N = gwn.signed_distance(np.array([[0.5, 0.5, 0.5]]), V, F, gwn.SIGN_PSEUDONORMAL)[3]
+    assert np.allclose(N[0], np.full(3, 1 / np.sqrt(3)), atol=1e-15)
     with pytest.raises(gwn.GWNError):
-        gwn.signed_distance(np.zeros((1, 3)), V, F, gwn.SIGN_PSEUDONORMAL)
+        gwn.signed_distance(np.zeros((1, 3)), V, F, 7)
 
 
 @pytest.mark.parametrize("sign", SIGNS)
@@ -60,9 +69,41 @@ def test_signed_distance_vs_oracle(sign):
     P = np.concatenate([D.uniform_queries(lo, hi, 3000, rng), D.near_surface_queries(V, F, 1000, 1e-3, rng),
                         V[:50] + 0.0])  # points on vertices: ties between faces
     bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
-    S, I, C = bvh.signed_distance(P, sign)
-    So, Io, Co = O.signed_distance(V, F, P, sign)
+    S, I, C, N = bvh.signed_distance(P, sign, with_normals=True)
+    So, Io, Co, No = O.signed_distance(V, F, P, sign, with_normals=True)
     _check(V, F, P, S, I, C, So, Io, Co, sign)
+    if sign == gwn.SIGN_PSEUDONORMAL:
+        # the same normal (at ties between faces the test lands on the shared
+        # vertex's or edge's normal from either face); on this closed mesh the
+        # sign is the winding number's
+        assert np.allclose(N, No, rtol=0, atol=1e-12)
+        Sw, _, _ = O.signed_distance(V, F, P, gwn.SIGN_WINDING_NUMBER)
+        assert np.array_equal(np.sign(S), np.sign(Sw))
+    else:
+        assert np.all(N == 0)  # not written for the other sign types
+
+
+def test_pseudonormal_cube_and_open_mesh():
+    """Corner, edge and face normals of a triangulated cube (closed forms, as
+    the oracle's test pins them), and an open mesh against the oracle."""
+    import test_oracle_pseudonormal as T
+    V, F = T.cube()
+    r3, r2 = 1 / np.sqrt(3), 1 / np.sqrt(2)
+    P = np.array([[1.5, -0.6, -0.7], [-0.4, 1.5, 1.6], [1.5, 1.5, 0.5], [0.5, -0.3, -0.3], [0.3, 0.6, 1.5],
+                  [0.9, 0.4, 0.6]])
+    S, I, C, N = gwn.signed_distance(P, V, F, gwn.SIGN_PSEUDONORMAL)
+    want = np.array([[r3, -r3, -r3], [-r3, r3, r3], [r2, r2, 0], [0, -r2, -r2], [0, 0, 1], [1, 0, 0]])
+    assert np.allclose(N, want, atol=1e-14)
+    assert np.all(S[:5] > 0) and abs(S[5] + 0.1) < 1e-15
+    Vo, Fo = D.noisy_sphere(3, 0.01, 8)
+    Fo = Fo[Vo[Fo].mean(1)[:, 2] < 0.4].copy()  # open: a cap removed
+    rng = np.random.default_rng(12)
+    Po = rng.uniform(-1.3, 1.3, (3000, 3))
+    S, I, C, N = gwn.FastWindingNumberBVH().build(Vo, Fo, 2).signed_distance(Po, gwn.SIGN_PSEUDONORMAL,
+                                                                             with_normals=True)
+    So, Io, Co, No = O.signed_distance(Vo, Fo, Po, gwn.SIGN_PSEUDONORMAL, with_normals=True)
+    _check(Vo, Fo, Po, S, I, C, So, Io, Co, gwn.SIGN_PSEUDONORMAL)
+    assert np.allclose(N, No, rtol=0, atol=1e-12)
 
 
 @pytest.mark.parametrize("bounds", [(-0.05, 0.05), (0.02, 0.1), (-0.2, -0.01)])
