@@ -4,7 +4,7 @@ Workload (the headline configuration): level-6 icosphere, 81,920 triangles,
 radially displaced by 16 seeded Gaussian bumps; 1,000,000 queries, half
 uniform in the bbox and half within 1e-3 of the surface; order 2.
 
-A step is one query pass over the 1M-query batch (Morton ordering of the
+A step is one query pass over the 1M-query batch (Hilbert ordering of the
 queries + traversal), with the tree built beforehand and the queries resident
 in HBM.  accuracy_scale (beta) is chosen as the smallest value on a grid for
 which max |W - W_exact| <= 1e-4 over all 1M queries, W_exact being the FP64
@@ -233,10 +233,11 @@ def main():
     err_fp64 = float(np.abs(dW.numpy() - Wx).max())
 
     # counted FLOPs per query from the oracle's instrumented traversal of the
-    # same tree on a sample of the queries
+    # tree the kernel walks (its node array, moments rebuilt by the oracle) on
+    # a sample of the queries
     flops_per_q = None
     try:
-        soup_o = O.Soup(V, F)
+        soup_o = O.Soup(V, F, nodes=bvh.nodes())
         samp = np.arange(0, nq, max(1, nq // 4000))
         tot = np.zeros(3)
         for i in samp:
@@ -319,7 +320,7 @@ def main():
                     "frac": round(achieved / 157.3, 4), "traffic": traffic,
                     "hbm_GBps": round(traffic / (k_ms * 1e-3) / 1e9, 1) if traffic else None,
                     "hbm_peak_GBps": 8000.0, "traffic_note": traffic_note,
-                    "kernel": "k_traverse<2,false>", "kernel_ms": round(k_ms, 4),
+                    "kernel": "k_traverse<2,false,false>", "kernel_ms": round(k_ms, 4),
                     "flop_per_query": round(flops_per_q, 1),
                     "note": "FP32 VALU; no MFMA on this path; packed FP32 disabled so 78.6 TFLOP/s is the "
                             "attainable scalar-FP32 ceiling; FLOPs counted on the tree the kernel walks"}
@@ -422,7 +423,7 @@ def run_boundary(bvh, Q, beta):
 
 
 def measure_traffic(args, beta):
-    """HBM bytes per k_traverse<2,false> launch of the headline run, measured
+    """HBM bytes per k_traverse<2,false,false> launch of the headline run, measured
     now: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch) over a child run of this script at the same accuracy_scale and
     tree, which also makes three 256 MiB copies with the library's plain copy
@@ -442,7 +443,7 @@ def measure_traffic(args, beta):
     child = [sys.executable, os.path.abspath(__file__), "--beta", repr(float(beta)), "--steps", "2", "--warmup", "1",
              "--tree", args.tree, "--nq", str(args.nq), "--calibrate-copy", str(copy_bytes)]
     env = dict(os.environ, TMPDIR="/tmp")
-    kre = re.compile(r"k_traverse<\s*2\s*,\s*false\s*>")
+    kre = re.compile(r"k_traverse<\s*2\s*,\s*false\s*,\s*false\s*>")
     vals = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="gwn_pmc_", dir="/tmp")
@@ -542,17 +543,41 @@ def run_mixed_workload(beta_grid):
     def rate(prec, steps):
         run(beta, prec)
         dm.synchronize()
+        nat.lib().gwn_profile_reset()
+        nat.lib().gwn_profile_enable(1)
         t = time.perf_counter()
         for _ in range(steps):
             run(beta, prec)
         dm.synchronize()
-        return nq * steps / (time.perf_counter() - t)
+        dt = time.perf_counter() - t
+        nat.lib().gwn_profile_enable(0)
+        kernels = {k: round(_kernel_ms(k), 4) for k in ("soup_traverse", "soup_reeval64", "soup_traverse64")}
+        return nq * steps / dt, {k: v for k, v in kernels.items() if v > 0}
 
+    # counted FLOPs per query (the oracle's instrumented walk of this tree, on
+    # a sample): the roofline of each precision's walk kernel
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    soup_o = O.Soup(V, F, nodes=bvh.nodes())
+    samp = np.arange(0, nq, nq // 2000)
+    Qf = Q.astype(np.float32)
+    c = np.array([soup_o.count(Qf[i], beta) for i in samp], np.float64).mean(0)
+    fpq = FLOP_TEST * 4 * c[0] + FLOP_FAR * c[1] + FLOP_EXACT * c[2]
     out = {"workload": "level-8 icosphere 1,310,720 tris, 64 bumps, 10M queries (80% uniform, 20% within 1e-4)",
-           "accuracy_scale": beta, "build_s": round(build_s, 3), "error_sample": int(sub.size)}
+           "accuracy_scale": beta, "build_s": round(build_s, 3), "error_sample": int(sub.size),
+           "flop_per_query": round(fpq, 1)}
     for name, prec in (("fp32", gwn.PREC_FP32), ("mixed", gwn.PREC_MIXED), ("fp64", gwn.PREC_FP64)):
         mx, p99 = err(beta, prec)
-        out[name] = {"queries_per_s": round(rate(prec, 3), 1), "max_abs_dw": mx, "p99_abs_dw": p99}
+        qps, kms = rate(prec, 3)
+        out[name] = {"queries_per_s": round(qps, 1), "max_abs_dw": mx, "p99_abs_dw": p99, "kernel_ms": kms}
+        walk = "soup_traverse64" if prec == gwn.PREC_FP64 else "soup_traverse"
+        if walk in kms:
+            peak = 78.6 if prec == gwn.PREC_FP64 else 157.3
+            ach = fpq * nq / (kms[walk] * 1e-3) / 1e12
+            out[name]["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                                     "frac": round(ach / peak, 4),
+                                     "kernel": "k_traverse64<2>" if prec == gwn.PREC_FP64 else
+                                     f"k_traverse<2,{'true' if prec == gwn.PREC_MIXED else 'false'},false>"}
         log(f"[bench] 1M-triangle mesh, 10M queries, {name}: {out[name]}")
     out["mixed_over_fp64"] = round(out["mixed"]["queries_per_s"] / out["fp64"]["queries_per_s"], 3)
     del dQ, dW, dV, dF, dQs, dWx
@@ -615,10 +640,21 @@ def run_cloud(no_cpu):
     out = {"workload": "2M points, 160^3 grid queries, order 2, beta 2", "cells": int(c.ncells()),
            "build_s": round(build_s, 4), "build_warm_s": round(build_warm_s, 4), "queries_per_s": round(nq / dt, 1),
            "kernel_ms": round(kms, 4)}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    oc = O.Cloud(P.astype(np.float32), N.astype(np.float32), A.astype(np.float32), 2, np.float32)
+    # counted work per query from the oracle's instrumented walk of the same
+    # octree, on a sample of the grid
+    samp = np.arange(0, nq, nq // 2000)
+    cnt = np.array([oc.count(Q[i].astype(np.float32), 2.0) for i in samp], np.float64).mean(0)
+    fpq = CLOUD_TEST * cnt[0] + CLOUD_DIPOLE * cnt[1] + CLOUD_EXP2 * cnt[2]
+    achieved = fpq * nq / (kms * 1e-3) / 1e12
+    out["work_per_query"] = {"cell_tests": round(cnt[0], 1), "dipoles": round(cnt[1], 1),
+                             "expansions": round(cnt[2], 1), "flop": round(fpq, 1)}
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": 157.3, "unit": "TFLOP/s",
+                       "frac": round(achieved / 157.3, 4), "kernel": "k_cloud_traverse<39>",
+                       "note": "FLOPs counted on the reference's algorithm (fast_winding_number.cpp:159-311)"}
     if not no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as O
-        oc = O.Cloud(P.astype(np.float32), N.astype(np.float32), A.astype(np.float32), 2, np.float32)
         sub = np.arange(0, nq, nq // 20_000)
         threads = max(1, min(16, host_info()['nproc'] or 1))
         t = time.perf_counter()
@@ -630,6 +666,14 @@ def run_cloud(no_cpu):
     log(f"[bench] cloud: {out}")
     return out
 
+
+# Counted FLOPs of the point-cloud walk (fast_winding_number.cpp:159-311):
+# a child cell's test |CM - q| > beta R (3 sub, a norm of 6, 1 mul, compare);
+# a direct dipole term with its A n and offset (3 sub, 3 mul, norm 6, dot 5,
+# r^3 and 4 pi 3, division, add); an order-2 expansion (the dipole part 15,
+# |loc| 6, the second derivative and its dot 46, the three 3x3 blocks of the
+# third derivative with their dots 182).
+CLOUD_TEST, CLOUD_DIPOLE, CLOUD_EXP2 = 11, 22, 249
 
 # FP64 FLOPs of one solid angle (solid_angle.cpp:10-55): 9 subtractions, 3
 # norms (6 each), the determinant (17), 3 dot products (15), the denominator

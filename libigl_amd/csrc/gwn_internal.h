@@ -239,9 +239,11 @@ void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float ac
                         hipStream_t st);
 // traversal kernel: WALK_WAVE (wave-coherent, sorted batch queries),
 // WALK_LANE (per-lane over the sorted queries, used by the serving engine for
-// small chunks) or WALK_LANE_UNSORTED (per-lane in input order: no sort, a
-// workspace of 4 nq + 4 bytes); bitwise the same results
-constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2;
+// small chunks), WALK_LANE_UNSORTED (per-lane in input order: no sort, a
+// workspace of 4 nq + 4 bytes) or WALK_WAVE_HANDOVER (wave-coherent until a
+// wavefront's lanes diverge, then per-lane: shorter batches on large trees
+// with thin-triangle fans, slower on coherent ones); bitwise the same results
+constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2, WALK_WAVE_HANDOVER = 3;
 // flags of soup_query_device_ws: WS_DEVICE_COUNT sizes the mixed form's FP64
 // re-evaluation on the device (no host round trip); WS_NO_PROFILE creates no
 // kernel-timing events

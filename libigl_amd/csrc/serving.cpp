@@ -214,8 +214,12 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     // wavefront coherence); large ones the wave-coherent walk, whose
     // throughput is the higher one there (measured on the mix above: at scale
     // 5 and 1000 requests/s, p99 15 ms against 42 ms with the per-lane walk
-    // everywhere).  Both walks give bitwise the same result.
-    const int walk = n >= 65536 ? WALK_WAVE : WALK_LANE;
+    // everywhere).  On trees of 2^18+ slots the wave walk hands over to
+    // per-lane walks where a wavefront's lanes diverge: a request is done when
+    // its slowest wavefront is, and on a 434k-triangle UV sphere (fans of thin
+    // triangles at the poles) that cut a 1M-query batch at scale 4.9 from 8.0
+    // to 5.2 ms.  All walks give bitwise the same result.
+    const int walk = n < 65536 ? WALK_LANE : m->nslots >= (1 << 18) ? WALK_WAVE_HANDOVER : WALK_WAVE;
     soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn), walk);
     GWN_HIP(hipMemcpyAsync(sl->hW[b], dW, 8 * (size_t)n, hipMemcpyDeviceToHost, sl->stream));
     GWN_HIP(hipEventRecord(sl->ev[b], sl->stream));

@@ -575,6 +575,44 @@ __device__ __forceinline__ double finish_walk(float sum, const NearList& L, cons
 // Both walks are held to 8 wavefronts per SIMD (64 VGPRs): the walk loop fits
 // in that, and only the mixed form's end-of-walk FP64 code would take more
 // (94 VGPRs, 5 wavefronts); it keeps some of its constants in scratch instead.
+// Per-lane rope walk from record j to the end: each lane follows its own skip
+// pointers and gathers its own records.  k_traverse_lane runs it from record
+// 0; the wave-coherent walk hands over to it where its wavefront's lanes have
+// stopped sharing records.  Records, terms and their order are the same in
+// both walks.
+template <int ORDER, bool MIXED>
+__device__ __forceinline__ void lane_walk(const float4* __restrict__ slots, uint32_t nslots,
+                                          const float* __restrict__ tri, uint32_t j, float qx, float qy, float qz,
+                                          float acc2, float near_d, float& sum, NearList& L) {
+  while (j < nslots) {
+    const float4* r = slots + 6 * (size_t)j;
+    const float4 a = r[0], b = r[1];
+    const uint32_t link = __float_as_uint(b.w);
+    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+    const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+    const bool leaf = (link & LINK_LEAF) != 0;
+    if (!(l2 <= __fmul_rn(a.w, acc2))) {
+      float4 c, d, e, f;
+      if (ORDER >= 1) {
+        c = r[2];
+        d = r[3];
+      }
+      if (ORDER >= 2) {
+        e = r[4];
+        f = r[5];
+      }
+      const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
+      if (isfinite(o)) {
+        sum = __fadd_rn(sum, o);
+        j = leaf ? j + 1 : link;
+        continue;
+      }
+    }
+    if (leaf) exact_term<MIXED>(tri, link & ~LINK_LEAF, a.w, l2, qx, qy, qz, near_d, sum, L);
+    ++j;
+  }
+}
+
 // A kernel argument copied into a VGPR: a VALU instruction reads at most one
 // SGPR, so the product of the slot radius (an SGPR in the wave-coherent walk)
 // and accuracy_scale^2 would otherwise need a move at every step.
@@ -584,7 +622,14 @@ __device__ __forceinline__ float in_vgpr(float x) {
   return v;
 }
 
-template <int ORDER, bool MIXED>
+// HANDOVER: a wavefront whose lanes have stopped sharing records (fewer than
+// 16 lanes per step on average over a window of 256 steps: the union of their
+// paths exceeds the longest by far, as on fans of thin triangles) hands over
+// to the per-lane walk, each lane from the record it needs next.  Results are
+// unchanged (same records, terms and order); the check costs about a tenth of
+// the walk's time on coherent batches, so only latency-bound callers (the
+// serving engine, on large trees) take it.
+template <int ORDER, bool MIXED, bool HANDOVER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse(
     const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
     const uint32_t* __restrict__ perm, int64_t nq, float acc2_arg, float near_d, double* __restrict__ W) {
@@ -598,7 +643,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   NearList L{near_lds + threadIdx.x, 0};
   uint32_t skip = 0;
   uint32_t i = 0;
+  uint32_t steps = 0, taken = 0;  // wave-uniform: steps in this window, lanes taking part summed over them
   while (i < nslots) {
+    if (HANDOVER) taken += (uint32_t)__popcll(__ballot(skip <= i));  // lanes taking part in record i
     const float4* r = slots + 6 * (size_t)i;
     const float4 a = r[0], b = r[1];
     float4 c, d, e, f;
@@ -641,7 +688,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
     const bool step = leaf || __ballot(skip <= i + 1) != 0;
     i = wave_uniform(step ? i + 1 : link);
+    if (HANDOVER && ++steps == 256) {
+      if (taken < 256u * 16u) break;
+      steps = taken = 0;
+    }
   }
+  // after a hand-over each lane continues from the record it needs next
+  if (HANDOVER && i < nslots)
+    lane_walk<ORDER, MIXED>(slots, nslots, tri, skip > i ? skip : i, qx, qy, qz, acc2, near_d, sum, L);
   W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
 }
 
@@ -663,34 +717,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   float sum = 0.0f;
   __shared__ uint32_t near_lds[MIXED ? kDefer * kTravThreads : 1];  // mixed form: noted near triangles
   NearList L{near_lds + threadIdx.x, 0};
-  uint32_t i = 0;
-  while (i < nslots) {
-    const float4* r = slots + 6 * (size_t)i;
-    const float4 a = r[0], b = r[1];
-    const uint32_t link = __float_as_uint(b.w);
-    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
-    const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
-    const bool leaf = (link & LINK_LEAF) != 0;
-    if (!(l2 <= __fmul_rn(a.w, acc2))) {
-      float4 c, d, e, f;
-      if (ORDER >= 1) {
-        c = r[2];
-        d = r[3];
-      }
-      if (ORDER >= 2) {
-        e = r[4];
-        f = r[5];
-      }
-      const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
-      if (isfinite(o)) {
-        sum = __fadd_rn(sum, o);
-        i = leaf ? i + 1 : link;
-        continue;
-      }
-    }
-    if (leaf) exact_term<MIXED>(tri, link & ~LINK_LEAF, a.w, l2, qx, qy, qz, near_d, sum, L);
-    ++i;
-  }
+  lane_walk<ORDER, MIXED>(slots, nslots, tri, 0u, qx, qy, qz, acc2, near_d, sum, L);
   W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
 }
 
@@ -1296,7 +1323,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   if (precision == GWN_PREC_FP64) {
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st, prof_on);
-    launch_traverse64(s, walk == WALK_WAVE, g, st, Q, perm, nq, nullptr, a2, W);
+    launch_traverse64(s, walk == WALK_WAVE || walk == WALK_WAVE_HANDOVER, g, st, Q, perm, nq, nullptr, a2, W);
     return;
   }
   const bool mixed = precision == GWN_PREC_MIXED;
@@ -1304,8 +1331,11 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
 #define GWN_LAUNCH_T(O, M)                                                                                      \
   do {                                                                                                          \
     if (walk == WALK_WAVE)                                                                                      \
-      hipLaunchKernelGGL((k_traverse<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq,  \
-                         acc2, nd, W);                                                                          \
+      hipLaunchKernelGGL((k_traverse<O, M, false>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
+                         nq, acc2, nd, W);                                                                      \
+    else if (walk == WALK_WAVE_HANDOVER)                                                                        \
+      hipLaunchKernelGGL((k_traverse<O, M, true>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
+                         nq, acc2, nd, W);                                                                      \
     else                                                                                                        \
       hipLaunchKernelGGL((k_traverse_lane<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, \
                          acc2, nd, W);                                                                          \

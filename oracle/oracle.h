@@ -78,6 +78,9 @@ double oracle_solid_angle_tri_f64(const double* a, const double* b, const double
 typedef struct oracle_soup oracle_soup;
 /* UT_SolidAngle::init (FastWindingNumberForSoups.h:6543-7133) */
 oracle_soup* oracle_soup_build(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order);
+/* The same over a given tree (nnodes x 4 uint32 in the reference's encoding). */
+oracle_soup* oracle_soup_build_nodes(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order,
+                                     const uint32_t* nodes, int32_t nnodes);
 void oracle_soup_free(oracle_soup* s);
 int32_t oracle_soup_num_nodes(const oracle_soup* s);
 /* node child arrays, 4 uint32 per node (FastWindingNumberForSoups.h:4217-4232) */
@@ -195,6 +198,12 @@ void oracle_cloud_eval_f32(const float* P, const float* N, const float* A,
                            const oracle_octree_t* t, int order,
                            const float* CM, const float* R, const float* EC,
                            const float* Q, int64_t nq, double beta, float* WN, int nthreads);
+/* Counted work of one query (bench.py's roofline): child-cell distance
+ * tests, direct dipole terms, Taylor expansions. */
+void oracle_cloud_count_f64(const oracle_octree_t* t, const double* CM, const double* R, const double* q,
+                            double beta, int64_t* tests, int64_t* dipoles, int64_t* exps);
+void oracle_cloud_count_f32(const oracle_octree_t* t, const float* CM, const float* R, const float* q,
+                            double beta, int64_t* tests, int64_t* dipoles, int64_t* exps);
 
 /* ---- antipodal method (WindingNumberAntipodalScene.h:48-256) ----
  * x0 NULL: the reference's default direction.  *nseg_out (may be NULL)

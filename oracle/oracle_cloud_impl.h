@@ -346,6 +346,48 @@ static void FN(cev_range)(int64_t b, int64_t e, void* vctx) {
   }
 }
 
+/* Counted events of one query's evaluation (the helper's control flow, no
+ * arithmetic on the moments): child-cell distance tests, direct dipole terms
+ * and Taylor expansions. */
+static void FN(cev_count)(const oracle_octree_t* t, const REAL* CM, const REAL* R, const REAL* q, double beta,
+                          const int32_t* near, int32_t nnear, int64_t* tests, int64_t* dipoles, int64_t* exps) {
+  int32_t* nn = (int32_t*)malloc(sizeof(int32_t) * 8 * (size_t)(nnear > 0 ? nnear : 1));
+  int32_t nnn = 0;
+  for (int32_t i = 0; i < nnear; ++i) {
+    const int32_t index = near[i];
+    if (t->CH[8 * (int64_t)index] == -1) {
+      *dipoles += t->cell_count[index];
+    } else {
+      for (int ch = 0; ch < 8; ++ch) {
+        const int32_t ci = t->CH[8 * (int64_t)index + ch];
+        if (t->cell_count[ci] > 0) {
+          ++*tests;
+          REAL cmq[3];
+          for (int d = 0; d < 3; ++d) cmq[d] = CM[3 * (int64_t)ci + d] - q[d];
+          const REAL nrm = (REAL)sqrt(cmq[0] * cmq[0] + cmq[1] * cmq[1] + cmq[2] * cmq[2]);
+          if ((double)nrm > beta * (double)R[ci]) {
+            if (t->CH[8 * (int64_t)ci] == -1)
+              *dipoles += t->cell_count[ci];
+            else
+              ++*exps;
+          } else {
+            nn[nnn++] = ci;
+          }
+        }
+      }
+    }
+  }
+  if (nnn > 0) FN(cev_count)(t, CM, R, q, beta, nn, nnn, tests, dipoles, exps);
+  free(nn);
+}
+
+void FN(oracle_cloud_count)(const oracle_octree_t* t, const REAL* CM, const REAL* R, const REAL* q, double beta,
+                            int64_t* tests, int64_t* dipoles, int64_t* exps) {
+  *tests = *dipoles = *exps = 0;
+  const int32_t root = 0;
+  if (t->ncells > 0) FN(cev_count)(t, CM, R, q, beta, &root, 1, tests, dipoles, exps);
+}
+
 void FN(oracle_cloud_eval)(const REAL* P, const REAL* N, const REAL* A, const oracle_octree_t* t, int order,
                            const REAL* CM, const REAL* R, const REAL* EC, const REAL* Q, int64_t nq, double beta,
                            REAL* WN, int nthreads) {

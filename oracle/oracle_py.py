@@ -49,6 +49,8 @@ def lib():
         L.oracle_winding_number_bf.argtypes = [_f64p, C.c_int64, _i32p, C.c_int64, _f64p, C.c_int64, _f64p, C.c_int]
         L.oracle_soup_build.restype = C.c_void_p
         L.oracle_soup_build.argtypes = [_f32p, C.c_int32, _i32p, C.c_int32, C.c_int]
+        L.oracle_soup_build_nodes.restype = C.c_void_p
+        L.oracle_soup_build_nodes.argtypes = [_f32p, C.c_int32, _i32p, C.c_int32, C.c_int, _u32p, C.c_int32]
         L.oracle_soup_free.argtypes = [C.c_void_p]
         L.oracle_soup_num_nodes.restype = C.c_int32
         L.oracle_soup_num_nodes.argtypes = [C.c_void_p]
@@ -69,6 +71,8 @@ def lib():
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp]
             f = getattr(L, f"oracle_cloud_eval_{sfx}")
             f.argtypes = [fp, fp, fp, C.POINTER(OctreeT), C.c_int, fp, fp, fp, fp, C.c_int64, C.c_double, fp, C.c_int]
+            f = getattr(L, f"oracle_cloud_count_{sfx}")
+            f.argtypes = [C.POINTER(OctreeT), fp, fp, fp, C.c_double] + [C.POINTER(C.c_int64)] * 3
         L.oracle_octree_release.argtypes = [C.POINTER(OctreeT)]
         L.oracle_curve_build.restype = C.c_void_p
         L.oracle_curve_build.argtypes = [_f32p, C.c_int32, _i32p, C.c_int32, C.c_int]
@@ -256,10 +260,17 @@ def solid_angle(a, b, c, p):
 class Soup:
     """Oracle UT_SolidAngle<float,float> (restated)."""
 
-    def __init__(self, V, F, order=2):
+    def __init__(self, V, F, order=2, nodes=None):
+        """nodes (n x 4 uint32, the reference's encoding): build the moments
+        over that tree instead of the reference's own build."""
         self.U = _c(V, np.float32)
         self.F = _c(F, np.int32)
-        self.h = lib().oracle_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
+        if nodes is None:
+            self.h = lib().oracle_soup_build(self.U, self.U.shape[0], self.F, self.F.shape[0], order)
+        else:
+            self._nodes = _c(nodes, np.uint32).reshape(-1, 4)
+            self.h = lib().oracle_soup_build_nodes(self.U, self.U.shape[0], self.F, self.F.shape[0], order,
+                                                   self._nodes, self._nodes.shape[0])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -531,6 +542,14 @@ class Cloud:
                                                         self.CM, self.R, self.EC, Q, Q.shape[0], float(beta), W,
                                                         nthreads)
         return W
+
+    def count(self, q, beta=2.0):
+        """(distance tests, dipole terms, expansions) of one query's walk."""
+        q = _c(q, self.dt).reshape(3)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        getattr(lib(), f"oracle_cloud_count_{self.sfx}")(C.byref(self.oct.t), self.CM, self.R, q, float(beta),
+                                                         C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
 
 # ---------------- knn, point areas ----------------

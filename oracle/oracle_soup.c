@@ -777,7 +777,22 @@ static void pc_traverse(const oracle_soup* s, uint32_t nodei, localdata_t* out) 
   pc_post(s, nodei, out, sidx, local);
 }
 
+static oracle_soup* soup_build(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order,
+                               const uint32_t* nodes, int32_t nnodes);
+
 oracle_soup* oracle_soup_build(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order) {
+  return soup_build(V, nv, F, nf, order, NULL, 0);
+}
+
+/* The moments (UT_SolidAngle::init's precompute, 6644-7118) over a given
+ * 4-wide tree in the reference's node encoding, e.g. a device-built LBVH. */
+oracle_soup* oracle_soup_build_nodes(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order,
+                                     const uint32_t* nodes, int32_t nnodes) {
+  return soup_build(V, nv, F, nf, order, nodes, nnodes);
+}
+
+static oracle_soup* soup_build(const float* V, int32_t nv, const int32_t* F, int32_t nf, int order,
+                               const uint32_t* nodes, int32_t nnodes) {
   oracle_soup* s = (oracle_soup*)calloc(1, sizeof(oracle_soup));
   s->nv = nv;
   s->nf = nf;
@@ -800,7 +815,13 @@ oracle_soup* oracle_soup_build(const float* V, int32_t nv, const int32_t* F, int
       }
     }
   }
-  s->nnodes = oracle_bvh4_build((const float*)s->tri_boxes, (uint32_t)nf, 3, (uint32_t**)&s->nodes);
+  if (nodes) {
+    s->nnodes = (uint32_t)nnodes;
+    s->nodes = (onode*)malloc(sizeof(onode) * (size_t)(nnodes > 0 ? nnodes : 1));
+    memcpy(s->nodes, nodes, sizeof(onode) * (size_t)nnodes);
+  } else {
+    s->nnodes = oracle_bvh4_build((const float*)s->tri_boxes, (uint32_t)nf, 3, (uint32_t**)&s->nodes);
+  }
   s->data = (boxdata_t*)calloc(s->nnodes > 0 ? s->nnodes : 1, sizeof(boxdata_t));
   if (s->nnodes > 0) {
     localdata_t root;

@@ -25,6 +25,19 @@ def test_lbvh_nodes_bit_exact_vs_cpu_build(level, nb, seed):
     assert np.array_equal(np.sort(items), np.arange(F.shape[0]))
 
 
+def test_lbvh_moments_equal_oracle_over_same_nodes():
+    """The moment precompute over the device-built tree equals the oracle's
+    restatement of the reference's precompute over the same node array, bit
+    for bit; W over it agrees with the oracle's walk of that tree."""
+    V, F = D.bumpy_sphere(4, 16, 8)
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2, gwn.TREE_LBVH)
+    o = O.Soup(V, F, nodes=bvh.nodes())
+    assert np.array_equal(bvh.boxdata().view(np.uint32), o.boxdata().view(np.uint32))
+    Q = np.random.default_rng(4).uniform(-1.3, 1.3, (20000, 3))
+    for beta in (2.0, 4.9):
+        assert np.abs(bvh.query(Q, beta) - o.query(Q, beta)).max() < 1e-5
+
+
 def test_lbvh_duplicate_codes_and_nonfinite():
     # many triangles share a centre (equal Morton codes) and some are dropped
     V, F = D.noisy_sphere(2, 0.01, 5)

@@ -200,6 +200,21 @@ def test_cloud_taylor_terms_closed_form():
     assert e[2] < 0.1 * e[1] and e[1] < 0.5 * e[0]
 
 
+def test_cloud_count_closed_forms():
+    """The instrumented cloud walk: at an enormous beta every point is a
+    direct dipole term and no expansion is used; far outside the cloud the
+    root's non-empty children are tested and expanded, nothing else."""
+    rng = np.random.default_rng(9)
+    P = rng.standard_normal((3000, 3))
+    N = P / np.linalg.norm(P, axis=1)[:, None]
+    A = np.full(3000, 4 * np.pi / 3000)
+    c = O.Cloud(P, N, A, 2, np.float64)
+    t, d, e = c.count(np.array([0.1, 0.2, 0.3]), 1e12)
+    assert d == 3000 and e == 0
+    t, d, e = c.count(np.array([1e4, 0.0, 0.0]), 2.0)
+    assert d == 0 and e == t and 1 <= t <= 8
+
+
 def test_cloud_large_beta_equals_brute_force():
     V, F = D.noisy_sphere(2, 0.01, 3)
     rng = np.random.default_rng(5)
