@@ -25,6 +25,14 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving row runs SERVING_SLOTS HIP streams, one per slot.  HIP maps
+# streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by default);
+# slots that share a queue run one after the other, so a long kernel of one
+# holds back the other (measured on the serving trace at accuracy_scale 4.9,
+# 8 slots: p99 26-34 ms on 4 queues, 10-14 ms on 8).  Read when the process
+# first uses HIP, so it is set before anything here does.
+SERVING_SLOTS = 8
+os.environ["GPU_MAX_HW_QUEUES"] = str(SERVING_SLOTS)
 
 METRIC = "winding queries/s at max |dw|<=1e-4 vs FP64 oracle"
 ACC_TARGET = 1e-4
@@ -396,7 +404,8 @@ def run_boundary(bvh, Q, beta):
         dt = (time.perf_counter() - t) / reps
         nat.lib().gwn_profile_enable(0)
         out[f"nq_{n}"] = {"call_ms": round(1e3 * dt, 4), "queries_per_s": round(n / dt, 1),
-                          "traverse_kernel_ms": round(_kernel_ms("soup_traverse"), 4)}
+                          "walk_kernel_ms": round(_kernel_ms("soup_traverse") + _kernel_ms("soup_walk_ring"), 4),
+                          "walk": "one wavefront per query" if n < 65536 else "wave-coherent"}
     per, nth = 250, 8
     Wt = np.zeros(per * nth)
     errs = []
@@ -1047,7 +1056,7 @@ def run_serving(duration, beta_acc):
 
     t0 = time.time()
     meshes = D.serving_meshes()
-    srv = Server(slots=8, device_budget_bytes=64 << 30)
+    srv = Server(slots=SERVING_SLOTS, device_budget_bytes=64 << 30)
     ids = [srv.add_mesh(V, F, 2, gwn.TREE_SAH4) for V, F in meshes]
     srv.reserve(1_000_000)
     rng = np.random.default_rng(77)
@@ -1107,7 +1116,8 @@ def run_serving(duration, beta_acc):
     res["cpu_baseline"] = cpu
     resident, held, budget = srv.memory()
     srv.close()
-    res.update({"meshes": 64, "resident_triangles": int(ntris), "resident_bytes": int(resident), "slots": 8,
+    res.update({"meshes": 64, "resident_triangles": int(ntris), "resident_bytes": int(resident), "slots": SERVING_SLOTS,
+                "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
                 "slot_buffer_bytes": int(held), "lambda_per_s": 2000.0, "duration_s": duration,
                 "precision": "mixed"})
     return res

@@ -89,7 +89,8 @@ const char* gwn_version(void);
 
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
- * "soup_traverse64", "soup_reeval64", "exact", "exact_tree", "cloud_traverse", "radix_sort",
+ * "soup_traverse64", "soup_reeval64", "soup_walk_ring" (smaller batches, see gwn_soup_query), "exact",
+ * "exact_tree", "cloud_traverse", "radix_sort",
  * "closest_point", "knn", "point_areas", "cloud_brute", "exact2d",
  * "curve_traverse") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises
@@ -167,9 +168,16 @@ int gwn_soup_get_info(const gwn_soup* s, gwn_soup_info* info);
 /* Replaces igl::fast_winding_number(fwn_bvh, accuracy_scale, Q, W),
  * fast_winding_number.h:191-196 (fast_winding_number.cpp:418-437): each query
  * is narrowed to float (on the device), W = Omega / (4 pi) in double.  The
- * single-point overload (fast_winding_number.h:203-206) is nq = 1: batches
- * under 4,096 queries skip the query sort and walk the tree one lane per
- * query, so a single point costs one kernel and two small copies. */
+ * single-point overload (fast_winding_number.h:203-206) is nq = 1.  Batches
+ * under 65,536 queries skip the query sort and give each query a wavefront of
+ * its own, whose lanes share the query's walk (a query takes about tree-depth
+ * many dependent fetches instead of path-length many), so a single point
+ * costs one kernel and two small copies.  Larger batches are sorted along a
+ * Hilbert curve and walked wave-coherently.  The two walks visit the same
+ * records and sum the same terms in different orders: results agree to FP32
+ * rounding (FP64 rounding in the FP64 form), and each is deterministic,
+ * depending only on the query, the tree, the precision and which side of
+ * 65,536 the batch size falls. */
 int gwn_soup_query(const gwn_soup* s, const double* Q, int64_t nq, float accuracy_scale, int precision,
                    double* W);
 /* Device form: Q float nq x 3, W double nq. */
@@ -427,7 +435,10 @@ void gwn_free(void* p);
  * one chunk of at most 1,048,576 queries, doubled for larger requests, which run
  * as a pipeline of such chunks) fits in the remaining budget.
  * Admitted requests run concurrently on different streams, so their kernels
- * overlap on the device.  A request's result is written to the caller's W
+ * overlap on the device; HIP maps a process's streams onto GPU_MAX_HW_QUEUES
+ * hardware queues (4 unless set before the process first uses HIP), and slots
+ * that share a queue run one after the other, so set it to at least `slots`.
+ * A request's result is written to the caller's W
  * (which must stay valid until the request is done) and is bit-identical to
  * gwn_soup_query on the same mesh and queries.  Timestamps are seconds on a
  * steady clock shared by all requests of a server.

@@ -238,12 +238,25 @@ void curve_build_device(gwn_curve* c, hipStream_t st);
 void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
                         hipStream_t st);
 // traversal kernel: WALK_WAVE (wave-coherent, sorted batch queries),
-// WALK_LANE (per-lane over the sorted queries, used by the serving engine for
-// small chunks), WALK_LANE_UNSORTED (per-lane in input order: no sort, a
-// workspace of 4 nq + 4 bytes) or WALK_WAVE_HANDOVER (wave-coherent until a
-// wavefront's lanes diverge, then per-lane: shorter batches on large trees
-// with thin-triangle fans, slower on coherent ones); bitwise the same results
-constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2, WALK_WAVE_HANDOVER = 3;
+// WALK_LANE (per-lane over the sorted queries), WALK_LANE_UNSORTED (per-lane
+// in input order: no sort, a workspace of 4 nq + 4 bytes) or
+// WALK_WAVE_HANDOVER (wave-coherent until a wavefront's lanes diverge, then
+// per-lane: shorter batches on large trees with thin-triangle fans, slower on
+// coherent ones) give bitwise the same results (sequential sums in rope
+// order).  WALK_RING (one wavefront per query, in input order, no workspace)
+// sums per lane and then across lanes: deterministic, and within rounding of
+// the sequential walks.
+constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2, WALK_WAVE_HANDOVER = 3, WALK_RING = 4;
+constexpr int64_t kRingMaxQueries = 65536;
+// Walk of a batch of nq queries on tree s, used by every entry point and by
+// the serving engine (on the whole request), so that a batch's result does not
+// depend on the path it took: the ring walk below kRingMaxQueries, the
+// wave-coherent walk above; latency_bound callers get its hand-over form on
+// trees of 2^18+ slots.  (The ring walk, over Hilbert-sorted queries, also
+// finishes batches of up to ~500k queries sooner on large trees when run
+// alone, but it keeps the whole GPU busy for it: under the serving load it
+// cost throughput and latency, p50 30 ms against 11 ms at scale 4.9.)
+int soup_walk_for(const gwn_soup* s, int64_t nq, bool latency_bound);
 // flags of soup_query_device_ws: WS_DEVICE_COUNT sizes the mixed form's FP64
 // re-evaluation on the device (no host round trip); WS_NO_PROFILE creates no
 // kernel-timing events

@@ -210,16 +210,18 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)cn));
     uint8_t* ws = base + a256(12 * (size_t)cn) + a256(8 * (size_t)cn);
     GWN_HIP(hipMemcpyAsync(dQ, hQ, 12 * (size_t)n, hipMemcpyHostToDevice, sl->stream));
-    // Small chunks take the per-lane walk (their queries are too sparse for
-    // wavefront coherence); large ones the wave-coherent walk, whose
-    // throughput is the higher one there (measured on the mix above: at scale
-    // 5 and 1000 requests/s, p99 15 ms against 42 ms with the per-lane walk
-    // everywhere).  On trees of 2^18+ slots the wave walk hands over to
-    // per-lane walks where a wavefront's lanes diverge: a request is done when
-    // its slowest wavefront is, and on a 434k-triangle UV sphere (fans of thin
-    // triangles at the poles) that cut a 1M-query batch at scale 4.9 from 8.0
-    // to 5.2 ms.  All walks give bitwise the same result.
-    const int walk = n < 65536 ? WALK_LANE : m->nslots >= (1 << 18) ? WALK_WAVE_HANDOVER : WALK_WAVE;
+    // The walk is chosen on the whole request as a standalone query of the
+    // same batch chooses it, so the result is the standalone one.  Smaller
+    // requests take the walk with one wavefront per query: sequential walks
+    // cost a small request its longest path whatever its size, and with
+    // per-lane walks requests of a few thousand queries held a slot for
+    // 1.5-2 ms at scale 4.9 while the big requests queued behind them.
+    // Larger requests take the wave-coherent walk, here in its hand-over form
+    // on trees of 2^18+ slots: a request is done when its slowest wavefront
+    // is, and on a 434k-triangle UV sphere (fans of thin triangles at the
+    // poles) that cut a 1M-query batch at scale 4.9 from 8.0 to 5.2 ms, with
+    // bitwise the same result.
+    const int walk = soup_walk_for(m, r->nq, true);
     soup_query_device_ws(m, dQ, n, r->beta, r->prec, dW, sl->stream, ws, soup_query_work_bytes(cn), walk);
     GWN_HIP(hipMemcpyAsync(sl->hW[b], dW, 8 * (size_t)n, hipMemcpyDeviceToHost, sl->stream));
     GWN_HIP(hipEventRecord(sl->ev[b], sl->stream));

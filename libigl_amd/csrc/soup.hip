@@ -862,8 +862,94 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// list[k] for k < count (count = *dcount when given, else n): 4 queries per
-// 256-thread block, one per wavefront, grid-strided.
+// LDS writes of the wavefront's lanes become visible to all of them
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The ring walk of one query by the calling wavefront (ring: its kReevalRing
+// entries of LDS).  visit(i, link) evaluates record i for the query, adds its
+// term (if any) to the lane's own partial sums, sets link to the record's link
+// word and returns false when the walk descends into the record.
+template <typename Visit>
+__device__ __forceinline__ void ring_walk(uint2* __restrict__ ring, int lane, uint32_t nslots, Visit&& visit) {
+  if (lane == 0) ring[0] = make_uint2(0u, nslots);
+  uint32_t head = 0, tail = 1;  // wave-uniform ring positions
+  wave_lds_sync();
+  while (head != tail) {
+    const uint32_t take = min(tail - head, 64u);
+    const bool have = (uint32_t)lane < take;
+    uint2 it = make_uint2(0u, 0u);
+    if (have) it = ring[(head + lane) % kReevalRing];
+    head += take;
+    uint2 push0 = make_uint2(0u, 0u), push1 = make_uint2(0u, 0u);
+    uint32_t np = 0;
+    if (have) {
+      const uint32_t i = it.x;
+      uint32_t link;
+      const bool done = visit(i, link);
+      const uint32_t next = (link & LINK_LEAF) ? i + 1 : link;
+      if (!done) push0 = make_uint2(i + 1, link), np = 1;  // child range
+      if (next < it.y) {
+        if (np) push1 = make_uint2(next, it.y);
+        else push0 = make_uint2(next, it.y);
+        ++np;
+      }
+    }
+    const uint64_t b0 = __ballot(np & 1u), b1 = __ballot(np >> 1);
+    const uint32_t below = lanes_below(b0) + 2u * lanes_below(b1);
+    const uint32_t room = kReevalRing - (tail - head);
+    const bool fits = below + np <= room;
+    if (fits) {
+      if (np > 0) ring[(tail + below) % kReevalRing] = push0;
+      if (np > 1) ring[(tail + below + 1) % kReevalRing] = push1;
+    } else {
+      // ring full: walk the ranges sequentially (rope order)
+      for (uint32_t m = 0; m < np; ++m) {
+        const uint2 rg = m ? push1 : push0;
+        uint32_t i = rg.x;
+        while (i < rg.y) {
+          uint32_t link;
+          const bool done = visit(i, link);
+          i = (done && !(link & LINK_LEAF)) ? link : i + 1;
+        }
+      }
+    }
+    // lanes that fit are a prefix in lane order: their ranges are contiguous
+    tail += (uint32_t)__popcll(__ballot(fits && (np & 1u))) + 2u * (uint32_t)__popcll(__ballot(fits && (np >> 1)));
+    wave_lds_sync();
+  }
+}
+
+// sum of the lanes' partial sums in a fixed order; every lane gets the same value
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = __dadd_rn(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v = __fadd_rn(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// FP64 ring walk of query q: W = sum / 4 pi, the same on every lane
+template <int ORDER>
+__device__ __forceinline__ double ring_walk64(uint2* __restrict__ ring, int lane, const float4* __restrict__ slots,
+                                              const double* __restrict__ s64, uint32_t nslots,
+                                              const float* __restrict__ tri, double qx, double qy, double qz,
+                                              double acc2) {
+  double part = 0.0;
+  ring_walk(ring, lane, nslots, [&](uint32_t i, uint32_t& link) {
+    link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+    return record_term64<ORDER>(s64, tri, i, link, qx, qy, qz, acc2, part);
+  });
+  return wave_sum(part) / (4.0 * PI_D);
+}
+
+// list[k] (k itself when list is null) for k < count (count = *dcount when
+// given, else n): 4 queries per 256-thread block, one per wavefront,
+// grid-strided.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_reeval64(const float4* __restrict__ slots, const double* __restrict__ s64,
                                                   uint32_t nslots, const float* __restrict__ tri,
@@ -872,64 +958,116 @@ __global__ __launch_bounds__(256) void k_reeval64(const float4* __restrict__ slo
                                                   double* __restrict__ W) {
   __shared__ uint2 ring_all[4][kReevalRing];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint2* ring = ring_all[wv];
   const int64_t count = dcount ? (int64_t)*dcount : n;
   for (int64_t k = (int64_t)blockIdx.x * 4 + wv; k < count; k += (int64_t)gridDim.x * 4) {
-    const int64_t qi = list[k];
-    const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
-    double part = 0.0;
-    if (lane == 0) ring[0] = make_uint2(0u, nslots);
-    uint32_t head = 0, tail = 1;  // wave-uniform ring positions
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    while (head != tail) {
-      const uint32_t take = min(tail - head, 64u);
-      const bool have = (uint32_t)lane < take;
-      uint2 it = make_uint2(0u, 0u);
-      if (have) it = ring[(head + lane) % kReevalRing];
-      head += take;
-      uint2 push0 = make_uint2(0u, 0u), push1 = make_uint2(0u, 0u);
-      uint32_t np = 0;
-      if (have) {
-        const uint32_t i = it.x;
-        const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
-        const bool done = record_term64<ORDER>(s64, tri, i, link, qx, qy, qz, acc2, part);
-        const uint32_t next = (link & LINK_LEAF) ? i + 1 : link;
-        if (!done) push0 = make_uint2(i + 1, link), np = 1;  // child range
-        if (next < it.y) {
-          if (np) push1 = make_uint2(next, it.y);
-          else push0 = make_uint2(next, it.y);
-          ++np;
-        }
-      }
-      const uint64_t b0 = __ballot(np & 1u), b1 = __ballot(np >> 1);
-      const uint32_t below = lanes_below(b0) + 2u * lanes_below(b1);
-      const uint32_t room = kReevalRing - (tail - head);
-      const bool fits = below + np <= room;
-      if (fits) {
-        if (np > 0) ring[(tail + below) % kReevalRing] = push0;
-        if (np > 1) ring[(tail + below + 1) % kReevalRing] = push1;
-      } else {
-        // ring full: walk the ranges sequentially (rope order)
-        for (uint32_t m = 0; m < np; ++m) {
-          const uint2 rg = m ? push1 : push0;
-          uint32_t i = rg.x;
-          while (i < rg.y) {
-            const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
-            const bool done = record_term64<ORDER>(s64, tri, i, link, qx, qy, qz, acc2, part);
-            i = (done && !(link & LINK_LEAF)) ? link : i + 1;
+    const int64_t qi = list ? (int64_t)list[k] : k;
+    const double w = ring_walk64<ORDER>(ring_all[wv], lane, slots, s64, nslots, tri, Q[3 * qi], Q[3 * qi + 1],
+                                        Q[3 * qi + 2], acc2);
+    if (lane == 0) W[qi] = w;
+  }
+}
+
+// ---- small batches: one wavefront per query for the whole query ----
+// A batch of a few thousand queries fills few wavefronts with the lane walk,
+// and each lane's sequential walk is a chain of dependent record fetches as
+// long as its path: at accuracy_scale 4.9 on a 434k-triangle mesh a batch of
+// 1,000 queries takes 1.4 ms however few queries it has.  Here each query gets
+// a wavefront and the ring walk above, so a query takes depth-many rounds.
+// The records visited and each term are those of the sequential walk (FP32
+// decisions and terms with the reference's rounding, FP64 near triangles in
+// the mixed form); the sum is taken per lane and then across lanes in a fixed
+// order, so results are deterministic and differ from the sequential walks'
+// running sums only by rounding.  In the mixed form k_reeval64_band then
+// walks the queries whose W falls in the ambiguous band again in FP64, so the
+// batch needs no collection pass and no count.
+// Mixed form: a lane notes the near triangles it meets (up to kRingDefer, in
+// LDS) and sums their FP64 solid angles after the walk, keeping the FP64 code
+// out of the walk's registers; a query with more than kRingDefer on one lane
+// gets W = NaN, which k_reeval64_band walks again in FP64.
+constexpr int kRingDefer = 4;
+
+// FP32 and mixed forms (the FP64 form is k_reeval64 over every query).  Held
+// to 8 wavefronts per SIMD: the walk fits in 64 VGPRs; the mixed form's FP64
+// tail would take more and keeps some of its values in scratch instead.
+template <int ORDER, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_walk_ring(
+    const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
+    int64_t nq, float acc2f, float near_d, double* __restrict__ W) {
+  __shared__ uint2 ring_all[4][kReevalRing];
+  __shared__ uint32_t near_all[MIXED ? 4 * kRingDefer * 64 : 1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint2* ring = ring_all[wv];
+  uint32_t* near = near_all + (MIXED ? wv * kRingDefer * 64 + lane : 0);  // this lane's column
+  for (int64_t qi = (int64_t)blockIdx.x * 4 + wv; qi < nq; qi += (int64_t)gridDim.x * 4) {
+    const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
+    double w;
+    {
+      float p32 = 0.0f;
+      uint32_t nn = 0;  // near triangles noted (kRingDefer + 1 after an overflow)
+      ring_walk(ring, lane, nslots, [&](uint32_t i, uint32_t& link) {
+        const float4* r = slots + 6 * (size_t)i;
+        // the whole record in one round trip (loading the far-field part only
+        // where it is used measured no faster)
+        const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4], f = r[5];
+        link = __float_as_uint(b.w);
+        const bool leaf = (link & LINK_LEAF) != 0;
+        const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+        // rounded as the reference rounds it (no fused multiply-add)
+        const float l2 = __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+        if (!(l2 <= __fmul_rn(a.w, acc2f))) {
+          const float o = far_field_poly<ORDER>(x, y, z, l2, b, c, d, e, f);
+          if (isfinite(o)) {
+            p32 = __fadd_rn(p32, o);
+            return true;
           }
         }
+        if (!leaf) return false;
+        const uint32_t t = link & ~LINK_LEAF;
+        if (MIXED && near_triangle(a.w, l2, near_d)) {
+          if (nn < (uint32_t)kRingDefer) near[nn * 64] = t;
+          nn = nn < (uint32_t)kRingDefer ? nn + 1 : (uint32_t)kRingDefer + 1;
+        } else {
+          const float* p = tri + 9 * (size_t)t;
+          p32 = __fadd_rn(p32, tri_solid_angle_fast(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
+                                                    qz));
+        }
+        return true;
+      });
+      const float s32 = wave_sum(p32);
+      if (!MIXED) {
+        w = (double)s32 / (4.0 * PI_D);
+      } else if (__ballot(nn > (uint32_t)kRingDefer) != 0) {
+        w = __longlong_as_double(0x7FF8000000000000ll);
+      } else {
+        double p64 = 0.0;
+        for (uint32_t k = 0; k < nn; ++k) {
+          const float* p = tri + 9 * (size_t)near[k * 64];
+          p64 = __dadd_rn(p64, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy,
+                                                       qz));
+        }
+        w = ((double)s32 + wave_sum(p64)) / (4.0 * PI_D);
       }
-      // lanes that fit are a prefix in lane order: their ranges are contiguous
-      tail += (uint32_t)__popcll(__ballot(fits && (np & 1u))) + 2u * (uint32_t)__popcll(__ballot(fits && (np >> 1)));
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    for (int off = 32; off >= 1; off >>= 1) part = __dadd_rn(part, __shfl_xor(part, off, 64));
-    if (lane == 0) W[qi] = part / (4.0 * PI_D);
+    if (lane == 0) W[qi] = w;
+  }
+}
+
+// Mixed form after k_walk_ring: queries whose W falls in the ambiguous band
+// are walked again in FP64, one wavefront per query (the others exit at once).
+// A kernel of its own so that the FP64 walk's registers do not limit the
+// occupancy of the FP32 walk.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_reeval64_band(const float4* __restrict__ slots,
+                                                       const double* __restrict__ s64, uint32_t nslots,
+                                                       const float* __restrict__ tri, const float* __restrict__ Q,
+                                                       int64_t nq, double acc2, double band, double* __restrict__ W) {
+  __shared__ uint2 ring_all[4][kReevalRing];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t qi = (int64_t)blockIdx.x * 4 + wv; qi < nq; qi += (int64_t)gridDim.x * 4) {
+    if (fabs(W[qi] - 0.5) >= band) continue;  // wave-uniform; NaN (near-list overflow) is walked again
+    const double w = ring_walk64<ORDER>(ring_all[wv], lane, slots, s64, nslots, tri, Q[3 * qi], Q[3 * qi + 1],
+                                        Q[3 * qi + 2], acc2);
+    if (lane == 0) W[qi] = w;
   }
 }
 
@@ -1230,18 +1368,21 @@ static uint8_t* noalloc_scratch() {
 
 int64_t soup_noalloc_max_queries() { return kNoAllocMaxQueries; }
 
-// Batches below this size are walked per lane in input order: the Hilbert
-// sort (a key kernel and four radix passes of three kernels) would cost more
-// than the coherence it buys.
-constexpr int64_t kSortMin = 4096;
+int soup_walk_for(const gwn_soup* s, int64_t nq, bool latency_bound) {
+  if (nq < kRingMaxQueries) return WALK_RING;
+  return latency_bound && s->nslots >= (1 << 18) ? WALK_WAVE_HANDOVER : WALK_WAVE;
+}
 
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                        double* W, hipStream_t st) {
   if (nq <= 0) return;
-  const bool sorted = nq >= kSortMin;
-  Scratch ws(sorted ? soup_query_work_bytes(nq) : soup_small_work_bytes(nq), st);
-  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws.p, ws.capacity,
-                       sorted ? WALK_WAVE : WALK_LANE_UNSORTED);
+  const int walk = soup_walk_for(s, nq, false);
+  if (walk == WALK_RING) {
+    soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, nullptr, 0, WALK_RING);
+    return;
+  }
+  Scratch ws(soup_query_work_bytes(nq), st);
+  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws.p, ws.capacity, walk);
 }
 
 // The scratch-free form of soup_query_device: same kernels, same results;
@@ -1252,10 +1393,14 @@ void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, fl
   GWN_CHECK(nq <= kNoAllocMaxQueries, GWN_ERR_INVALID,
             "scratch-free query: at most 1,048,576 queries per call (got " + std::to_string(nq) + ")");
   if (nq <= 0) return;
+  const int walk = soup_walk_for(s, nq, false);
+  if (walk == WALK_RING) {
+    soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, nullptr, 0, WALK_RING, WS_NO_PROFILE);
+    return;
+  }
   uint8_t* ws = noalloc_scratch();
-  const bool sorted = nq >= kSortMin;
-  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws, kNoAllocScratchBytes,
-                       sorted ? WALK_WAVE : WALK_LANE_UNSORTED, WS_DEVICE_COUNT | WS_NO_PROFILE);
+  soup_query_device_ws(s, Q, nq, accuracy_scale, precision, W, st, ws, kNoAllocScratchBytes, walk,
+                       WS_DEVICE_COUNT | WS_NO_PROFILE);
 }
 
 // Query with caller-provided device workspace (soup_query_work_bytes(nq)
@@ -1287,12 +1432,55 @@ static void launch_traverse64(const gwn_soup* s, bool wave, unsigned g, hipStrea
   check_launch(wave ? "k_traverse64" : "k_traverse64_lane");
 }
 
+// one wavefront per query (k_walk_ring), 4 per block
+static void launch_walk_ring(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
+                             double* W, hipStream_t st, bool prof_on) {
+  const uint32_t ns = (uint32_t)s->nslots;
+  const float acc2 = accuracy_scale * accuracy_scale;
+  const double a2 = (double)accuracy_scale * (double)accuracy_scale;
+  const float nd = s->mixed.near_dist;
+  const double band = (double)s->mixed.ambiguous_band;
+  const unsigned g = (unsigned)div_up(nq, 4);
+  ProfileScope prof("soup_walk_ring", st, prof_on);
+  if (precision == GWN_PREC_FP64) {
+    switch (s->order) {
+      case 0: hipLaunchKernelGGL(k_reeval64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nullptr, nq, nullptr, a2, W); break;
+      case 1: hipLaunchKernelGGL(k_reeval64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nullptr, nq, nullptr, a2, W); break;
+      default: hipLaunchKernelGGL(k_reeval64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nullptr, nq, nullptr, a2, W); break;
+    }
+    check_launch("k_reeval64");
+    return;
+  }
+#define GWN_LAUNCH_R(O, M)                                                                                      \
+  hipLaunchKernelGGL((k_walk_ring<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, nq, acc2, nd, W)
+  const bool mixed = precision == GWN_PREC_MIXED;
+  switch (s->order) {
+    case 0: if (mixed) GWN_LAUNCH_R(0, true); else GWN_LAUNCH_R(0, false); break;
+    case 1: if (mixed) GWN_LAUNCH_R(1, true); else GWN_LAUNCH_R(1, false); break;
+    default: if (mixed) GWN_LAUNCH_R(2, true); else GWN_LAUNCH_R(2, false); break;
+  }
+#undef GWN_LAUNCH_R
+  check_launch("k_walk_ring");
+  if (mixed) {
+    switch (s->order) {
+      case 0: hipLaunchKernelGGL(k_reeval64_band<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nq, a2, band, W); break;
+      case 1: hipLaunchKernelGGL(k_reeval64_band<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nq, a2, band, W); break;
+      default: hipLaunchKernelGGL(k_reeval64_band<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, nq, a2, band, W); break;
+    }
+    check_launch("k_reeval64_band");
+  }
+}
+
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags) {
   if (nq <= 0) return;
   const bool prof_on = (flags & WS_NO_PROFILE) == 0;
   if (s->nslots == 0) {
     fill_device<double>(W, nq, 0.0, st);
+    return;
+  }
+  if (walk == WALK_RING) {
+    launch_walk_ring(s, Q, nq, accuracy_scale, precision, W, st, prof_on);
     return;
   }
   const bool sorted = walk != WALK_LANE_UNSORTED;

@@ -12,7 +12,9 @@ from ._native import check, lib
 class Server:
     """Resident meshes + continuous batching over `slots` HIP streams within a
     device-memory budget.  submit() returns at once; the result array is
-    filled when wait()/drain() says the request is done."""
+    filled when wait()/drain() says the request is done.  Give the process at
+    least `slots` hardware queues (GPU_MAX_HW_QUEUES, set before it first uses
+    HIP; 4 otherwise): slots that share a queue run one after the other."""
 
     def __init__(self, slots=4, device_budget_bytes=32 << 30):
         h = nat.vp()

@@ -143,6 +143,10 @@ def test_signed_distance_large_batch():
     sub = np.arange(0, Q.shape[0], 997)
     So, Io, Co = O.signed_distance(V, F, Q[sub], gwn.SIGN_FAST_WINDING_NUMBER)
     _check(V, F, Q[sub], S[sub], I[sub], C[sub], So, Io, Co, gwn.SIGN_FAST_WINDING_NUMBER)
-    # a batch of one gives the same answer as the point in the large batch
+    # a batch of one gives the same closest point as the point in the large
+    # batch; S = (1 - 2|w|) d (signed_distance.cpp:121-123) with w from the
+    # fast winding number, whose batches of one and of 1M sum the same terms
+    # in different orders (they agree to FP32 rounding, 1e-5 in w)
     S1, I1, C1 = bvh.signed_distance(Q[12345:12346], gwn.SIGN_FAST_WINDING_NUMBER)
-    assert S1[0] == S[12345] and I1[0] == I[12345] and np.array_equal(C1[0], C[12345])
+    assert I1[0] == I[12345] and np.array_equal(C1[0], C[12345])
+    assert abs(S1[0] - S[12345]) <= 2e-5 * np.linalg.norm(C1[0] - Q[12345])

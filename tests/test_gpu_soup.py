@@ -97,6 +97,36 @@ def test_soup_one_shot_and_single_point_overloads():
     assert isinstance(w0, float) and w0 == W2[0]
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_ring_walk_and_sequential_walks_agree(order):
+    """Batches under 65,536 queries take the walk with one wavefront per
+    query (per-lane partial sums, then a fixed-order sum across lanes); larger
+    ones the wave-coherent walk (one running sum in rope order).  Same records
+    and terms, so the results differ only by FP32 summation rounding (FP64
+    rounding for the FP64 form and for the mixed form's queries near 0.5)."""
+    V, F = D.bumpy_sphere(4, 16, 20 + order)
+    rng = np.random.default_rng(order)
+    Q = np.concatenate([rng.uniform(-1.4, 1.4, (40000, 3)), D.near_surface_queries(V, F, 30000, 1e-3, rng)])
+    bvh = gwn.FastWindingNumberBVH().build(V, F, order)
+    ref = _ref_soup(V, F, order)
+    for beta in (2.0, 4.9):
+        Wr = ref.query(Q, beta)
+        for prec, tol in ((gwn.PREC_FP32, 1e-5), (gwn.PREC_MIXED, 1e-5), (gwn.PREC_FP64, 1e-12)):
+            Wbig = bvh.query(Q, beta, prec)  # 70,000 queries: wave-coherent walk
+            Wsmall = np.concatenate([bvh.query(Q[k:k + 20000], beta, prec) for k in range(0, Q.shape[0], 20000)])
+            d = np.abs(Wbig - Wsmall)
+            print(f"order {order} beta {beta} prec {prec}: max |ring - wave| {d.max():.2e}")
+            assert d.max() <= tol
+            if prec == gwn.PREC_MIXED:
+                assert np.array_equal(Wbig > 0.5, Wsmall > 0.5)
+            if prec == gwn.PREC_FP32:
+                for W in (Wbig, Wsmall):
+                    assert np.abs(W - Wr).max() <= TOL_REF
+                    assert _classify_same(W, Wr)
+            # a batch's result does not depend on how it was split
+            assert np.array_equal(bvh.query(Q[:20000], beta, prec), Wsmall[:20000])
+
+
 def test_soup_edge_cases():
     V, F = D.noisy_sphere(2, 0.01, 5)
     # empty mesh: W = 0 (the reference leaves it uninitialised)
