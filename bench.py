@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-rows", action="store_true", help="skip the cloud / exact / signed-distance legs")
     ap.add_argument("--rows-only", action="store_true",
                     help="run only the rows named by --rows and print them as one JSON line (profiling runs)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="run only the 1.31M-triangle FP32 / mixed / FP64 comparison and print it (profiling runs)")
     ap.add_argument("--rows", default="cloud,exact,signed_distance,point_areas,antipodal,curve",
                     help="comma-separated rows reported beside the headline")
     ap.add_argument("--serving-duration", type=float, default=2.0, help="seconds of Poisson trace (0: skip)")
@@ -84,6 +86,7 @@ def main():
     ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
                     help="measure the headline kernel's HBM bytes with two rocprofv3 counter passes at the end")
     ap.add_argument("--no-boundary", action="store_true", help="skip the host-array call timings")
+    ap.add_argument("--no-lbvh", action="store_true", help="skip the GPU-built (LBVH) tree on the headline workload")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         import numpy as np
@@ -113,6 +116,10 @@ def main():
     if args.rows_only:
         dm.synchronize()
         print(json.dumps({"rows": run_rows(args.rows, args.no_cpu)}), flush=True)
+        return
+    if args.mixed_only:
+        dm.synchronize()
+        print(json.dumps({"mixed_vs_fp64": run_mixed_workload(BETA_GRID)}, default=float), flush=True)
         return
     t0 = time.time()
     V, F, Q = D.soup_workload(nq=args.nq)
@@ -300,6 +307,13 @@ def main():
         except Exception as e:
             log(f"[bench] boundary timing skipped: {e}")
 
+    tree_lbvh = None
+    if args.tree == "sah4" and not args.no_lbvh:
+        try:
+            tree_lbvh = run_lbvh_tree(V, F, Qf, dQ, nq, Wx, qps)
+        except Exception as e:
+            log(f"[bench] LBVH row skipped: {e}")
+
     mixed_1m_tri = None
     if not args.no_mixed:
         try:
@@ -349,7 +363,8 @@ def main():
         "fp64": {"queries_per_s": round(nq / (dtd / max(3, args.steps // 4)), 1), "max_abs_dw": err_fp64,
                  "kernel_ms": round(kd_ms, 4)},
         "build_s": round(build_s, 4), "build_warm_s": round(build_warm_s, 4), "sort_ms": round(sort_ms, 4),
-        "roofline": roofline, "cpu_baseline": cpu, "host": host, "boundary": boundary, "serving": serving,
+        "roofline": roofline, "cpu_baseline": cpu, "host": host, "boundary": boundary, "tree_lbvh": tree_lbvh,
+        "serving": serving,
         "mixed_vs_fp64": mixed_1m_tri, "other_rows": others,
     }
     print(json.dumps(out), flush=True)
@@ -380,6 +395,72 @@ def cpu_threads(args):
     if args.cpu_threads:
         return args.cpu_threads
     return max(1, min(16, host_info()["nproc"] or 1))
+
+
+def run_lbvh_tree(V, F, Qf, dQ, nq, Wx, qps_sah4):
+    """The headline workload on the GPU-built tree (Morton LBVH, 4-wide
+    collapse, moments by the same upward pass): build time, the smallest
+    accuracy_scale on the grid meeting 1e-4 against the FP64 sum (the
+    truncation error depends on the tree), and queries/s there, beside SAH4."""
+    import numpy as np
+
+    from libigl_amd import devmem as dm
+    from libigl_amd import gwn
+    from libigl_amd import _native as nat
+    st = _stream()
+    b = gwn.FastWindingNumberBVH()
+    b.build(V, F, 2, gwn.TREE_LBVH)  # first build loads the code paths
+    dm.synchronize()
+    tb = time.perf_counter()
+    b = gwn.FastWindingNumberBVH()
+    b.build(V, F, 2, gwn.TREE_LBVH)
+    dm.synchronize()
+    build_s = time.perf_counter() - tb
+    info = b.info()
+    dW = dm.DeviceArray((nq,), np.float64)
+    beta, err = None, None
+    for bt in BETA_GRID:
+        b.query_device(dQ.data_ptr, nq, dW.data_ptr, bt, gwn.PREC_FP32, st.ptr)
+        dm.synchronize()
+        e = float(np.abs(dW.numpy() - Wx).max())
+        if e <= ACC_TARGET:
+            beta, err = bt, e
+            break
+    if beta is None:
+        return {"note": "no accuracy_scale on the grid meets 1e-4"}
+    for _ in range(2):
+        b.query_device(dQ.data_ptr, nq, dW.data_ptr, beta, gwn.PREC_FP32, st.ptr)
+    dm.synchronize()
+    nat.lib().gwn_profile_reset()
+    nat.lib().gwn_profile_enable(1)
+    steps = 10
+    t = time.perf_counter()
+    for _ in range(steps):
+        b.query_device(dQ.data_ptr, nq, dW.data_ptr, beta, gwn.PREC_FP32, st.ptr)
+    dm.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    nat.lib().gwn_profile_enable(0)
+    out = {"tree": "LBVH (GPU Morton codes, LDS radix sort, Karras hierarchy, 4-wide collapse)",
+           "build_s": round(build_s, 4), "nodes": int(info.num_nodes), "slots": int(info.num_slots),
+           "depth": int(info.depth), "accuracy_scale": beta, "max_abs_dw": err,
+           "queries_per_s": round(nq / dt, 1), "traverse_ms": round(_kernel_ms("soup_traverse"), 4),
+           "vs_sah4": round(nq / dt / qps_sah4, 3)}
+    try:  # counted work on the LBVH itself (the oracle's walk over its node array)
+        import oracle_py as O
+        so = O.Soup(V, F, nodes=b.nodes())
+        samp = np.arange(0, nq, max(1, nq // 2000))
+        c = np.array([so.count(Qf[i], beta) for i in samp], np.float64).mean(0)
+        fpq = FLOP_TEST * 4 * c[0] + FLOP_FAR * c[1] + FLOP_EXACT * c[2]
+        out["per_query"] = {"node_visits": round(c[0], 1), "far_slots": round(c[1], 1), "exact_tris": round(c[2], 1),
+                            "flop": round(fpq, 1)}
+        if out["traverse_ms"] > 0:
+            ach = fpq * nq / (out["traverse_ms"] * 1e-3) / 1e12
+            out["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": 157.3, "unit": "TFLOP/s",
+                               "frac": round(ach / 157.3, 4)}
+    except Exception as e:
+        out["per_query"] = f"not counted: {e}"
+    log(f"[bench] LBVH tree: {out}")
+    return out
 
 
 def run_boundary(bvh, Q, beta):

@@ -35,8 +35,9 @@
 //                                                     signed_distance.h:300-311 (one point, .cpp:688-711)
 //   igl::signed_distance_winding_number(tree,V,F,hier,q[,s,sqrd,i,c])
 //                                                     signed_distance.h:211-242 (3D, .cpp:579-629)
-// (signed distance for the unsigned, winding-number and fast-winding-number
-// sign types; the pseudonormal type throws)
+// (signed distance for all five sign types: unsigned, winding number, fast
+// winding number, pseudonormal, and the default, which is the winding-number
+// type as in the reference; N is filled for the pseudonormal type)
 // A program switches by including this header instead of the libigl ones and
 // linking libgwn.so.  Matrices of any storage order and scalar type are
 // accepted (converted to the row-major float/double buffers of the C ABI);

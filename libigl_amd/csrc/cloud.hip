@@ -36,6 +36,7 @@
 #include <cmath>
 #include <vector>
 
+#include "device_math.h"
 #include "gwn_internal.h"
 
 namespace gwn {
@@ -604,59 +605,98 @@ __global__ void k_rec_write(int64_t ncells, const int32_t* __restrict__ CH, cons
     meta[p] = make_uint2(REC_LEAF | (uint32_t)cstart[c], (uint32_t)ccount[c]);
 }
 
+// The expansion of each internal record as one polynomial in the direction
+// to the query (far_field_poly, device_math.h), formed in FP64 from the cell's
+// FP32 EC and rounded once.  With l = CM - q, x = l/|l| and m = 1/|l|, the
+// terms of the reference's expansion (fast_winding_number.cpp:174-228) are
+//   order 0: m^2 (x.E0) / 4pi
+//   order 1: m^3 sum_uv (delta_uv - 3 x_u x_v) E1[u][v] / 4pi
+//          = m^3 x^T (tr(E1) I - 3 E1) x / 4pi                  (|x| = 1)
+//   order 2: m^4 [15 sum_iuv x_i x_u x_v E2[i][u][v]
+//                 - 3 sum_iuv (delta_uv x_i + delta_ui x_v + delta_vi x_u) E2[i][u][v]] / 4pi
+//          = m^4 [15 T(x) - 3 (x.s)|x|^2] / 4pi,
+//     s_k = sum_u E2[k][u][u] + sum_i E2[i][i][k] + sum_i E2[i][k][i],
+// with E1[u][v] = EC[3 + u + 3v] and E2[i][u][v] = EC[12 + 9i + u + 3v].  This
+// is exact algebra; only the rounding of the far field changes (~1e-7
+// relative), and the near/far decisions do not use it.
+template <int NT>
+__global__ void k_rec_poly(uint32_t nrec, const uint2* __restrict__ meta, const float* __restrict__ EC,
+                           float4* __restrict__ poly) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nrec) return;
+  const uint2 m = meta[p];
+  if (m.x & REC_LEAF) return;  // leaf records sum their points; no expansion
+  const float* ec = EC + (int64_t)NT * m.y;
+  const double k = 1.0 / (4.0 * CL_PI);
+  double b[3], q[6] = {0, 0, 0, 0, 0, 0}, g[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int u = 0; u < 3; ++u) b[u] = (double)ec[u] * k;
+  if (NT > 3) {
+    double E1[3][3];
+    for (int u = 0; u < 3; ++u)
+      for (int v = 0; v < 3; ++v) E1[u][v] = ec[3 + u + 3 * v];
+    const double tr = E1[0][0] + E1[1][1] + E1[2][2];
+    // c.x c.y c.z c.w d.x d.y = xx yy zz xy yz xz
+    q[0] = (tr - 3.0 * E1[0][0]) * k;
+    q[1] = (tr - 3.0 * E1[1][1]) * k;
+    q[2] = (tr - 3.0 * E1[2][2]) * k;
+    q[3] = -3.0 * (E1[0][1] + E1[1][0]) * k;
+    q[4] = -3.0 * (E1[1][2] + E1[2][1]) * k;
+    q[5] = -3.0 * (E1[0][2] + E1[2][0]) * k;
+  }
+  if (NT > 12) {
+    // cubic monomials by exponents (ex, ey, ez), in the order of g[]:
+    // x^3 y^3 z^3 x^2y x^2z xy^2 y^2z xz^2 yz^2 xyz
+    auto slot = [](int ex, int ey, int ez) {
+      if (ex == 3) return 0;
+      if (ey == 3) return 1;
+      if (ez == 3) return 2;
+      if (ex == 2) return ey ? 3 : 4;
+      if (ey == 2) return ex ? 5 : 6;
+      if (ez == 2) return ex ? 7 : 8;
+      return 9;
+    };
+    double sv[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+      for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) {
+          const double e = ec[12 + 9 * i + u + 3 * v];
+          int ex[3] = {0, 0, 0};
+          ++ex[i];
+          ++ex[u];
+          ++ex[v];
+          g[slot(ex[0], ex[1], ex[2])] += 15.0 * e;
+          if (u == v) sv[i] += e;
+          if (u == i) sv[v] += e;
+          if (v == i) sv[u] += e;
+        }
+    // -3 (x.s)(x^2 + y^2 + z^2)
+    g[0] -= 3.0 * sv[0];  // x^3
+    g[5] -= 3.0 * sv[0];  // x y^2
+    g[7] -= 3.0 * sv[0];  // x z^2
+    g[3] -= 3.0 * sv[1];  // x^2 y
+    g[1] -= 3.0 * sv[1];  // y^3
+    g[8] -= 3.0 * sv[1];  // y z^2
+    g[4] -= 3.0 * sv[2];  // x^2 z
+    g[6] -= 3.0 * sv[2];  // y^2 z
+    g[2] -= 3.0 * sv[2];  // z^3
+    for (int j = 0; j < 10; ++j) g[j] *= k;
+  }
+  // record layout of far_field_poly: b, c, d, e, f
+  float4* r = poly + 5 * (size_t)p;
+  r[0] = make_float4((float)b[0], (float)b[1], (float)b[2], 0.0f);
+  r[1] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+  // d = (yz, xz, x^3, y^3), e = (z^3, x^2y, x^2z, xy^2), f = (y^2z, xz^2, yz^2, xyz)
+  r[2] = make_float4((float)q[4], (float)q[5], (float)g[0], (float)g[1]);
+  r[3] = make_float4((float)g[2], (float)g[3], (float)g[4], (float)g[5]);
+  r[4] = make_float4((float)g[6], (float)g[7], (float)g[8], (float)g[9]);
+}
+
 // ---------------- evaluation ----------------
 __device__ __forceinline__ float dipole(float lx, float ly, float lz, float ax, float ay, float az) {
   const float PI_4 = (float)(4.0 * CL_PI);
   const float n = sqrtf(lx * lx + ly * ly + lz * lz);
   if (n == 0.0f) return 0.5f;
   return (lx * ax + ly * ay + lz * az) / (PI_4 * (n * n * n));
-}
-
-template <int NT>
-__device__ __forceinline__ float expansion(float lx, float ly, float lz, const float* __restrict__ ec) {
-  const float PI_4 = (float)(4.0 * CL_PI);
-  float wn = dipole(lx, ly, lz, ec[0], ec[1], ec[2]);
-  const float loc[3] = {lx, ly, lz};
-  const float r = sqrtf(lx * lx + ly * ly + lz * lz);
-  float r3 = 0, r5 = 0;
-  if (NT > 3) {
-    r3 = PI_4 * r * r * r;
-    r5 = r3 * r * r;
-    const float d = (float)(1.0 / (double)r3);
-    const float s = (float)(-3.0 / (double)r5);
-    float acc = 0.0f;
-#pragma unroll
-    for (int v = 0; v < 3; ++v)
-#pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        float S = (loc[u] * loc[v]) * s;
-        if (u == v) S += d;
-        acc += S * ec[3 + u + 3 * v];
-      }
-    wn += acc;
-  }
-  if (NT > 12) {
-    const float r7 = r5 * r * r;
-    const float s = (float)(-3.0 / (double)r5);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const float f15 = (float)(15.0 * (double)loc[i]);
-      float acc = 0.0f;
-#pragma unroll
-      for (int v = 0; v < 3; ++v)
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-          float rcd = 0.0f;
-          if (u == v) rcd += loc[i];
-          if (u == i) rcd += loc[v];
-          if (v == i) rcd += loc[u];
-          const float L = loc[u] * (loc[v] / r7);
-          acc += (f15 * L + s * rcd) * ec[12 + 9 * i + u + 3 * v];
-        }
-      wn += acc;
-    }
-  }
-  return wn;
 }
 
 // Wave-coherent rope walk (the scheme of the soup traversal): each lane keeps
@@ -670,7 +710,7 @@ __device__ __forceinline__ float expansion(float lx, float ly, float lz, const f
 // its lanes and is fetched once through the scalar cache.
 template <int NT>
 __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict__ head, const uint2* __restrict__ meta,
-                                                        uint32_t nrec, const float* __restrict__ EC,
+                                                        uint32_t nrec, const float4* __restrict__ poly,
                                                         const float* __restrict__ Ps, const float* __restrict__ NAs,
                                                         int64_t npts, bool root_leaf, const float* __restrict__ Q,
                                                         const uint32_t* __restrict__ perm, int64_t nq, double beta,
@@ -688,10 +728,12 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
     while (i < nrec) {
       const uint2 m = meta[i];
       const float4 h = head[i];
-      // the record's meta and head in one scalar-cache round trip (head is
-      // not used on a leaf, but the compiler would otherwise wait for meta
-      // before requesting it)
-      asm volatile("" ::"s"(m.x), "s"(m.y), "s"(h.x), "s"(h.w));
+      const float4* r = poly + 5 * (size_t)i;  // in bounds for every record; unused on a leaf
+      const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
+      // the record's meta, head and polynomial in one scalar-cache round trip
+      // (the compiler would otherwise wait for meta, then for the decision,
+      // before requesting the rest; measured 1.84 -> 1.72 ms on 4.1M queries)
+      asm volatile("" ::"s"(m.x), "s"(m.y), "s"(h.x), "s"(h.w), "s"(r0.x), "s"(r2.x), "s"(r4.w));
       const bool leaf = (m.x & REC_LEAF) != 0;
       if (skip == i) {
         skip = i + 1;
@@ -704,9 +746,10 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
           }
         } else {
           const float lx = h.x - qx, ly = h.y - qy, lz = h.z - qz;
-          const float nrm = sqrtf(lx * lx + ly * ly + lz * lz);
-          if ((double)nrm > beta * (double)h.w) {
-            wn += expansion<NT>(lx, ly, lz, EC + (int64_t)NT * m.y);
+          const float l2 = lx * lx + ly * ly + lz * lz;
+          if ((double)sqrtf(l2) > beta * (double)h.w) {
+            constexpr int ORDER = NT == 3 ? 0 : NT == 12 ? 1 : 2;
+            wn += far_field_poly<ORDER>(lx, ly, lz, l2, r0, r1, r2, r3, r4);
             skip = m.x;
           }
         }
@@ -923,6 +966,15 @@ static void rope_records(gwn_cloud* c, const SplitCell* all, const std::vector<u
   hipLaunchKernelGGL(k_rec_write, dim3(div_up(ncells, 256)), dim3(256), 0, st, ncells, c->CH.p, c->ccount.p,
                      leaf_start, rc.p, pos.p, c->CM.p, c->R.p, (uint32_t)c->nrec, c->head.p, c->meta.p);
   check_launch("k_rec_write");
+  c->poly.alloc(5 * (size_t)c->nrec);
+  const unsigned gp = (unsigned)div_up(c->nrec, 256);
+  if (c->nterms == 3)
+    hipLaunchKernelGGL(k_rec_poly<3>, dim3(gp), dim3(256), 0, st, (uint32_t)c->nrec, c->meta.p, c->EC.p, c->poly.p);
+  else if (c->nterms == 12)
+    hipLaunchKernelGGL(k_rec_poly<12>, dim3(gp), dim3(256), 0, st, (uint32_t)c->nrec, c->meta.p, c->EC.p, c->poly.p);
+  else
+    hipLaunchKernelGGL(k_rec_poly<39>, dim3(gp), dim3(256), 0, st, (uint32_t)c->nrec, c->meta.p, c->EC.p, c->poly.p);
+  check_launch("k_rec_poly");
   GWN_HIP(hipStreamSynchronize(st));
 }
 
@@ -1232,13 +1284,13 @@ void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double b
   ProfileScope prof("cloud_traverse", st);
   if (c->nterms == 3)
     hipLaunchKernelGGL(k_cloud_traverse<3>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   else if (c->nterms == 12)
     hipLaunchKernelGGL(k_cloud_traverse<12>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   else
     hipLaunchKernelGGL(k_cloud_traverse<39>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->EC.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
   check_launch("k_cloud_traverse");
 }
 

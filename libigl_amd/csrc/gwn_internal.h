@@ -361,4 +361,5 @@ struct gwn_cloud {
   gwn::DevBuf<float> CM, R, EC;        // ncells x 3, ncells, ncells x nterms
   gwn::DevBuf<float4> head;            // nrec: CM.xyz, R of each non-empty non-root cell (rope order)
   gwn::DevBuf<uint2> meta;             // nrec: (skip, cell) or (LEAF|first point, count)
+  gwn::DevBuf<float4> poly;            // nrec x 5: the cell's expansion as one polynomial (internal records)
 };

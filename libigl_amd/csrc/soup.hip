@@ -459,37 +459,6 @@ __device__ __forceinline__ T far_field(T x, T y, T z, T l2, const V4& b, const V
   return o;
 }
 
-// FP32 far field from the rewritten slot record (see k_write_slots), for
-// q = query - P and l2 = |q|^2.  With x = q/|q| (kept normalised so that the
-// cubic cannot overflow for distant queries) and m = 1/|q| the value is
-//   m^2 [ x.(-N) + m ( Qf(x) + m G(x) ) ].
-template <int ORDER>
-__device__ __forceinline__ float far_field_poly(float x, float y, float z, float l2, const float4& b,
-                                                const float4& c, const float4& d, const float4& e,
-                                                const float4& f) {
-  // explicit multiply-adds (no compiler contraction): every kernel that
-  // inlines this computes each term bit for bit the same
-  const float m = __builtin_amdgcn_rsqf(l2);
-  x *= m;
-  y *= m;
-  z *= m;
-  float t = fmaf(z, b.z, fmaf(y, b.y, x * b.x));  // x.(-N)
-  if (ORDER >= 1) {
-    const float z2 = z * z;
-    const float q1 = fmaf(d.y, z, fmaf(c.w, y, c.x * x)), q2 = fmaf(d.x, z, c.y * y);
-    float qf = fmaf(c.z, z2, fmaf(y, q2, x * q1));
-    if (ORDER >= 2) {
-      const float g1 = fmaf(e.z, z, fmaf(e.y, y, d.z * x)), g2 = fmaf(f.w, z, e.w * y);
-      const float gx = fmaf(f.y, z2, fmaf(y, g2, x * g1));
-      const float gy = fmaf(f.z, z2, y * fmaf(f.x, z, d.w * y));
-      const float g = fmaf(e.x, z2 * z, fmaf(y, gy, x * gx));
-      qf = fmaf(m, g, qf);
-    }
-    t = fmaf(m, qf, t);
-  }
-  return (m * m) * t;
-}
-
 struct D4 {
   double x, y, z, w;
 };
