@@ -252,7 +252,7 @@ constexpr int64_t kRingMaxQueries = 65536;
 // the serving engine (on the whole request), so that a batch's result does not
 // depend on the path it took: the ring walk below kRingMaxQueries, the
 // wave-coherent walk above; latency_bound callers get its hand-over form on
-// trees of 2^18+ slots.  (The ring walk, over Hilbert-sorted queries, also
+// trees of 2^17+ slots.  (The ring walk, over Hilbert-sorted queries, also
 // finishes batches of up to ~500k queries sooner on large trees when run
 // alone, but it keeps the whole GPU busy for it: under the serving load it
 // cost throughput and latency, p50 30 ms against 11 ms at scale 4.9.)

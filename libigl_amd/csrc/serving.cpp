@@ -217,7 +217,7 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     // per-lane walks requests of a few thousand queries held a slot for
     // 1.5-2 ms at scale 4.9 while the big requests queued behind them.
     // Larger requests take the wave-coherent walk, here in its hand-over form
-    // on trees of 2^18+ slots: a request is done when its slowest wavefront
+    // on trees of 2^17+ slots: a request is done when its slowest wavefront
     // is, and on a 434k-triangle UV sphere (fans of thin triangles at the
     // poles) that cut a 1M-query batch at scale 4.9 from 8.0 to 5.2 ms, with
     // bitwise the same result.

@@ -1339,7 +1339,7 @@ int64_t soup_noalloc_max_queries() { return kNoAllocMaxQueries; }
 
 int soup_walk_for(const gwn_soup* s, int64_t nq, bool latency_bound) {
   if (nq < kRingMaxQueries) return WALK_RING;
-  return latency_bound && s->nslots >= (1 << 18) ? WALK_WAVE_HANDOVER : WALK_WAVE;
+  return latency_bound && s->nslots >= (1 << 17) ? WALK_WAVE_HANDOVER : WALK_WAVE;
 }
 
 void soup_query_device(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
