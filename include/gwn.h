@@ -448,6 +448,8 @@ typedef struct {
   double t_submit, t_start, t_done; /* arrival, admission, result in host memory */
   int64_t num_queries;
   int32_t mesh, status;             /* status: GWN_OK or the request's error */
+  double t_uploaded;                /* last chunk narrowed and its copy to the device enqueued */
+  double t_device;                  /* last chunk's results back in pinned host memory */
 } gwn_request_info;
 
 int gwn_server_create(int slots, int64_t device_budget_bytes, gwn_server** out);

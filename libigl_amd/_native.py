@@ -42,7 +42,8 @@ class MixedParams(C.Structure):
 
 class RequestInfo(C.Structure):
     _fields_ = [("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
-                ("num_queries", C.c_int64), ("mesh", C.c_int32), ("status", C.c_int32)]
+                ("num_queries", C.c_int64), ("mesh", C.c_int32), ("status", C.c_int32),
+                ("t_uploaded", C.c_double), ("t_device", C.c_double)]
 
 
 # name -> (restype, argtypes); mirrors include/gwn.h

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -70,13 +71,19 @@ struct Request {
   float beta = 2.0f;
   int prec = GWN_PREC_FP32;
   double* W = nullptr;
-  double t_submit = 0, t_start = 0, t_done = 0;
+  double t_submit = 0, t_start = 0, t_done = 0, t_uploaded = 0, t_device = 0;
   int status = GWN_OK;
   bool done = false;
 };
 
 // Persistent host threads for the narrowing and copy-out loops of large
-// chunks (shared by all slots of all servers; callers run one part inline).
+// chunks, shared by all slots of all servers.  A loop is cut into parts that
+// the caller and idle pool threads claim one at a time, so the caller never
+// waits for a part nobody has started: when every pool thread is busy with
+// other slots' loops it runs its own parts itself.  (With parts handed to the
+// pool first-come first-served, a big request's narrowing waited behind other
+// requests' parts: 4.6 ms at p99 against 0.26 ms at p50 on the 64-mesh trace
+// at accuracy_scale 4.9.)
 class HostPool {
  public:
   static HostPool& get() {
@@ -86,35 +93,44 @@ class HostPool {
   template <typename Fn>
   void parallel(int64_t n, Fn fn) {
     constexpr int64_t kMin = 1 << 16;
-    constexpr int kParts = 4;
+    constexpr int kParts = 8;
     if (n < kMin) {
       fn(0, n);
       return;
     }
-    const int64_t chunk = (n + kParts - 1) / kParts;
-    std::mutex m;
-    std::condition_variable cv;
-    int left = 0;
+    auto job = std::make_shared<Job>();
+    job->n = n;
+    job->chunk = (n + kParts - 1) / kParts;
+    job->parts = (int)((n + job->chunk - 1) / job->chunk);
+    job->fn = fn;
     {
       std::lock_guard<std::mutex> l(mu_);
-      for (int t = 1; t < kParts; ++t) {
-        const int64_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) continue;
-        ++left;
-        tasks_.push_back([&, b, e] {
-          fn(b, e);
-          std::lock_guard<std::mutex> g(m);
-          if (--left == 0) cv.notify_one();
-        });
-      }
+      for (int t = 1; t < job->parts; ++t) tasks_.push_back(job);
     }
     cv_.notify_all();
-    fn(0, std::min(n, chunk));
-    std::unique_lock<std::mutex> g(m);
-    cv.wait(g, [&] { return left == 0; });
+    job->work();
+    std::unique_lock<std::mutex> g(job->m);
+    job->cv.wait(g, [&] { return job->done == job->parts; });
   }
 
  private:
+  struct Job {
+    std::atomic<int> next{0};
+    int parts = 0, done = 0;  // done: under m
+    int64_t n = 0, chunk = 0;
+    std::function<void(int64_t, int64_t)> fn;
+    std::mutex m;
+    std::condition_variable cv;
+    // claims and runs parts until none is left
+    void work() {
+      for (int p; (p = next.fetch_add(1)) < parts;) {
+        const int64_t b = p * chunk, e = std::min(n, b + chunk);
+        fn(b, e);
+        std::lock_guard<std::mutex> g(m);
+        if (++done == parts) cv.notify_all();
+      }
+    }
+  };
   HostPool() {
     for (int i = 0; i < 8; ++i) th_.emplace_back([this] { run(); });
   }
@@ -131,16 +147,16 @@ class HostPool {
     for (;;) {
       cv_.wait(l, [&] { return stop_ || !tasks_.empty(); });
       if (tasks_.empty()) return;
-      auto t = std::move(tasks_.front());
+      auto job = std::move(tasks_.front());
       tasks_.pop_front();
       l.unlock();
-      t();
+      job->work();  // nothing left to claim: returns at once
       l.lock();
     }
   }
   std::mutex mu_;
   std::condition_variable cv_;
-  std::deque<std::function<void()>> tasks_;
+  std::deque<std::shared_ptr<Job>> tasks_;
   std::vector<std::thread> th_;
   bool stop_ = false;
 };
@@ -188,6 +204,7 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
   auto copy_out = [&](int64_t c) {
     const int b = (int)(c & 1);
     GWN_HIP(hipEventSynchronize(sl->ev[b]));
+    if (c == nchunk - 1) r->t_device = now();
     const int64_t off = c * kChunk, n = std::min(kChunk, r->nq - off);
     double* Wd = r->W + off;
     const double* hW = sl->hW[b];
@@ -210,6 +227,7 @@ void gwn_server::run(gwn::Slot* sl, gwn::Request* r) {
     double* dW = reinterpret_cast<double*>(base + a256(12 * (size_t)cn));
     uint8_t* ws = base + a256(12 * (size_t)cn) + a256(8 * (size_t)cn);
     GWN_HIP(hipMemcpyAsync(dQ, hQ, 12 * (size_t)n, hipMemcpyHostToDevice, sl->stream));
+    if (c == nchunk - 1) r->t_uploaded = now();
     // The walk is chosen on the whole request as a standalone query of the
     // same batch chooses it, so the result is the standalone one.  Smaller
     // requests take the walk with one wavefront per query: sequential walks
@@ -243,7 +261,7 @@ void gwn_server::worker(gwn::Slot* sl) {
         if ((int64_t)need > budget - resident) {  // can never fit: fail it
           pending.pop_front();
           head->status = GWN_ERR_NOMEM;
-          head->t_start = head->t_done = now();
+          head->t_start = head->t_done = head->t_uploaded = head->t_device = now();
           head->done = true;
           --outstanding;
           cv_done.notify_all();
@@ -430,7 +448,7 @@ int gwn_server_submit(gwn_server* s, int32_t mesh_id, const double* Q, int64_t n
   Request* rp = r.get();
   s->reqs.push_back(std::move(r));
   if (nq == 0) {
-    rp->t_start = rp->t_done = rp->t_submit;
+    rp->t_start = rp->t_done = rp->t_uploaded = rp->t_device = rp->t_submit;
     rp->done = true;
     s->cv_done.notify_all();
     return GWN_OK;
@@ -472,6 +490,8 @@ int gwn_server_request_info(const gwn_server* cs, int64_t id, gwn_request_info* 
   info->t_submit = r->t_submit;
   info->t_start = r->t_start;
   info->t_done = r->t_done;
+  info->t_uploaded = r->t_uploaded;
+  info->t_device = r->t_device;
   info->num_queries = r->nq;
   info->mesh = r->mesh;
   info->status = r->done ? r->status : -1;

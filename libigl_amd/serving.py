@@ -161,10 +161,18 @@ def poisson_trace(server, mesh_ids, batches, lam, duration, seed=0, accuracy_sca
     span = max(i.t_done for i in infos) - float(arr[0])
     ok = all(i.status == 0 for i in infos)
     big = q >= np.quantile(q, 0.9)
+    # stages of the largest decile: narrowing and upload, device work and the
+    # copy back, copy into the caller's W
+    up = np.array([i.t_uploaded - i.t_start for i in infos])[big]
+    dev = np.array([i.t_device - i.t_uploaded for i in infos])[big]
+    out = np.array([i.t_done - i.t_device for i in infos])[big]
+    stages = {k: (round(1e3 * float(np.quantile(v, 0.5)), 3), round(1e3 * float(np.quantile(v, 0.99)), 3))
+              for k, v in (("upload", up), ("device", dev), ("copy_out", out))}
     return {"requests": n, "queries": int(q.sum()), "span_s": span, "queries_per_s": q.sum() / span,
             "offered_queries_per_s": q.sum() / duration, "p50_ms": 1e3 * float(np.quantile(lat, 0.5)),
             "p99_ms": 1e3 * float(np.quantile(lat, 0.99)), "max_ms": 1e3 * float(lat.max()),
             "queue_p99_ms": 1e3 * float(np.quantile(wait, 0.99)),
             "service_p99_ms": 1e3 * float(np.quantile(serv, 0.99)),
             "largest_decile_service_p50_ms": 1e3 * float(np.quantile(serv[big], 0.5)),
-            "submit_lag_ms": 1e3 * late, "all_ok": ok, "lambda": lam}
+            "submit_lag_ms": 1e3 * late, "all_ok": ok, "lambda": lam,
+            "largest_decile_stages_p50_p99_ms": stages}
