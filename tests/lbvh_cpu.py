@@ -3,8 +3,11 @@
 Restates libigl_amd/csrc/lbvh.hip step by step in numpy/Python so the GPU
 node array can be compared bit for bit: float32 box centres and scaling (no
 fused multiply-add on either side), 30-bit Morton codes, a stable sort of the
-codes, the Karras binary radix tree and the collapse of even-depth internal
-nodes into 4-wide nodes numbered in binary-node order.
+codes, the Karras binary radix tree, the binary nodes' boxes, and the
+top-down collapse into 4-wide nodes: each starts from its binary node's two
+children and opens its largest member (half area dx dy + dy dz + dz dx in
+float32, first on ties) until it has four or only leaves; nodes are numbered
+level by level.
 """
 import numpy as np
 
@@ -111,25 +114,59 @@ def lbvh_nodes(V, F):
             parent[L] = i
         if not R & LEAF:
             parent[R] = i
-    even = np.zeros(m, np.uint32)
-    for i in range(m):
-        dd = 0
-        p = parent[i]
-        while p >= 0:
-            dd += 1
-            p = parent[p]
-        even[i] = 0 if dd & 1 else 1
-    number = np.concatenate([[0], np.cumsum(even)[:-1]]).astype(np.uint32)
-    nodes = np.full((int(even.sum()), 4), EMPTY, np.uint32)
-    for i in range(m):
-        if not even[i]:
+    # binary node boxes (children before parents: a post-order walk)
+    lo_t, hi_t = tri_boxes(V, F)
+    box = {}
+
+    def leaf_box(k):
+        t = int(tri[k])
+        return (lo_t[t].copy(), hi_t[t].copy())
+
+    stack = [(0, False)]
+    while stack:
+        i, ready = stack.pop()
+        if ready:
+            bs = [leaf_box(c & ~LEAF) if c & LEAF else box[c] for c in (left[i], right[i])]
+            box[i] = (np.minimum(bs[0][0], bs[1][0]).astype(np.float32), np.maximum(bs[0][1], bs[1][1]).astype(np.float32))
             continue
-        out = []
-        for k in (left[i], right[i]):
-            if k & LEAF:
-                out.append(int(tri[k & ~LEAF]))
-            else:
-                for g in (left[k], right[k]):
-                    out.append(int(tri[g & ~LEAF]) if g & LEAF else INTERNAL | int(number[g]))
-        nodes[number[i], : len(out)] = out
-    return nodes
+        stack.append((i, True))
+        for c in (left[i], right[i]):
+            if not c & LEAF:
+                stack.append((c, False))
+
+    def area(i):
+        lo, hi = box[i]
+        d = (hi - lo).astype(np.float32)
+        return np.float32(np.float32(d[0] * d[1]) + np.float32(d[1] * d[2])) + np.float32(d[2] * d[0])
+
+    out_nodes = []
+    frontier = [0]
+    base = 0
+    while frontier:
+        nxt = []
+        level = []
+        for b in frontier:
+            mem = [left[b], right[b]]
+            while len(mem) < 4:
+                best, ba = -1, np.float32(-1.0)
+                for t, x in enumerate(mem):
+                    if not x & LEAF:
+                        a = area(x)
+                        if a > ba:
+                            best, ba = t, a
+                if best < 0:
+                    break
+                x = mem[best]
+                mem[best:best + 1] = [left[x], right[x]]
+            row = []
+            for x in mem:
+                if x & LEAF:
+                    row.append(int(tri[x & ~LEAF]))
+                else:
+                    row.append(INTERNAL | (base + len(frontier) + len(nxt)))
+                    nxt.append(x)
+            level.append(row + [EMPTY] * (4 - len(row)))
+        out_nodes.extend(level)
+        base += len(frontier)
+        frontier = nxt
+    return np.array(out_nodes, np.uint32).reshape(-1, 4)
