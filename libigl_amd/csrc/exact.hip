@@ -264,6 +264,57 @@ __global__ __launch_bounds__(256) void k_wn_tree(const WNRecord* __restrict__ re
   W[q] = sum;
 }
 
+// Wave-coherent form of k_wn_tree for Hilbert-sorted queries (the scheme of
+// the soup traversal): the wavefront walks one shared record index i, and a
+// lane takes part in record i when it is the record its own walk needs next
+// (skip == i; every other lane needs a record at or beyond the subtree's end,
+// the subtrees of the rope being nested intervals).  A lane inside a non-leaf
+// box descends (i + 1); every other taking part sums the record's out list
+// (the cap of the node's faces) or, inside a leaf, its in list (the faces),
+// and jumps to link.  Records and triangles are the same for every lane of a
+// wavefront and come through the scalar cache; each lane adds the same terms
+// in the same order as in k_wn_tree, so the results are bitwise the same.
+__global__ __launch_bounds__(256) void k_wn_tree_wave(const WNRecord* __restrict__ rec, uint32_t nrec,
+                                                       const double* __restrict__ tri,
+                                                       const double* __restrict__ O,
+                                                       const uint32_t* __restrict__ perm, int64_t no,
+                                                       double* __restrict__ W) {
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= no) return;
+  const int64_t q = perm[t];
+  const double px = O[3 * q], py = O[3 * q + 1], pz = O[3 * q + 2];
+  double sum = 0.0;
+  uint32_t i = 0, skip = 0;
+  while (i < nrec) {
+    const WNRecord& r = rec[i];
+    const bool part = skip == i;
+    const bool inside = !(px < r.lo[0] || px > r.hi[0] || py < r.lo[1] || py > r.hi[1] || pz < r.lo[2] ||
+                          pz > r.hi[2]);
+    const bool leaf = r.leaf != 0;
+    const bool out_sum = part && !inside;
+    const bool in_sum = part && inside && leaf;
+    if (part) skip = (inside && !leaf) ? i + 1 : r.link;
+    // the lists are wave-uniform: one scalar fetch per triangle for all lanes
+    if (__ballot(out_sum) != 0) {
+      const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.out_start), n0 = __builtin_amdgcn_readfirstlane(r.out_count);
+      for (uint32_t k = 0; k < n0; ++k) {
+        const double* tk = tri + 9 * (size_t)(s0 + k);
+        if (out_sum) sum += vos_atan2(tk, px, py, pz) / (2.0 * EX_PI);
+      }
+    }
+    if (__ballot(in_sum) != 0) {
+      const uint32_t s1 = __builtin_amdgcn_readfirstlane(r.in_start), n1 = __builtin_amdgcn_readfirstlane(r.in_count);
+      for (uint32_t k = 0; k < n1; ++k) {
+        const double* tk = tri + 9 * (size_t)(s1 + k);
+        if (in_sum) sum += vos_atan2(tk, px, py, pz) / (2.0 * EX_PI);
+      }
+    }
+    const bool step = __ballot(skip == i + 1) != 0;
+    i = __builtin_amdgcn_readfirstlane(step ? i + 1 : r.link);
+  }
+  W[q] = sum;
+}
+
 __global__ void k_wn_narrow(const double* __restrict__ O, int64_t n3, float* __restrict__ Of) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n3) Of[i] = (float)O[i];
@@ -304,9 +355,15 @@ void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, co
     radix_sort_u32(keys, perm, no, 30, tmp, tb, st);
   }
   ProfileScope prof("exact_tree", st);
-  hipLaunchKernelGGL(k_wn_tree, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec), (uint32_t)nrec,
-                     tri, O, perm, no, W);
-  check_launch("k_wn_tree");
+  if (sorted) {
+    hipLaunchKernelGGL(k_wn_tree_wave, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec),
+                       (uint32_t)nrec, tri, O, perm, no, W);
+    check_launch("k_wn_tree_wave");
+  } else {
+    hipLaunchKernelGGL(k_wn_tree, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec),
+                       (uint32_t)nrec, tri, O, perm, no, W);
+    check_launch("k_wn_tree");
+  }
 }
 
 }  // namespace gwn
