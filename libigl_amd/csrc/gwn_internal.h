@@ -237,16 +237,14 @@ void to_float_device(const double* x, int64_t n, float* y, hipStream_t st);
 void curve_build_device(gwn_curve* c, hipStream_t st);
 void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float accuracy_scale, float* angle,
                         hipStream_t st);
-// traversal kernel: WALK_WAVE (wave-coherent, sorted batch queries),
-// WALK_LANE (per-lane over the sorted queries), WALK_LANE_UNSORTED (per-lane
-// in input order: no sort, a workspace of 4 nq + 4 bytes) or
-// WALK_WAVE_HANDOVER (wave-coherent until a wavefront's lanes diverge, then
+// traversal kernel: WALK_WAVE (wave-coherent over the Hilbert-sorted batch)
+// and WALK_WAVE_HANDOVER (the same until a wavefront's lanes diverge, then
 // per-lane: shorter batches on large trees with thin-triangle fans, slower on
 // coherent ones) give bitwise the same results (sequential sums in rope
 // order).  WALK_RING (one wavefront per query, in input order, no workspace)
 // sums per lane and then across lanes: deterministic, and within rounding of
 // the sequential walks.
-constexpr int WALK_WAVE = 0, WALK_LANE = 1, WALK_LANE_UNSORTED = 2, WALK_WAVE_HANDOVER = 3, WALK_RING = 4;
+constexpr int WALK_WAVE = 0, WALK_WAVE_HANDOVER = 3, WALK_RING = 4;
 constexpr int64_t kRingMaxQueries = 65536;
 // Walk of a batch of nq queries on tree s, used by every entry point and by
 // the serving engine (on the whole request), so that a batch's result does not
@@ -263,8 +261,6 @@ int soup_walk_for(const gwn_soup* s, int64_t nq, bool latency_bound);
 constexpr int WS_DEVICE_COUNT = 1, WS_NO_PROFILE = 2;
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags = 0);
-// workspace bytes of a WALK_LANE_UNSORTED call
-size_t soup_small_work_bytes(int64_t nq);
 // scratch-free soup query over the static device scratch (soup.hip)
 void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                                double* W, hipStream_t st);

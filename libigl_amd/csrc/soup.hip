@@ -545,10 +545,9 @@ __device__ __forceinline__ double finish_walk(float sum, const NearList& L, cons
 // in that, and only the mixed form's end-of-walk FP64 code would take more
 // (94 VGPRs, 5 wavefronts); it keeps some of its constants in scratch instead.
 // Per-lane rope walk from record j to the end: each lane follows its own skip
-// pointers and gathers its own records.  k_traverse_lane runs it from record
-// 0; the wave-coherent walk hands over to it where its wavefront's lanes have
-// stopped sharing records.  Records, terms and their order are the same in
-// both walks.
+// pointers and gathers its own records.  The wave-coherent walk hands over to
+// it where its wavefront's lanes have stopped sharing records; records, terms
+// and their order are the same in both walks.
 template <int ORDER, bool MIXED>
 __device__ __forceinline__ void lane_walk(const float4* __restrict__ slots, uint32_t nslots,
                                           const float* __restrict__ tri, uint32_t j, float qx, float qy, float qz,
@@ -668,28 +667,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
 }
 
-// Per-lane rope walk: each lane follows its own skip-pointer path and gathers
-// its own records.  It visits the same records in the same order and sums the
-// same terms as k_traverse; the running sums use non-contracting adds in both
-// kernels, so no multiply is fused into them differently and the results are
-// bitwise the same.  Used for small batches (whose queries are too sparse
-// for wavefront coherence), unsorted below a few thousand queries, and by
-// the serving engine for chunks under 64k queries.
-template <int ORDER, bool MIXED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse_lane(
-    const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
-    const uint32_t* __restrict__ perm, int64_t nq, float acc2, float near_d, double* __restrict__ W) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
-  const int64_t qi = perm ? (int64_t)perm[t] : t;
-  const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
-  float sum = 0.0f;
-  __shared__ uint32_t near_lds[MIXED ? kDefer * kTravThreads : 1];  // mixed form: noted near triangles
-  NearList L{near_lds + threadIdx.x, 0};
-  lane_walk<ORDER, MIXED>(slots, nslots, tri, 0u, qx, qy, qz, acc2, near_d, sum, L);
-  W[qi] = finish_walk<MIXED>(sum, L, tri, qx, qy, qz);
-}
-
 // all-FP64 form: FP64 moments, far field, atan2 and sum
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
@@ -736,51 +713,6 @@ __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ s
     }
     const bool step = leaf || __ballot(skip == i + 1) != 0;
     i = wave_uniform(step ? i + 1 : link);
-  }
-  W[qi] = sum / (4.0 * PI_D);
-}
-
-// Per-lane form of k_traverse64: each lane follows its own skip pointers and
-// gathers its own FP64 records; same records, terms and order as the
-// wave-coherent walk.  A wavefront of this walk costs its longest path, where
-// the wave-coherent walk costs the union of its 64 paths: the mixed form's
-// re-evaluation list (a few queries scattered over the mesh) and unsorted small
-// batches take this one.  dcount as in k_traverse64.
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_traverse64_lane(const float4* __restrict__ slots,
-                                                         const double* __restrict__ s64, uint32_t nslots,
-                                                         const float* __restrict__ tri, const float* __restrict__ Q,
-                                                         const uint32_t* __restrict__ perm, int64_t nq,
-                                                         const uint32_t* __restrict__ dcount, double acc2,
-                                                         double* __restrict__ W) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= (dcount ? (int64_t)*dcount : nq)) return;
-  const int64_t qi = perm ? (int64_t)perm[t] : t;
-  const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
-  double sum = 0.0;
-  uint32_t i = 0;
-  while (i < nslots) {
-    const double* r = s64 + 24 * (size_t)i;
-    const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
-    const double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-    const bool leaf = (link & LINK_LEAF) != 0;
-    const double x = qx - r0, y = qy - r1, z = qz - r2;
-    const double l2 = x * x + y * y + z * z;
-    if (!(l2 <= r3 * acc2)) {
-      const D4 b{r[4], r[5], r[6], 0.0}, c{r[8], r[9], r[10], r[11]}, d{r[12], r[13], r[14], r[15]},
-          e{r[16], r[17], r[18], r[19]}, f{r[20], r[21], r[22], r[23]};
-      const double o = far_field<ORDER, double, D4>(x, y, z, l2, b, c, d, e, f);
-      if (isfinite(o)) {
-        sum = __dadd_rn(sum, o);
-        i = leaf ? i + 1 : link;
-        continue;
-      }
-    }
-    if (leaf) {
-      const float* p = tri + 9 * (size_t)(link & ~LINK_LEAF);
-      sum = __dadd_rn(sum, tri_solid_angle<double>(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], qx, qy, qz));
-    }
-    ++i;
   }
   W[qi] = sum / (4.0 * PI_D);
 }
@@ -1277,7 +1209,6 @@ size_t soup_query_work_bytes(int64_t nq) {
   return up(4 * (size_t)nq) * 2 + up(radix_sort_temp_bytes(nq, false)) + up(4 * (size_t)nq) + up(4);
 }
 // Workspace of an unsorted (per-lane) call: the ambiguous list and count only.
-size_t soup_small_work_bytes(int64_t nq) { return (4 * (size_t)nq + 255) / 256 * 256 + 256; }
 
 // 30-bit Morton keys of the queries over a box (queries outside are clamped
 // to it), and the identity permutation to be sorted along with them
@@ -1372,33 +1303,16 @@ void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, fl
                        WS_DEVICE_COUNT | WS_NO_PROFILE);
 }
 
-// Query with caller-provided device workspace (soup_query_work_bytes(nq)
-// bytes, or soup_small_work_bytes(nq) for WALK_LANE_UNSORTED); calls with
-// distinct workspaces may run concurrently on distinct streams against the
-// same tree.  flags: WS_DEVICE_COUNT launches the mixed form's FP64
-// re-evaluation over every query with the count read on the device (no host
-// round trip); WS_NO_PROFILE creates no kernel-timing events.
-// FP64 walk over nq queries (perm: their order, or null): wave-coherent for
-// sorted batches, per-lane otherwise.
-static void launch_traverse64(const gwn_soup* s, bool wave, unsigned g, hipStream_t st, const float* Q,
-                              const uint32_t* perm, int64_t nq, const uint32_t* dcount, double a2, double* W) {
+// FP64 walk over nq queries in the order perm
+static void launch_traverse64(const gwn_soup* s, unsigned g, hipStream_t st, const float* Q, const uint32_t* perm,
+                              int64_t nq, double a2, double* W) {
   const uint32_t ns = (uint32_t)s->nslots;
-#define GWN_LAUNCH_64(O)                                                                                         \
-  do {                                                                                                           \
-    if (wave)                                                                                                    \
-      hipLaunchKernelGGL(k_traverse64<O>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, \
-                         perm, nq, dcount, a2, W);                                                               \
-    else                                                                                                         \
-      hipLaunchKernelGGL(k_traverse64_lane<O>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, \
-                         Q, perm, nq, dcount, a2, W);                                                            \
-  } while (0)
   switch (s->order) {
-    case 0: GWN_LAUNCH_64(0); break;
-    case 1: GWN_LAUNCH_64(1); break;
-    default: GWN_LAUNCH_64(2); break;
+    case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+    case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+    default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
   }
-#undef GWN_LAUNCH_64
-  check_launch(wave ? "k_traverse64" : "k_traverse64_lane");
+  check_launch("k_traverse64");
 }
 
 // one wavefront per query (k_walk_ring), 4 per block
@@ -1440,6 +1354,12 @@ static void launch_walk_ring(const gwn_soup* s, const float* Q, int64_t nq, floa
   }
 }
 
+// Query with caller-provided device workspace of soup_query_work_bytes(nq)
+// bytes (none for WALK_RING); calls with distinct workspaces may run
+// concurrently on distinct streams against the same tree.  flags:
+// WS_DEVICE_COUNT launches the mixed form's FP64 re-evaluation over every
+// query with the count read on the device (no host round trip); WS_NO_PROFILE
+// creates no kernel-timing events.
 void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float accuracy_scale, int precision,
                           double* W, hipStream_t st, uint8_t* workspace, size_t ws_bytes, int walk, int flags) {
   if (nq <= 0) return;
@@ -1452,19 +1372,17 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
     launch_walk_ring(s, Q, nq, accuracy_scale, precision, W, st, prof_on);
     return;
   }
-  const bool sorted = walk != WALK_LANE_UNSORTED;
-  GWN_CHECK(ws_bytes >= (sorted ? soup_query_work_bytes(nq) : soup_small_work_bytes(nq)), GWN_ERR_INVALID,
-            "query workspace too small");
+  GWN_CHECK(walk == WALK_WAVE || walk == WALK_WAVE_HANDOVER, GWN_ERR_INVALID, "unknown walk");
+  GWN_CHECK(ws_bytes >= soup_query_work_bytes(nq), GWN_ERR_INVALID, "query workspace too small");
   uint8_t* w = workspace;
   auto take = [&](size_t b) {
     uint8_t* p = w;
     w += (b + 255) / 256 * 256;
     return p;
   };
-  uint32_t* perm = nullptr;
-  if (sorted) {
-    uint32_t* keys = (uint32_t*)take(4 * (size_t)nq);
-    perm = (uint32_t*)take(4 * (size_t)nq);
+  uint32_t* keys = (uint32_t*)take(4 * (size_t)nq);
+  uint32_t* perm = (uint32_t*)take(4 * (size_t)nq);
+  {
     const size_t tmpb = radix_sort_temp_bytes(nq, false);
     void* tmp = take(tmpb);
     hilbert_keys(Q, nq, s->bbox, keys, perm, st);
@@ -1480,7 +1398,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   if (precision == GWN_PREC_FP64) {
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st, prof_on);
-    launch_traverse64(s, walk == WALK_WAVE || walk == WALK_WAVE_HANDOVER, g, st, Q, perm, nq, nullptr, a2, W);
+    launch_traverse64(s, g, st, Q, perm, nq, a2, W);
     return;
   }
   const bool mixed = precision == GWN_PREC_MIXED;
@@ -1490,12 +1408,9 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
     if (walk == WALK_WAVE)                                                                                      \
       hipLaunchKernelGGL((k_traverse<O, M, false>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
                          nq, acc2, nd, W);                                                                      \
-    else if (walk == WALK_WAVE_HANDOVER)                                                                        \
+    else                                                                                                        \
       hipLaunchKernelGGL((k_traverse<O, M, true>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
                          nq, acc2, nd, W);                                                                      \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_traverse_lane<O, M>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, nq, \
-                         acc2, nd, W);                                                                          \
   } while (0)
   {
     ProfileScope prof("soup_traverse", st, prof_on);
