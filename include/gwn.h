@@ -90,7 +90,7 @@ const char* gwn_version(void);
 /* Kernel timing for measurement tools (no counterpart in the reference).
  * While enabled, the library brackets its named kernels ("soup_traverse",
  * "soup_traverse64", "soup_reeval64", "soup_walk_ring" (smaller batches, see gwn_soup_query), "exact",
- * "exact_tree", "cloud_traverse", "radix_sort",
+ * "exact_tree", "cloud_traverse", "radix_sort", "block_order",
  * "closest_point", "knn", "point_areas", "cloud_brute", "exact2d",
  * "curve_traverse") with HIP events
  * recorded on the stream they are launched on; gwn_profile_read synchronises

@@ -246,6 +246,7 @@ void curve_query_device(const gwn_curve* c, const float* Q, int64_t nq, float ac
 // the sequential walks.
 constexpr int WALK_WAVE = 0, WALK_WAVE_HANDOVER = 3, WALK_RING = 4;
 constexpr int64_t kRingMaxQueries = 65536;
+constexpr int kCostGrid = 32;
 // Walk of a batch of nq queries on tree s, used by every entry point and by
 // the serving engine (on the whole request), so that a batch's result does not
 // depend on the path it took: the ring walk below kRingMaxQueries, the
@@ -288,6 +289,10 @@ struct gwn_soup {
   // nv x 3, edge normals per face and corner nf x 3 x 3; built on first use
   gwn::DevBuf<double> pn_face, pn_vertex, pn_edge;
   gwn_mixed_params mixed{1e-3f, 0.05f};
+  // Walk cost of a query at the centre of each cell of a kCostGrid^3 grid over
+  // bbox (records visited at accuracy scale 2, as a log-scale byte), built
+  // with the tree: the wave walks start their costliest blocks first.
+  gwn::DevBuf<uint8_t> cost_grid;
   // After the build a handle is read-only to queries, except for dbox and the
   // pseudonormals, which the first query that needs them builds under
   // dist_mu / pn_mu (dist_ready / pn_ready set after).

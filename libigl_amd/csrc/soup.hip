@@ -600,8 +600,9 @@ __device__ __forceinline__ float in_vgpr(float x) {
 template <int ORDER, bool MIXED, bool HANDOVER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_traverse(
     const float4* __restrict__ slots, uint32_t nslots, const float* __restrict__ tri, const float* __restrict__ Q,
-    const uint32_t* __restrict__ perm, int64_t nq, float acc2_arg, float near_d, double* __restrict__ W) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t* __restrict__ perm, const uint32_t* __restrict__ order, int64_t nq, float acc2_arg, float near_d,
+    double* __restrict__ W) {
+  const int64_t t = (int64_t)(order ? order[xcd_block()] : xcd_block()) * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const float acc2 = in_vgpr(acc2_arg);
   const int64_t qi = perm ? (int64_t)perm[t] : t;
@@ -672,11 +673,13 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_traverse64(const float4* __restrict__ slots, const double* __restrict__ s64,
                                                     uint32_t nslots, const float* __restrict__ tri,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ perm,
-                                                    int64_t nq, const uint32_t* __restrict__ dcount, double acc2,
+                                                    const uint32_t* __restrict__ order, int64_t nq,
+                                                    const uint32_t* __restrict__ dcount, double acc2,
                                                     double* __restrict__ W) {
   // the same wave-coherent rope walk as k_traverse, over the FP64 records;
-  // dcount (when given) holds the number of queries, read on the device
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  // dcount (when given) holds the number of queries, read on the device;
+  // order (when given) is the block order
+  const int64_t t = (int64_t)(order ? order[xcd_block()] : xcd_block()) * blockDim.x + threadIdx.x;
   if (t >= (dcount ? (int64_t)*dcount : nq)) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const double qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
@@ -1125,8 +1128,11 @@ static void build_fp64_moments(gwn_soup* s, const Levels& L, hipStream_t st) {
   GWN_HIP(hipStreamSynchronize(st));
 }
 
-// Tree levels, FP32 BoxData (the reference's), the rope-ordered slot records
-// and their FP64 moments.  After this the handle is read-only to queries.
+static void build_cost_grid(gwn_soup* s, hipStream_t st);
+
+// Tree levels, FP32 BoxData (the reference's), the rope-ordered slot records,
+// their FP64 moments and the cost grid.  After this the handle is read-only
+// to queries.
 void soup_finish_build(gwn_soup* s, hipStream_t st) {
   const int64_t nn = s->nnodes;
   if (nn == 0) {
@@ -1149,6 +1155,8 @@ void soup_finish_build(gwn_soup* s, hipStream_t st) {
                      first.p, s->slots.p);
   check_launch("k_write_slots");
   build_fp64_moments(s, L, st);
+  build_cost_grid(s, st);
+  GWN_HIP(hipStreamSynchronize(st));
 }
 
 // Slot boxes for closest-point queries (and the antipodal rays), built by the
@@ -1218,6 +1226,117 @@ void morton_keys(const float* Q, int64_t nq, const float* bb, uint32_t* keys, ui
   hipLaunchKernelGGL(k_morton_q, dim3(div_up(nq, 256)), dim3(256), 0, st, Q, nq, bb[0], bb[1], bb[2], 1024.0f / ex,
                      1024.0f / ey, 1024.0f / ez, keys, vals);
   check_launch("k_morton_q");
+}
+
+
+// ---------------------------------------------------------------------------
+// Block order of the wave walks.  A wavefront's walk time is the union of its
+// 64 queries' paths, and it varies several-fold over a batch (queries near the
+// surface open many more records); dispatched in Hilbert order, the costly
+// blocks of a region can start last and run on alone at the end of the
+// launch.  The walks instead take each XCD's blocks costliest first (longest
+// processing time first), with a block's cost estimated from the cost grid
+// built with the tree.  The order changes which blocks run when, never which
+// queries a wavefront holds, so every result is the same bit for bit.
+
+// Records a query at each cell centre of the cost grid visits at accuracy
+// scale 2 (the near/far decisions of k_traverse, without the sums), as a
+// log-scale byte: min(255, 15 log2(1 + n)).
+__global__ __launch_bounds__(256) void k_cost_grid(const float4* __restrict__ slots, uint32_t nslots, float lx,
+                                                   float ly, float lz, float hx, float hy, float hz,
+                                                   uint8_t* __restrict__ grid) {
+  constexpr int G = kCostGrid;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G * G * G) return;
+  const float qx = lx + (c % G + 0.5f) * hx, qy = ly + ((c / G) % G + 0.5f) * hy, qz = lz + (c / (G * G) + 0.5f) * hz;
+  uint32_t i = 0, n = 0;
+  while (i < nslots) {
+    const float4 a = slots[6 * (size_t)i];
+    const uint32_t link = __float_as_uint(slots[6 * (size_t)i + 1].w);
+    const bool leaf = (link & LINK_LEAF) != 0;
+    const float x = qx - a.x, y = qy - a.y, z = qz - a.z;
+    const float l2 = x * x + y * y + z * z;
+    ++n;
+    if (!(l2 <= a.w * 4.0f)) {
+      i = leaf ? i + 1 : link;
+    } else {
+      n += leaf;
+      ++i;
+    }
+  }
+  grid[c] = (uint8_t)fminf(255.0f, 15.0f * log2f(1.0f + (float)n));
+}
+
+static void build_cost_grid(gwn_soup* s, hipStream_t st) {
+  constexpr int G = kCostGrid;
+  s->cost_grid.alloc((size_t)G * G * G);
+  const float* bb = s->bbox;
+  hipLaunchKernelGGL(k_cost_grid, dim3(div_up(G * G * G, 256)), dim3(256), 0, st, s->slots.p, (uint32_t)s->nslots,
+                     bb[0], bb[1], bb[2], (bb[3] - bb[0]) / G, (bb[4] - bb[1]) / G, (bb[5] - bb[2]) / G,
+                     s->cost_grid.p);
+  check_launch("k_cost_grid");
+}
+
+// Estimated cost of each block of 256 sorted queries: the grid's largest
+// value over the block's queries.
+__global__ __launch_bounds__(256) void k_block_cost(const float* __restrict__ Q, const uint32_t* __restrict__ perm,
+                                                    int64_t nq, float lx, float ly, float lz, float sx, float sy,
+                                                    float sz, const uint8_t* __restrict__ grid,
+                                                    uint8_t* __restrict__ cost) {
+  constexpr int G = kCostGrid;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int v = 0;
+  if (t < nq) {
+    const int64_t qi = perm[t];
+    const int ix = min(G - 1, max(0, (int)((Q[3 * qi] - lx) * sx)));
+    const int iy = min(G - 1, max(0, (int)((Q[3 * qi + 1] - ly) * sy)));
+    const int iz = min(G - 1, max(0, (int)((Q[3 * qi + 2] - lz) * sz)));
+    v = grid[(iz * G + iy) * G + ix];
+  }
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+  __shared__ int wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) cost[blockIdx.x] = (uint8_t)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+// Block order: one workgroup per XCD's range of logical blocks (the ranges of
+// xcd_block()), a counting sort of its blocks by decreasing cost.  Blocks of
+// equal cost land in any order.
+__global__ __launch_bounds__(256) void k_block_order(const uint8_t* __restrict__ cost, uint32_t nb,
+                                                     uint32_t* __restrict__ order) {
+  __shared__ uint32_t count[256];
+  const uint32_t x = blockIdx.x, q = nb / 8, r = nb % 8;
+  const uint32_t lo = x * q + (x < r ? x : r), hi = lo + q + (x < r ? 1 : 0);
+  count[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t b = lo + threadIdx.x; b < hi; b += blockDim.x) atomicAdd(&count[255 - cost[b]], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = lo;
+    for (int k = 0; k < 256; ++k) {
+      const uint32_t n = count[k];
+      count[k] = run;
+      run += n;
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = lo + threadIdx.x; b < hi; b += blockDim.x) order[atomicAdd(&count[255 - cost[b]], 1u)] = b;
+}
+
+// order (nb = ceil(nq / 256) entries) and cost (nb bytes) are workspace
+static void block_order(const gwn_soup* s, const float* Q, const uint32_t* perm, int64_t nq, uint32_t* order,
+                        uint8_t* cost, hipStream_t st) {
+  constexpr int G = kCostGrid;
+  const uint32_t nb = (uint32_t)div_up(nq, 256);
+  const float* bb = s->bbox;
+  const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
+              ez = std::max(bb[5] - bb[2], 1e-30f);
+  hipLaunchKernelGGL(k_block_cost, dim3(nb), dim3(256), 0, st, Q, perm, nq, bb[0], bb[1], bb[2], G / ex, G / ey,
+                     G / ez, s->cost_grid.p, cost);
+  check_launch("k_block_cost");
+  hipLaunchKernelGGL(k_block_order, dim3(8), dim3(256), 0, st, cost, nb, order);
+  check_launch("k_block_order");
 }
 
 // 30-bit Hilbert keys of the queries over a box (clamped to it) and the
@@ -1305,12 +1424,12 @@ void soup_query_device_noalloc(const gwn_soup* s, const float* Q, int64_t nq, fl
 
 // FP64 walk over nq queries in the order perm
 static void launch_traverse64(const gwn_soup* s, unsigned g, hipStream_t st, const float* Q, const uint32_t* perm,
-                              int64_t nq, double a2, double* W) {
+                              const uint32_t* order, int64_t nq, double a2, double* W) {
   const uint32_t ns = (uint32_t)s->nslots;
   switch (s->order) {
-    case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-    case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
-    default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, nq, nullptr, a2, W); break;
+    case 0: hipLaunchKernelGGL(k_traverse64<0>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, order, nq, nullptr, a2, W); break;
+    case 1: hipLaunchKernelGGL(k_traverse64<1>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, order, nq, nullptr, a2, W); break;
+    default: hipLaunchKernelGGL(k_traverse64<2>, dim3(g), dim3(256), 0, st, s->slots.p, s->slots64.p, ns, s->tri.p, Q, perm, order, nq, nullptr, a2, W); break;
   }
   check_launch("k_traverse64");
 }
@@ -1391,14 +1510,21 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   }
   uint32_t* amb = (uint32_t*)take(4 * (size_t)nq);
   uint32_t* ambn = (uint32_t*)take(4);
+  const unsigned g = div_up(nq, 256);
+  // the sort keys are spent: their space holds the block order (g entries)
+  // and the block costs (g bytes), 5g <= 4 nq bytes rounded up to 256
+  uint32_t* border = keys;
+  {
+    ProfileScope prof("block_order", st, prof_on);
+    block_order(s, Q, perm, nq, border, (uint8_t*)(border + g), st);
+  }
 
   const float acc2 = accuracy_scale * accuracy_scale;
-  const unsigned g = div_up(nq, 256);
   const uint32_t ns = (uint32_t)s->nslots;
   if (precision == GWN_PREC_FP64) {
     const double a2 = (double)accuracy_scale * (double)accuracy_scale;
     ProfileScope prof("soup_traverse64", st, prof_on);
-    launch_traverse64(s, g, st, Q, perm, nq, a2, W);
+    launch_traverse64(s, g, st, Q, perm, border, nq, a2, W);
     return;
   }
   const bool mixed = precision == GWN_PREC_MIXED;
@@ -1407,10 +1533,10 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   do {                                                                                                          \
     if (walk == WALK_WAVE)                                                                                      \
       hipLaunchKernelGGL((k_traverse<O, M, false>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
-                         nq, acc2, nd, W);                                                                      \
+                         border, nq, acc2, nd, W);                                                              \
     else                                                                                                        \
       hipLaunchKernelGGL((k_traverse<O, M, true>), dim3(g), dim3(256), 0, st, s->slots.p, ns, s->tri.p, Q, perm, \
-                         nq, acc2, nd, W);                                                                      \
+                         border, nq, acc2, nd, W);                                                              \
   } while (0)
   {
     ProfileScope prof("soup_traverse", st, prof_on);

@@ -127,6 +127,47 @@ def test_ring_walk_and_sequential_walks_agree(order):
             assert np.array_equal(bvh.query(Q[:20000], beta, prec), Wsmall[:20000])
 
 
+def test_wave_walk_block_order_covers_every_query():
+    """Batches of 65,536+ queries run their blocks of 256 costliest first (a
+    per-XCD counting sort of costs read from a grid built with the tree).
+    Every block runs once: an output prefilled with NaN comes back finite
+    everywhere, with block counts that do not divide by the 8 XCD ranges,
+    queries far outside the grid (clamped to its border cells) and a flat
+    mesh whose grid has no extent along z.  Which block runs when never
+    changes a result: repeated calls agree bit for bit."""
+    from libigl_amd.devmem import DeviceArray, Stream
+    rng = np.random.default_rng(91)
+    V, F = D.bumpy_sphere(4, 16, 91)
+    n = 300_001  # 1,172 blocks: XCD ranges of 146 and 147
+    Q = np.concatenate([rng.uniform(-1.4, 1.4, (n // 2, 3)), D.near_surface_queries(V, F, n - n // 2 - 1000, 1e-3, rng),
+                        rng.uniform(-1e4, 1e4, (1000, 3))]).astype(np.float32)
+    g = np.linspace(-1, 1, 60)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    Vf = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    idx = np.arange(60 * 60).reshape(60, 60)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    Ff = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.int32)
+    Qf = rng.uniform(-1.2, 1.2, (70_001, 3)).astype(np.float32)
+    st = Stream()
+    for (Vm, Fm, Qm) in ((V, F, Q), (Vf, Ff, Qf)):
+        bvh = gwn.FastWindingNumberBVH().build(Vm, Fm, 2)
+        ref = _ref_soup(Vm, Fm)
+        dQ = DeviceArray.from_numpy(Qm)
+        for prec in (gwn.PREC_FP32, gwn.PREC_MIXED, gwn.PREC_FP64):
+            runs = []
+            for _ in range(2):
+                dW = DeviceArray.from_numpy(np.full(Qm.shape[0], np.nan))
+                bvh.query_device(dQ.data_ptr, Qm.shape[0], dW.data_ptr, 4.9, prec, st.ptr)
+                st.synchronize()
+                runs.append(dW.numpy())
+                dW.free()
+            assert np.all(np.isfinite(runs[0])), (Fm.shape[0], prec)
+            assert np.array_equal(runs[0], runs[1])
+            sub = np.arange(0, Qm.shape[0], 97)
+            assert np.abs(runs[0][sub] - ref.query(Qm[sub].astype(np.float64), 4.9)).max() <= TOL_REF
+        dQ.free()
+
+
 def test_soup_edge_cases():
     V, F = D.noisy_sphere(2, 0.01, 5)
     # empty mesh: W = 0 (the reference leaves it uninitialised)
