@@ -15,11 +15,18 @@
 #include <cmath>
 
 #include "gwn_internal.h"
+#include "ring_walk.h"
 
 namespace gwn {
 namespace {
 
 constexpr uint32_t DL_LEAF = 0x80000000u;  // same tag as the rope links
+// Records a k_closest wavefront steps through before it hands its queries to
+// k_closest_ring: 1/64 of the tree, within [2048, 8192] (the best of the
+// budgets tried on 81,920 and 1,310,720 triangles, 1M points each).
+inline uint32_t closest_budget(int64_t nslots) {
+  return (uint32_t)std::min<int64_t>(8192, std::max<int64_t>(2048, nslots / 64));
+}
 
 __device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
@@ -113,65 +120,99 @@ __global__ void k_narrow3(const double* __restrict__ P, int64_t n, float* __rest
   if (i < 3 * n) Pf[i] = (float)P[i];
 }
 
+// |p - c|^2 for the closest point c on face f
+__device__ __forceinline__ double face_dist(const double* p, const double* __restrict__ V,
+                                            const int32_t* __restrict__ F, uint32_t f, double* c) {
+  const int32_t* tf = F + 3 * (size_t)f;
+  double A[3], B[3], C[3];
+  for (int k = 0; k < 3; ++k) {
+    A[k] = V[3 * (int64_t)tf[0] + k];
+    B[k] = V[3 * (int64_t)tf[1] + k];
+    C[k] = V[3 * (int64_t)tf[2] + k];
+  }
+  return tri_closest(p, A, B, C, c);
+}
+
+// Face f replaces the best so far when strictly closer (AABB.cpp set_min).
+__device__ __forceinline__ bool face_closer(const double* p, const double* __restrict__ V,
+                                            const int32_t* __restrict__ F, uint32_t f, double& best, double* bc) {
+  double c[3];
+  const double d2 = face_dist(p, V, F, f, c);
+  if (!(d2 < best)) return false;
+  best = d2;
+  for (int k = 0; k < 3; ++k) bc[k] = c[k];
+  return true;
+}
+
+// Initial bound: descend greedily into the child whose box is nearest (at
+// most 4 siblings per level) down to one triangle.  The walks then replace it
+// only by strictly closer triangles, so the answer is still the minimum; among
+// equally close faces the greedy one is kept.
+__device__ __forceinline__ void greedy_bound(const float4* __restrict__ dbox, uint32_t nslots,
+                                             const double* __restrict__ V, const int32_t* __restrict__ F,
+                                             const double* p, double& best, int32_t& bf, double* bc) {
+  uint32_t lo = 0, hi = nslots;
+  for (int depth = 0; depth < 128 && lo < hi; ++depth) {
+    uint32_t pick = 0xFFFFFFFFu, plink = 0;
+    double pd = best;
+    for (uint32_t k = lo; k < hi;) {
+      const float4 blo = dbox[2 * (size_t)k], bhi = dbox[2 * (size_t)k + 1];
+      const uint32_t l = __float_as_uint(blo.w);
+      const double d = box_d2(p, blo, bhi);
+      if (d < pd) {
+        pd = d;
+        pick = k;
+        plink = l;
+      }
+      k = (l & DL_LEAF) ? k + 1 : l;
+    }
+    if (pick == 0xFFFFFFFFu) break;
+    if (plink & DL_LEAF) {
+      const uint32_t f = plink & ~DL_LEAF;
+      if (face_closer(p, V, F, f, best, bc)) bf = (int32_t)f;
+      break;
+    }
+    lo = pick + 1;
+    hi = plink;
+  }
+}
+
 // Nearest triangle: depth-first branch and bound over the rope, a subtree
 // entered only when its box is closer than the best distance so far
 // (AABB.cpp:1135-1215 looks into a child when its exterior distance is below
 // the current minimum).  `best` starts at up2, as in the reference.
+//
+// A wavefront walks the union of its 64 queries' walks, and for queries deep
+// inside a closed surface (far from it in every direction) that union covers
+// most of the tree: such wavefronts ran 30x the median and set the length of
+// the whole launch.  A wavefront that has stepped through `budget` records
+// hands its queries over to k_closest_ring (their indices appended to
+// `deferred`, the count in *ndeferred) and writes nothing.
 __global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox, uint32_t nslots,
                                                  const double* __restrict__ V, const int32_t* __restrict__ F,
                                                  const double* __restrict__ P, const uint32_t* __restrict__ perm,
-                                                 int64_t np, double up2, double* __restrict__ best_out,
-                                                 int32_t* __restrict__ face_out, double* __restrict__ c_out) {
+                                                 int64_t np, double up2, uint32_t budget,
+                                                 uint32_t* __restrict__ deferred, uint32_t* __restrict__ ndeferred,
+                                                 double* __restrict__ best_out, int32_t* __restrict__ face_out,
+                                                 double* __restrict__ c_out) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= np) return;
   const int64_t pi = perm ? (int64_t)perm[t] : t;
   const double p[3] = {P[3 * pi], P[3 * pi + 1], P[3 * pi + 2]};
   double best = up2, bc[3] = {0.0, 0.0, 0.0};
   int32_t bf = -1;
-  // Initial bound: descend greedily into the child whose box is nearest (at
-  // most 4 siblings per level) down to one triangle.  The walk below then
-  // replaces it only by strictly closer triangles, so the answer is still the
-  // minimum; among equally close faces the greedy one is kept.
-  {
-    uint32_t lo = 0, hi = nslots;
-    for (int depth = 0; depth < 128 && lo < hi; ++depth) {
-      uint32_t pick = 0xFFFFFFFFu, plink = 0;
-      double pd = best;
-      for (uint32_t k = lo; k < hi;) {
-        const float4 blo = dbox[2 * (size_t)k], bhi = dbox[2 * (size_t)k + 1];
-        const uint32_t l = __float_as_uint(blo.w);
-        const double d = box_d2(p, blo, bhi);
-        if (d < pd) {
-          pd = d;
-          pick = k;
-          plink = l;
-        }
-        k = (l & DL_LEAF) ? k + 1 : l;
-      }
-      if (pick == 0xFFFFFFFFu) break;
-      if (plink & DL_LEAF) {
-        const uint32_t f = plink & ~DL_LEAF;
-        const int32_t* tf = F + 3 * (size_t)f;
-        double A[3], B[3], C[3], c[3];
-        for (int k = 0; k < 3; ++k) {
-          A[k] = V[3 * (int64_t)tf[0] + k];
-          B[k] = V[3 * (int64_t)tf[1] + k];
-          C[k] = V[3 * (int64_t)tf[2] + k];
-        }
-        const double d2 = tri_closest(p, A, B, C, c);
-        if (d2 < best) {
-          best = d2;
-          bf = (int32_t)f;
-          for (int k = 0; k < 3; ++k) bc[k] = c[k];
-        }
-        break;
-      }
-      lo = pick + 1;
-      hi = plink;
-    }
-  }
-  uint32_t skip = 0, i = 0;
+  greedy_bound(dbox, nslots, V, F, p, best, bf, bc);
+  uint32_t skip = 0, i = 0, steps = 0;
   while (i < nslots) {
+    if (++steps > budget) {
+      // the wave's live lanes are lanes 0..n-1 (t < np is a prefix of the wave)
+      const uint32_t lane = threadIdx.x & 63, live = (uint32_t)__popcll(__ballot(1));
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(ndeferred, live);
+      base = __shfl(base, 0, 64);
+      deferred[base + lane] = (uint32_t)pi;
+      return;
+    }
     const float4 lo = dbox[2 * (size_t)i], hi = dbox[2 * (size_t)i + 1];
     // the whole 32-byte record in one scalar-cache round trip, not the link
     // first and the box after it
@@ -183,20 +224,7 @@ __global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox
       if (!(box_d2(p, lo, hi) < best)) {
         if (!leaf) skip = link;
       } else if (leaf) {
-        const uint32_t f = link & ~DL_LEAF;
-        const int32_t* tf = F + 3 * (size_t)f;
-        double A[3], B[3], C[3], c[3];
-        for (int k = 0; k < 3; ++k) {
-          A[k] = V[3 * (int64_t)tf[0] + k];
-          B[k] = V[3 * (int64_t)tf[1] + k];
-          C[k] = V[3 * (int64_t)tf[2] + k];
-        }
-        const double d2 = tri_closest(p, A, B, C, c);
-        if (d2 < best) {  // AABB.cpp set_min: strictly smaller replaces
-          best = d2;
-          bf = (int32_t)f;
-          for (int k = 0; k < 3; ++k) bc[k] = c[k];
-        }
+        if (face_closer(p, V, F, link & ~DL_LEAF, best, bc)) bf = (int32_t)(link & ~DL_LEAF);
       }
     }
     const bool step = leaf || __ballot(skip == i + 1) != 0;
@@ -205,6 +233,91 @@ __global__ __launch_bounds__(256) void k_closest(const float4* __restrict__ dbox
   best_out[pi] = best;
   face_out[pi] = bf;
   for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = bc[k];
+}
+
+// The queries k_closest handed over, one wavefront per query: the greedy
+// bound, then the 64 lanes share the branch and bound through ring_walk,
+// pruning against the closest distance any lane has found (kept in LDS).
+// The answer is the sequential walk's: the greedy face when no face is
+// strictly closer, else the first face in rope order at the minimum
+// distance.  That face's boxes are no farther than the minimum, which no
+// bound here goes below, so it is always reached; ties between lanes go to
+// the smaller rope position.  The grid strides over the list, whose length
+// is read on the device.
+__global__ __launch_bounds__(256) void k_closest_ring(const float4* __restrict__ dbox, uint32_t nslots,
+                                                      const double* __restrict__ V, const int32_t* __restrict__ F,
+                                                      const double* __restrict__ P,
+                                                      const uint32_t* __restrict__ deferred,
+                                                      const uint32_t* __restrict__ ndeferred, double up2,
+                                                      double* __restrict__ best_out,
+                                                      int32_t* __restrict__ face_out, double* __restrict__ c_out) {
+  __shared__ uint2 ring_all[4][kReevalRing];
+  __shared__ unsigned long long bound_all[4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint2* ring = ring_all[wv];
+  const uint32_t n = *ndeferred;
+  for (uint32_t w = blockIdx.x * 4 + wv; w < n; w += gridDim.x * 4) {
+    const int64_t pi = deferred[w];
+    const double p[3] = {P[3 * pi], P[3 * pi + 1], P[3 * pi + 2]};
+    double gbest = up2, gc[3] = {0.0, 0.0, 0.0};
+    int32_t gf = -1;
+    greedy_bound(dbox, nslots, V, F, p, gbest, gf, gc);
+    // distances are >= 0 (or NaN, never stored), so their bit patterns order
+    // as unsigned integers
+    if (lane == 0) bound_all[wv] = (unsigned long long)__double_as_longlong(gbest);
+    // this lane's candidate: the closest face it evaluated, the earlier in
+    // rope order on a tie; none (cpos = ~0) unless strictly closer than up2
+    double cbest = up2, cc[3] = {0.0, 0.0, 0.0};
+    uint32_t cpos = 0xFFFFFFFFu, cf = 0;
+    ring_walk(ring, lane, nslots, [&](uint32_t i, uint32_t& link) {
+      const float4 lo = dbox[2 * (size_t)i], hi = dbox[2 * (size_t)i + 1];
+      link = __float_as_uint(lo.w);
+      const double bound = __longlong_as_double((long long)bound_all[wv]);
+      // boxes at the bound are entered: a face at the minimum distance is
+      // reached whatever bound another lane has set
+      if (box_d2(p, lo, hi) > bound) return true;
+      if (!(link & DL_LEAF)) return false;
+      const uint32_t f = link & ~DL_LEAF;
+      double c[3];
+      const double d2 = face_dist(p, V, F, f, c);
+      if (d2 < cbest || (d2 == cbest && cpos != 0xFFFFFFFFu && i < cpos)) {
+        cbest = d2;
+        cpos = i;
+        cf = f;
+        for (int k = 0; k < 3; ++k) cc[k] = c[k];
+      }
+      if (d2 < bound) atomicMin(&bound_all[wv], (unsigned long long)__double_as_longlong(d2));
+      return true;
+    });
+    // (distance, rope position) minimum over the lanes
+    double rd = cbest;
+    uint32_t rp = cpos;
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double od = __shfl_xor(rd, off, 64);
+      const uint32_t op = __shfl_xor(rp, off, 64);
+      if (od < rd || (od == rd && op < rp)) {
+        rd = od;
+        rp = op;
+      }
+    }
+    if (gf >= 0 && !(rd < gbest)) {  // no face strictly closer than the greedy one
+      if (lane == 0) {
+        best_out[pi] = gbest;
+        face_out[pi] = gf;
+        for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = gc[k];
+      }
+    } else if (rp != 0xFFFFFFFFu) {
+      if (cpos == rp) {
+        best_out[pi] = cbest;
+        face_out[pi] = (int32_t)cf;
+        for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = cc[k];
+      }
+    } else if (lane == 0) {  // nothing closer than up2
+      best_out[pi] = up2;
+      face_out[pi] = -1;
+      for (int k = 0; k < 3; ++k) c_out[3 * pi + k] = 0.0;
+    }
+  }
 }
 
 // pseudonormal_test.cpp:24-141: the normal at the closest point c of face f
@@ -408,7 +521,7 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
                       sign_type == GWN_SIGN_DEFAULT;
   const size_t tb = radix_sort_temp_bytes(np, false);
   const size_t n = (size_t)np;
-  Scratch ws(scratch_bytes({12 * n, 4 * n, 4 * n, tb, 8 * n, 24 * n, 4 * n, want_w ? 8 * n : 8}), st);
+  Scratch ws(scratch_bytes({12 * n, 4 * n, 4 * n, tb, 8 * n, 24 * n, 4 * n, want_w ? 8 * n : 8, 4}), st);
   float* Pf = ws.take_n<float>(3 * n);
   uint32_t* keys = ws.take_n<uint32_t>(n);
   uint32_t* perm = ws.take_n<uint32_t>(n);
@@ -417,15 +530,23 @@ void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_t
   double* cl = ws.take_n<double>(3 * n);
   int32_t* face = ws.take_n<int32_t>(n);
   double* W = want_w ? ws.take_n<double>(n) : nullptr;
+  uint32_t* ndeferred = ws.take_n<uint32_t>(1);
   hipLaunchKernelGGL(k_narrow3, dim3(div_up(3 * np, 256)), dim3(256), 0, st, P, np, Pf);
   check_launch("k_narrow3");
   hilbert_keys(Pf, np, s->bbox, keys, perm, st);
   radix_sort_u32(keys, perm, np, 30, tmp, tb, st);
   {
     ProfileScope prof("closest_point", st);
+    // the sort keys are spent: their space holds the handed-over queries
+    uint32_t* deferred = keys;
+    fill_device<uint32_t>(ndeferred, 1, 0u, st);
     hipLaunchKernelGGL(k_closest, dim3(g), dim3(256), 0, st, s->dbox.p, (uint32_t)s->nslots, s->V64.p, s->F.p, P,
-                       perm, np, up2, best, face, cl);
+                       perm, np, up2, closest_budget(s->nslots), deferred, ndeferred, best, face, cl);
     check_launch("k_closest");
+    const unsigned gr = (unsigned)std::min<int64_t>(div_up(np, 4), 2048);
+    hipLaunchKernelGGL(k_closest_ring, dim3(gr), dim3(256), 0, st, s->dbox.p, (uint32_t)s->nslots, s->V64.p, s->F.p,
+                       P, deferred, ndeferred, up2, best, face, cl);
+    check_launch("k_closest_ring");
   }
   gwn_exact_tree hier;
   if (sign_type == GWN_SIGN_FAST_WINDING_NUMBER) {

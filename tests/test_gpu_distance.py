@@ -135,6 +135,27 @@ def test_signed_distance_edge_cases():
     assert np.allclose(S, So, rtol=1e-15, atol=0) and np.array_equal(I, Io) and np.allclose(C, Co, atol=1e-15)
 
 
+def test_closest_point_deep_interior_queries():
+    """Points deep inside the closed 81,920-triangle surface: their wavefronts
+    outgrow the closest-point walk's record budget and hand their queries to
+    the walk with one wavefront per query (lanes sharing the branch and bound).
+    Face, closest point and distance match the oracle's brute force, with
+    points near the surface and a NaN point in the same batch."""
+    V, F, _ = D.soup_workload(nq=10)
+    rng = np.random.default_rng(17)
+    d = rng.normal(size=(3000, 3))
+    inner = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0, 0.3, (3000, 1))
+    near = D.near_surface_queries(V, F, 60000, 1e-3, rng)
+    P = np.concatenate([inner, near[:30000], [[np.nan, 0.0, 0.0]], near[30000:]])
+    bvh = gwn.FastWindingNumberBVH().build(V, F, 2)
+    for sign in (gwn.SIGN_UNSIGNED, gwn.SIGN_FAST_WINDING_NUMBER):
+        S, I, C = bvh.signed_distance(P, sign)
+        sub = np.concatenate([np.arange(3000), np.arange(3000, P.shape[0], 61), [33000]])
+        So, Io, Co = O.signed_distance(V, F, P[sub], sign)
+        _check(V, F, P[sub], S[sub], I[sub], C[sub], So, Io, Co, sign)
+        assert np.all(S[:3000] < -0.3) if sign == gwn.SIGN_FAST_WINDING_NUMBER else np.all(S[:3000] > 0.3)
+
+
 def test_signed_distance_large_batch():
     """1M points on the 81,920-triangle workload mesh, checked on a sample."""
     V, F, Q = D.soup_workload()
