@@ -825,7 +825,7 @@ def run_exact(no_cpu):
             "nodes": int(tree.nodes().shape[0]), "solid_angles_per_query": round(tris_per_q, 1),
             "max_abs_dw_vs_bruteforce": float(np.abs(Wh - Wb).max()),
             "prebuilt_equals_one_shot": bool(np.array_equal(Wt, Wh))}
-    hier["roofline"] = _fp64_roofline(tris_per_q * nq, kms2, "k_wn_tree_wave")
+    hier["roofline"] = _fp64_roofline(tris_per_q * nq, kms2, "k_wn_tree_lanes")
     out = {"workload": "20,480 tris, 100k queries, FP64", "hierarchy": hier, "bruteforce": brute,
            "queries_per_s": hier["queries_per_s"], "kernel_ms": hier["kernel_ms"]}
     sub = np.arange(0, nq, 50)

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gwn_internal.h"
+#include "ring_walk.h"
 #include "wntree.h"
 
 namespace gwn {
@@ -227,28 +228,33 @@ void exact_device(const double* V, int64_t nv, const int32_t* F, int64_t nf, con
 
 
 // ---------------------------------------------------------------------------
-// The reference's hierarchy (wntree.cpp): per query, a walk over the
-// pre-order records.  Inside an internal node's closed box the walk steps to
-// its first child; otherwise it sums the node's triangle list (inside a leaf:
-// its faces; outside the box: its cap, or its faces when the cap is not
-// smaller) and jumps past the subtree.  Each term is solid_angle(...)
-// (solid_angle.cpp: atan2 over 2 pi) added to one running FP64 sum in walk
-// order (the reference adds a node's children's sums; the two orders agree to
-// rounding).  One query per lane, queries in Hilbert order so that a
-// wavefront's lanes mostly read the same records and triangles.
-__global__ __launch_bounds__(256) void k_wn_tree(const WNRecord* __restrict__ rec, uint32_t nrec,
-                                                  const double* __restrict__ tri, const double* __restrict__ O,
-                                                  const uint32_t* __restrict__ perm, int64_t no,
-                                                  double* __restrict__ W) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= no) return;
-  const int64_t q = perm ? (int64_t)perm[t] : t;
+// The reference's hierarchy (wntree.cpp), one wavefront per query (queries in
+// Hilbert order when perm is given).  The wavefront walks the query's
+// pre-order records: inside an internal node's closed box it steps to the
+// first child; otherwise it sums the node's triangle list (inside a leaf: its
+// faces; outside the box: its cap, or its faces when the cap is not smaller)
+// and jumps past the subtree.  The 64 lanes share each list, lane l taking
+// its triangles l, l+64, ...; each term is solid_angle(...) (solid_angle.cpp:
+// atan2 over 2 pi), and the lanes' FP64 partial sums are added in a fixed
+// order (wave_sum).  The reference adds a node's children's sums; the orders
+// agree to rounding.  A query's result does not depend on the batch it is in.
+// One query per lane, wave-coherent or not, ran 6x (100k queries) to 100x
+// (1,000 queries) longer: the lists run to dozens of triangles, which a lane
+// summed alone while the wavefront's other lanes waited.
+__global__ __launch_bounds__(256) void k_wn_tree_lanes(const WNRecord* __restrict__ rec, uint32_t nrec,
+                                                        const double* __restrict__ tri,
+                                                        const double* __restrict__ O,
+                                                        const uint32_t* __restrict__ perm, int64_t no,
+                                                        double* __restrict__ W) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)xcd_block() * 4 + (threadIdx.x >> 6);
+  if (w >= no) return;
+  const int64_t q = perm ? (int64_t)perm[w] : w;
   const double px = O[3 * q], py = O[3 * q + 1], pz = O[3 * q + 2];
-  double sum = 0.0;
+  double part = 0.0;
   uint32_t i = 0;
   while (i < nrec) {
     const WNRecord& r = rec[i];
-    // WindingNumberAABB::inside (WindingNumberAABB.h:238-253): closed box
     const bool inside = !(px < r.lo[0] || px > r.hi[0] || py < r.lo[1] || py > r.hi[1] || pz < r.lo[2] ||
                           pz > r.hi[2]);
     if (inside && !r.leaf) {
@@ -257,62 +263,12 @@ __global__ __launch_bounds__(256) void k_wn_tree(const WNRecord* __restrict__ re
     }
     const uint32_t start = inside ? r.in_start : r.out_start;
     const uint32_t count = inside ? r.in_count : r.out_count;
-    for (uint32_t k = 0; k < count; ++k)
-      sum += vos_atan2(tri + 9 * (size_t)(start + k), px, py, pz) / (2.0 * EX_PI);
+    for (uint32_t k = (uint32_t)lane; k < count; k += 64)
+      part += vos_atan2(tri + 9 * (size_t)(start + k), px, py, pz) / (2.0 * EX_PI);
     i = r.link;
   }
-  W[q] = sum;
-}
-
-// Wave-coherent form of k_wn_tree for Hilbert-sorted queries (the scheme of
-// the soup traversal): the wavefront walks one shared record index i, and a
-// lane takes part in record i when it is the record its own walk needs next
-// (skip == i; every other lane needs a record at or beyond the subtree's end,
-// the subtrees of the rope being nested intervals).  A lane inside a non-leaf
-// box descends (i + 1); every other taking part sums the record's out list
-// (the cap of the node's faces) or, inside a leaf, its in list (the faces),
-// and jumps to link.  Records and triangles are the same for every lane of a
-// wavefront and come through the scalar cache; each lane adds the same terms
-// in the same order as in k_wn_tree, so the results are bitwise the same.
-__global__ __launch_bounds__(256) void k_wn_tree_wave(const WNRecord* __restrict__ rec, uint32_t nrec,
-                                                       const double* __restrict__ tri,
-                                                       const double* __restrict__ O,
-                                                       const uint32_t* __restrict__ perm, int64_t no,
-                                                       double* __restrict__ W) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= no) return;
-  const int64_t q = perm[t];
-  const double px = O[3 * q], py = O[3 * q + 1], pz = O[3 * q + 2];
-  double sum = 0.0;
-  uint32_t i = 0, skip = 0;
-  while (i < nrec) {
-    const WNRecord& r = rec[i];
-    const bool part = skip == i;
-    const bool inside = !(px < r.lo[0] || px > r.hi[0] || py < r.lo[1] || py > r.hi[1] || pz < r.lo[2] ||
-                          pz > r.hi[2]);
-    const bool leaf = r.leaf != 0;
-    const bool out_sum = part && !inside;
-    const bool in_sum = part && inside && leaf;
-    if (part) skip = (inside && !leaf) ? i + 1 : r.link;
-    // the lists are wave-uniform: one scalar fetch per triangle for all lanes
-    if (__ballot(out_sum) != 0) {
-      const uint32_t s0 = __builtin_amdgcn_readfirstlane(r.out_start), n0 = __builtin_amdgcn_readfirstlane(r.out_count);
-      for (uint32_t k = 0; k < n0; ++k) {
-        const double* tk = tri + 9 * (size_t)(s0 + k);
-        if (out_sum) sum += vos_atan2(tk, px, py, pz) / (2.0 * EX_PI);
-      }
-    }
-    if (__ballot(in_sum) != 0) {
-      const uint32_t s1 = __builtin_amdgcn_readfirstlane(r.in_start), n1 = __builtin_amdgcn_readfirstlane(r.in_count);
-      for (uint32_t k = 0; k < n1; ++k) {
-        const double* tk = tri + 9 * (size_t)(s1 + k);
-        if (in_sum) sum += vos_atan2(tk, px, py, pz) / (2.0 * EX_PI);
-      }
-    }
-    const bool step = __ballot(skip == i + 1) != 0;
-    i = __builtin_amdgcn_readfirstlane(step ? i + 1 : r.link);
-  }
-  W[q] = sum;
+  const double sum = wave_sum(part);
+  if (lane == 0) W[q] = sum;
 }
 
 __global__ void k_wn_narrow(const double* __restrict__ O, int64_t n3, float* __restrict__ Of) {
@@ -338,9 +294,9 @@ void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, co
     fill_device<double>(W, no, 0.0, st);
     return;
   }
-  const unsigned g = div_up(no, 256);
   uint32_t* perm = nullptr;
   // small batches walk in input order; larger ones along the Hilbert curve
+  // (neighbouring wavefronts then read the same records and triangles)
   const bool sorted = no >= 4096;
   const size_t tb = sorted ? radix_sort_temp_bytes(no, false) : 0;
   Scratch ws(sorted ? scratch_bytes({12 * (size_t)no, 4 * (size_t)no, 4 * (size_t)no, tb}) : 1, st);
@@ -355,15 +311,9 @@ void wntree_query_device(const uint8_t* rec, int64_t nrec, const double* tri, co
     radix_sort_u32(keys, perm, no, 30, tmp, tb, st);
   }
   ProfileScope prof("exact_tree", st);
-  if (sorted) {
-    hipLaunchKernelGGL(k_wn_tree_wave, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec),
-                       (uint32_t)nrec, tri, O, perm, no, W);
-    check_launch("k_wn_tree_wave");
-  } else {
-    hipLaunchKernelGGL(k_wn_tree, dim3(g), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec),
-                       (uint32_t)nrec, tri, O, perm, no, W);
-    check_launch("k_wn_tree");
-  }
+  hipLaunchKernelGGL(k_wn_tree_lanes, dim3(div_up(no, 4)), dim3(256), 0, st, reinterpret_cast<const WNRecord*>(rec),
+                     (uint32_t)nrec, tri, O, perm, no, W);
+  check_launch("k_wn_tree_lanes");
 }
 
 }  // namespace gwn

@@ -85,4 +85,14 @@ __device__ __forceinline__ void ring_walk(uint2* __restrict__ ring, int lane, ui
   }
 }
 
+// sum of the lanes' partial sums in a fixed order; every lane gets the same value
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = __dadd_rn(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off >= 1; off >>= 1) v = __fadd_rn(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 }  // namespace gwn

@@ -756,15 +756,6 @@ __device__ __forceinline__ bool record_term64(const double* __restrict__ s64, co
   return true;
 }
 
-// sum of the lanes' partial sums in a fixed order; every lane gets the same value
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v = __dadd_rn(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_sum(float v) {
-  for (int off = 32; off >= 1; off >>= 1) v = __fadd_rn(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 // FP64 ring walk of query q: W = sum / 4 pi, the same on every lane
 template <int ORDER>

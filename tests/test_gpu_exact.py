@@ -45,19 +45,20 @@ def test_exact_hierarchy_matches_oracle_tree():
     assert np.abs(W - gwn.winding_number_bruteforce(V, F, Q)).max() < 1e-12
 
 
-def test_exact_hierarchy_wave_walk_equals_lane_walk():
-    """Batches of 4,096+ queries walk the hierarchy wave-coherently over
-    Hilbert-sorted queries, smaller ones one lane per query in input order.
-    Each lane adds the same terms in the same order either way, so a query's
-    result does not depend on the walk: bit for bit."""
+def test_exact_hierarchy_result_independent_of_batch():
+    """One wavefront per query: batches of 4,096+ queries take them in
+    Hilbert order, smaller ones in input order, and a query's lanes add the
+    same terms in the same order either way, so its result does not depend on
+    the batch it is in: bit for bit."""
     V, F, Q = D.exact_workload()
     rng = np.random.default_rng(7)
     Q = np.concatenate([Q[:12000], D.near_surface_queries(V, F, 4000, 1e-4, rng), V[:50]])
     tree = gwn.WindingNumberAABB(V, F)
-    W = tree.winding_number(Q)  # 16,050 queries: the wave-coherent walk
+    W = tree.winding_number(Q)  # 16,050 queries, sorted
     Wl = np.concatenate([tree.winding_number(Q[k:k + 4000]) for k in range(0, Q.shape[0], 4000)])
     assert Q.shape[0] % 4000 != 0 and Q[16000:].shape[0] < 4096
     assert np.array_equal(W, Wl)
+    assert np.array_equal(tree.winding_number(Q[::-1][:3000]), W[::-1][:3000])
 
 
 def test_exact_hierarchy_1310720_tris_vs_brute_force():
