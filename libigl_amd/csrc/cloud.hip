@@ -713,9 +713,11 @@ __global__ __launch_bounds__(256) void k_cloud_traverse(const float4* __restrict
                                                         uint32_t nrec, const float4* __restrict__ poly,
                                                         const float* __restrict__ Ps, const float* __restrict__ NAs,
                                                         int64_t npts, bool root_leaf, const float* __restrict__ Q,
-                                                        const uint32_t* __restrict__ perm, int64_t nq, double beta,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ order, int64_t nq, double beta,
                                                         float* __restrict__ WN) {
-  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  // order (when given): the logical block this block takes (costliest first)
+  const int64_t t = (int64_t)(order ? order[xcd_block()] : xcd_block()) * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = perm ? (int64_t)perm[t] : t;
   const float qx = Q[3 * qi], qy = Q[3 * qi + 1], qz = Q[3 * qi + 2];
@@ -927,6 +929,44 @@ static void cell_moments(gwn_cloud* c, bool fold_cm, const SplitCell* all, const
 
 // Rope records over the non-empty non-root cells (split cells given level by
 // level in `all`); a leaf record points at leaf_start[cell] in Ps/NAs.
+// Walk cost at each cell centre of the cost grid: the records a query there
+// visits at beta 2 (the near/far decisions of k_cloud_traverse, without the
+// sums) plus the points of the leaves it sums, as min(255, 15 log2(1 + n)).
+__global__ __launch_bounds__(256) void k_cloud_cost_grid(const float4* __restrict__ head,
+                                                         const uint2* __restrict__ meta, uint32_t nrec, float lx,
+                                                         float ly, float lz, float hx, float hy, float hz,
+                                                         uint8_t* __restrict__ grid) {
+  constexpr int G = kCostGrid;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G * G * G) return;
+  const float qx = lx + (c % G + 0.5f) * hx, qy = ly + ((c / G) % G + 0.5f) * hy, qz = lz + (c / (G * G) + 0.5f) * hz;
+  uint32_t i = 0, n = 0;
+  while (i < nrec) {
+    const uint2 m = meta[i];
+    ++n;
+    if (m.x & REC_LEAF) {
+      n += m.y;
+      ++i;
+      continue;
+    }
+    const float4 h = head[i];
+    const float x = h.x - qx, y = h.y - qy, z = h.z - qz;
+    i = sqrtf(x * x + y * y + z * z) > 2.0f * h.w ? m.x : i + 1;
+  }
+  grid[c] = (uint8_t)fminf(255.0f, 15.0f * log2f(1.0f + (float)n));
+}
+
+static void build_cloud_cost_grid(gwn_cloud* c, hipStream_t st) {
+  constexpr int G = kCostGrid;
+  if (c->root_leaf || c->nrec == 0) return;
+  c->cost_grid.alloc((size_t)G * G * G);
+  const float* bb = c->bbox;
+  hipLaunchKernelGGL(k_cloud_cost_grid, dim3(div_up(G * G * G, 256)), dim3(256), 0, st, c->head.p, c->meta.p,
+                     (uint32_t)c->nrec, bb[0], bb[1], bb[2], (bb[3] - bb[0]) / G, (bb[4] - bb[1]) / G,
+                     (bb[5] - bb[2]) / G, c->cost_grid.p);
+  check_launch("k_cloud_cost_grid");
+}
+
 static void rope_records(gwn_cloud* c, const SplitCell* all, const std::vector<uint32_t>& lvl_off,
                          const int64_t* leaf_start, hipStream_t st) {
   const int64_t ncells = c->ncells;
@@ -1133,6 +1173,8 @@ void cloud_build_device(gwn_cloud* c, hipStream_t st, bool moments) {
   }
   cell_moments(c, true, all.p, lvl_off, st);
   rope_records(c, all.p, lvl_off, c->cstart.p, st);
+  build_cloud_cost_grid(c, st);
+  GWN_HIP(hipStreamSynchronize(st));
 }
 
 // Precompute and rope over a caller's octree: CH (ncells x 8, -1 for
@@ -1254,6 +1296,8 @@ void cloud_build_from_octree_device(gwn_cloud* c, const int32_t* hCH, int64_t nc
   DevBuf<int64_t> dls(ncells);
   GWN_HIP(hipMemcpyAsync(dls.p, lstart.data(), 8 * (size_t)ncells, hipMemcpyHostToDevice, st));
   rope_records(c, all.p, lvl_off, dls.p, st);
+  build_cloud_cost_grid(c, st);
+  GWN_HIP(hipStreamSynchronize(st));
 }
 
 void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double beta, float* WN, hipStream_t st) {
@@ -1278,19 +1322,26 @@ void cloud_query_device(const gwn_cloud* c, const float* Q, int64_t nq, double b
   hilbert_keys(Q, nq, c->bbox, keys, perm, st);
   radix_sort_u32(keys, perm, nq, 30, tmp, tb, st);
   const bool rl = c->root_leaf;
+  // the sort keys are spent: their space holds the block order (g entries)
+  // and the block costs (g bytes)
+  uint32_t* order = nullptr;
+  if (!rl && c->cost_grid.p) {
+    order = keys;
+    block_order(c->cost_grid.p, c->bbox, Q, perm, nq, order, (uint8_t*)(order + g), st);
+  }
   const float* Ps = c->Ps.p + (rl ? 3 * c->root_start : 0);
   const float* NAs = c->NAs.p + (rl ? 3 * c->root_start : 0);
   const int64_t npts = rl ? c->root_count : 0;
   ProfileScope prof("cloud_traverse", st);
   if (c->nterms == 3)
     hipLaunchKernelGGL(k_cloud_traverse<3>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, order, nq, beta, WN);
   else if (c->nterms == 12)
     hipLaunchKernelGGL(k_cloud_traverse<12>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, order, nq, beta, WN);
   else
     hipLaunchKernelGGL(k_cloud_traverse<39>, dim3(g), dim3(256), 0, st, c->head.p, c->meta.p, (uint32_t)c->nrec,
-                       c->poly.p, Ps, NAs, npts, rl, Q, perm, nq, beta, WN);
+                       c->poly.p, Ps, NAs, npts, rl, Q, perm, order, nq, beta, WN);
   check_launch("k_cloud_traverse");
 }
 

@@ -219,6 +219,12 @@ void read_triangle_mesh(const char* path, std::vector<double>& V, std::vector<in
 void morton_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
 // 30-bit Hilbert keys (10 bits per axis over bbox) + identity values: query order for the tree walks
 void hilbert_keys(const float* Q, int64_t nq, const float* bbox, uint32_t* keys, uint32_t* vals, hipStream_t st);
+// Costliest-first order of the blocks of 256 sorted queries (perm) within
+// each XCD's range of logical blocks, block costs read from a kCostGrid^3
+// grid of log-scale walk costs over bbox: order ceil(nq/256) entries and
+// cost ceil(nq/256) bytes of workspace.
+void block_order(const uint8_t* grid, const float* bbox, const float* Q, const uint32_t* perm, int64_t nq,
+                 uint32_t* order, uint8_t* cost, hipStream_t st);
 void signed_distance_device(gwn_soup* s, const double* P, int64_t np, int sign_type, double lower, double upper,
                             double* S, int32_t* I, double* C, double* N, hipStream_t st);
 // the normals of the pseudonormal sign test, in the reference's summation
@@ -363,4 +369,8 @@ struct gwn_cloud {
   gwn::DevBuf<float4> head;            // nrec: CM.xyz, R of each non-empty non-root cell (rope order)
   gwn::DevBuf<uint2> meta;             // nrec: (skip, cell) or (LEAF|first point, count)
   gwn::DevBuf<float4> poly;            // nrec x 5: the cell's expansion as one polynomial (internal records)
+  // records visited (and leaf points summed) by a query at each cell centre
+  // of a kCostGrid^3 grid over bbox, at beta 2, as a log-scale byte: the
+  // walk's block order (empty when the root is a leaf)
+  gwn::DevBuf<uint8_t> cost_grid;
 };

@@ -1245,15 +1245,14 @@ __global__ __launch_bounds__(256) void k_block_order(const uint8_t* __restrict__
 }
 
 // order (nb = ceil(nq / 256) entries) and cost (nb bytes) are workspace
-static void block_order(const gwn_soup* s, const float* Q, const uint32_t* perm, int64_t nq, uint32_t* order,
-                        uint8_t* cost, hipStream_t st) {
+void block_order(const uint8_t* grid, const float* bb, const float* Q, const uint32_t* perm, int64_t nq,
+                 uint32_t* order, uint8_t* cost, hipStream_t st) {
   constexpr int G = kCostGrid;
   const uint32_t nb = (uint32_t)div_up(nq, 256);
-  const float* bb = s->bbox;
   const float ex = std::max(bb[3] - bb[0], 1e-30f), ey = std::max(bb[4] - bb[1], 1e-30f),
               ez = std::max(bb[5] - bb[2], 1e-30f);
   hipLaunchKernelGGL(k_block_cost, dim3(nb), dim3(256), 0, st, Q, perm, nq, bb[0], bb[1], bb[2], G / ex, G / ey,
-                     G / ez, s->cost_grid.p, cost);
+                     G / ez, grid, cost);
   check_launch("k_block_cost");
   hipLaunchKernelGGL(k_block_order, dim3(8), dim3(256), 0, st, cost, nb, order);
   check_launch("k_block_order");
@@ -1436,7 +1435,7 @@ void soup_query_device_ws(const gwn_soup* s, const float* Q, int64_t nq, float a
   uint32_t* border = keys;
   {
     ProfileScope prof("block_order", st, prof_on);
-    block_order(s, Q, perm, nq, border, (uint8_t*)(border + g), st);
+    block_order(s->cost_grid.p, s->bbox, Q, perm, nq, border, (uint8_t*)(border + g), st);
   }
 
   const float acc2 = accuracy_scale * accuracy_scale;
